@@ -1,0 +1,148 @@
+/*
+ * sbh_hip.h - C ABI of the MI355X (gfx950) Stochastic Barnes-Hut library,
+ * libsbh_hip.so.
+ *
+ * This is the boundary under the pybind11 plugin module `sbh_gpu`
+ * (src/sbh_gpu.cu:566-661 in the reference) and under any other host
+ * language: plain pointers and sizes, integer status codes, no exceptions
+ * across the boundary, no framework types.
+ *
+ * Conventions
+ *  - Every function returns SBH_OK (0) or an SBH_ERR_* code; the message of
+ *    the last failure on the calling thread is sbh_last_error().
+ *  - Host pointers are borrowed for the duration of the call only.  With
+ *    SBH_EVAL_DEVICE_POINTERS the query / result / count pointers are device
+ *    pointers instead and the call neither copies nor allocates them.
+ *  - A tree owns its device memory and one HIP stream; calls on one tree are
+ *    serialised by the caller (thread-compatible, not thread-safe).
+ *  - Queries are a column-major d x q matrix: x[0..q), y[0..q), z[0..q)
+ *    (the reference's Eigen column-major query copy, src/sbh_gpu.cu:217).
+ */
+#ifndef SBH_HIP_H
+#define SBH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sbh_tree sbh_tree;
+
+/* Status codes. */
+#define SBH_OK 0
+#define SBH_ERR_INVALID 1   /* bad argument (shape, NaN, empty, unsupported mode) */
+#define SBH_ERR_HIP 2       /* HIP runtime failure */
+#define SBH_ERR_NOMEM 3     /* device allocation failed */
+#define SBH_ERR_NODEVICE 4  /* no usable GPU */
+#define SBH_ERR_BUDGET 5    /* serving admission refused: HBM budget exceeded */
+#define SBH_ERR_INTERNAL 6
+
+/* Kernel functions: include/kernel.h:24-28 (KernelFuncID). */
+#define SBH_GRAVITY 0
+#define SBH_WINDING_NUMBER 1
+#define SBH_SMOOTH_DIST 2
+
+/* Evaluation methods. */
+#define SBH_BRUTE_FORCE 0 /* src/sbh_gpu.cu:212-322  evalBruteForce_*            */
+#define SBH_BARNES_HUT 1  /* src/sbh_gpu.cu:324-434  evalBarnesHut_*             */
+#define SBH_STOCHASTIC 2  /* src/sbh_gpu.cu:436-550  multiLevelPrefixControlVariateSampler_* */
+
+/* sbh_tree_create flags. */
+#define SBH_TREE_MOMENTS_F64 1 /* node moments aggregated in float64 (default: the
+                                  reference's fp32 order, include/tree3d.h:255-327) */
+
+/* sbh_eval flags. */
+#define SBH_EVAL_DEVICE_POINTERS 1 /* queries / results / counts are device pointers */
+#define SBH_EVAL_SORT_QUERIES 2    /* evaluate in spatial (Morton) order; results are
+                                      returned in input order.  Barnes-Hut with
+                                      vote_group 1 gives identical values either way */
+
+typedef struct sbh_eval_params {
+  float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */
+  int32_t samples_per_subdomain; /* stochastic: samples per root child (>= 1) */
+  uint64_t seed;                 /* stochastic: Philox key */
+  int32_t vote_group;            /* Barnes-Hut: 1 = per query (src/sbh.cpp walk),
+                                    32 = 32-query warp vote (include/barnes_hut_gpu.h:34),
+                                    64 = wavefront vote */
+  int32_t flags;                 /* SBH_EVAL_* */
+} sbh_eval_params;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* sbh_last_error(void);
+
+/* Library ABI version (major * 100 + minor). */
+int32_t sbh_version(void);
+
+/* Number of visible GPUs. */
+int sbh_device_count(int32_t* count);
+
+/*
+ * Build a tree on the current GPU.  Replaces the CPU build plus device
+ * mirror of the reference: PyOctree / PyQuadtree (include/py_tree.h:30-74)
+ * followed by PyOctreeGPU / PyQuadtreeGPU (include/py_tree_gpu.h:39-80).
+ *   dim         2 or 3
+ *   split_log   k in 1..3: 2^k splits per axis (the class suffix of
+ *               OctreeK / QuadtreeK, src/sbh.cpp:171-177)
+ *   leaf_size   LEAF_SIZE template argument (the reference binds 1)
+ *   max_depth   100 in the reference (include/py_tree.h:44,72)
+ *   points      n x dim row-major; normals n x dim row-major or NULL (zeros)
+ *   masses      n; finite and > 0
+ *   min_corner / max_corner  dim floats; the root cell
+ *   alpha       smooth-distance exponent (include/py_tree.h:39,67)
+ *   build_ms    optional: wall time of the build
+ */
+int sbh_tree_create(int32_t dim, int32_t split_log, int32_t leaf_size, int32_t max_depth,
+                    int64_t n, const float* points, const float* normals, const float* masses,
+                    const float* min_corner, const float* max_corner, float alpha, int32_t flags,
+                    sbh_tree** out, float* build_ms);
+
+int sbh_tree_destroy(sbh_tree* tree);
+
+/* Sizes: points, nodes, deepest level, device bytes held.  Any output may be NULL. */
+int sbh_tree_info(const sbh_tree* tree, int64_t* num_points, int64_t* num_nodes, int32_t* depth,
+                  int64_t* device_bytes);
+
+/* Leaf order: out[i] = input row of the i-th point in tree order (host, n values).
+ * The reference exposes this implicitly by permuting leaf_data in place
+ * (include/tree3d.h:281-289). */
+int sbh_tree_leaf_order(const sbh_tree* tree, int64_t* out);
+
+/*
+ * Node arrays in breadth-first node order (include/tree3d.h:212-225 fields).
+ * Host outputs, each NULL or sized num_nodes (x dim for vectors, x 8 for
+ * moments64 = {cx, cy, cz, mass, nx, ny, nz, 0} in float64).  first_child is
+ * 0xFFFFFFFF for leaves; rope is the depth-first successor used by the walk.
+ */
+int sbh_tree_nodes(const sbh_tree* tree, uint32_t* start, uint32_t* end, uint32_t* first_child,
+                   uint32_t* num_children, int32_t* depth, uint32_t* parent, uint32_t* rope,
+                   float* min_corner, float* max_corner, float* diameter, float* centroid,
+                   float* mass, float* normal, double* moments64);
+
+/*
+ * Evaluate q queries.  results: q floats; counts (optional): kernel
+ * evaluations per query (num_contribs of include/barnes_hut.h:29-50 and
+ * include/tree_sampler.h:110-171); kernel_ms (optional): device time of the
+ * evaluation kernels, hipEvent-timed on the tree's stream as the reference
+ * times its launches (src/sbh_gpu.cu:227-241).
+ */
+int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_params* params,
+             const float* queries, int64_t q, float* results, int32_t* counts, float* kernel_ms);
+
+/* float64 all-pairs sums of the fp32 kernel formulas (error reference). */
+int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q,
+                 double* results, int32_t flags, float* kernel_ms);
+
+/* The permutation SBH_EVAL_SORT_QUERIES applies: order[i] = query evaluated in
+ * slot i (host or device pointers per flags). */
+int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* order,
+                    int32_t flags);
+
+/* Stream the tree's work is issued on (hipStream_t). */
+int sbh_tree_stream(sbh_tree* tree, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBH_HIP_H */

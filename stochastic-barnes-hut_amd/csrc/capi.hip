@@ -1,0 +1,387 @@
+// C ABI of libsbh_hip.so (declared in include/sbh_hip.h).  Validates
+// arguments, owns per-tree device scratch, times the evaluation kernels with
+// hipEvents on the tree's stream, and turns every C++ exception into a status
+// code plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sbh_hip.h"
+#include "sbh_eval.hpp"
+#include "sbh_runtime.hpp"
+
+struct sbh_tree {
+  sbh::DeviceTree t;
+  // evaluation scratch, grown on demand
+  sbh::DevBuf<float> q;
+  sbh::DevBuf<float> out;
+  sbh::DevBuf<int32_t> counts;
+  sbh::DevBuf<double> partial;
+  sbh::DevBuf<double> out64;
+  sbh::DevBuf<uint32_t> order;
+  sbh::DevBuf<uint32_t> sort_scratch;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return SBH_OK;
+  } catch (const sbh::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return SBH_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SBH_ERR_INTERNAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return SBH_ERR_INTERNAL;
+  }
+}
+
+void require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    throw sbh::Error(SBH_ERR_NODEVICE, "no GPU visible to the HIP runtime");
+}
+
+template <typename T>
+void ensure(sbh::DevBuf<T>& b, size_t n) {
+  if (b.size() < n) b.alloc(n);
+}
+
+bool all_finite(const float* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sbh_last_error(void) { return g_last_error.c_str(); }
+
+int32_t sbh_version(void) { return 100; }
+
+int sbh_device_count(int32_t* count) {
+  return guarded([&] {
+    SBH_REQUIRE(count, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+  });
+}
+
+int sbh_tree_create(int32_t dim, int32_t split_log, int32_t leaf_size, int32_t max_depth,
+                    int64_t n, const float* points, const float* normals, const float* masses,
+                    const float* min_corner, const float* max_corner, float alpha, int32_t flags,
+                    sbh_tree** out, float* build_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    SBH_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    SBH_REQUIRE(split_log >= 1 && split_log <= 3, "split_log must be 1, 2 or 3");
+    SBH_REQUIRE(leaf_size >= 1, "leaf_size must be >= 1");
+    SBH_REQUIRE(max_depth >= 0 && max_depth <= 1000, "max_depth must be in [0, 1000]");
+    SBH_REQUIRE(n >= 1, "a tree needs at least one point");
+    SBH_REQUIRE(n < (int64_t)0x7FFFFFFF, "too many points (limit 2^31 - 1)");
+    SBH_REQUIRE(points && masses && min_corner && max_corner, "NULL input array");
+    SBH_REQUIRE(all_finite(points, (size_t)n * dim), "points must be finite");
+    SBH_REQUIRE(!normals || all_finite(normals, (size_t)n * dim), "normals must be finite");
+    for (int64_t i = 0; i < n; ++i)
+      SBH_REQUIRE(std::isfinite(masses[i]) && masses[i] > 0.0f,
+                  "masses must be finite and > 0 (node centroids divide by the mass sum)");
+    for (int a = 0; a < dim; ++a)
+      SBH_REQUIRE(std::isfinite(min_corner[a]) && std::isfinite(max_corner[a]) &&
+                      min_corner[a] < max_corner[a],
+                  "min_corner must be finite and below max_corner on every axis");
+    SBH_REQUIRE(std::isfinite(alpha), "alpha must be finite");
+    require_device();
+    sbh_tree* h = new sbh_tree();
+    try {
+      sbh::DeviceTree& t = h->t;
+      t.dim = dim;
+      t.split_log = split_log;
+      t.leaf_size = leaf_size;
+      t.max_depth = max_depth;
+      t.moments64 = (flags & SBH_TREE_MOMENTS_F64) ? 1 : 0;
+      t.n = (uint32_t)n;
+      t.alpha = alpha;
+      for (int a = 0; a < dim; ++a) {
+        t.lo[a] = min_corner[a];
+        t.hi[a] = max_corner[a];
+      }
+      SBH_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+      SBH_HIP(hipEventCreate(&h->ev0));
+      SBH_HIP(hipEventCreate(&h->ev1));
+      sbh::build_device_tree(t, points, normals, masses, build_ms);
+    } catch (...) {
+      sbh_tree_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int sbh_tree_destroy(sbh_tree* tree) {
+  return guarded([&] {
+    if (!tree) return;
+    if (tree->t.stream) (void)hipStreamSynchronize(tree->t.stream);
+    if (tree->ev0) (void)hipEventDestroy(tree->ev0);
+    if (tree->ev1) (void)hipEventDestroy(tree->ev1);
+    hipStream_t s = tree->t.stream;
+    delete tree;
+    if (s) (void)hipStreamDestroy(s);
+  });
+}
+
+int sbh_tree_info(const sbh_tree* tree, int64_t* num_points, int64_t* num_nodes, int32_t* depth,
+                  int64_t* device_bytes) {
+  return guarded([&] {
+    SBH_REQUIRE(tree, "tree is NULL");
+    if (num_points) *num_points = tree->t.n;
+    if (num_nodes) *num_nodes = tree->t.num_nodes;
+    if (depth) *depth = tree->t.depth;
+    if (device_bytes) *device_bytes = (int64_t)tree->t.device_bytes();
+  });
+}
+
+int sbh_tree_stream(sbh_tree* tree, void** stream) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && stream, "NULL argument");
+    *stream = (void*)tree->t.stream;
+  });
+}
+
+int sbh_tree_leaf_order(const sbh_tree* tree, int64_t* out) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && out, "NULL argument");
+    const sbh::DeviceTree& t = tree->t;
+    std::vector<uint32_t> p(t.n);
+    SBH_HIP(hipMemcpyAsync(p.data(), t.perm.get(), t.n * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, t.stream));
+    SBH_HIP(hipStreamSynchronize(t.stream));
+    for (uint32_t i = 0; i < t.n; ++i) out[i] = p[i];
+  });
+}
+
+int sbh_tree_nodes(const sbh_tree* tree, uint32_t* start, uint32_t* end, uint32_t* first_child,
+                   uint32_t* num_children, int32_t* depth, uint32_t* parent, uint32_t* rope,
+                   float* min_corner, float* max_corner, float* diameter, float* centroid,
+                   float* mass, float* normal, double* moments64) {
+  return guarded([&] {
+    SBH_REQUIRE(tree, "tree is NULL");
+    const sbh::DeviceTree& t = tree->t;
+    const size_t nn = t.num_nodes;
+    const int d = t.dim;
+    hipStream_t s = t.stream;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+      if (dst) SBH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    };
+    cp(start, t.n_start.get(), nn * 4);
+    cp(end, t.n_end.get(), nn * 4);
+    cp(first_child, t.n_first.get(), nn * 4);
+    cp(num_children, t.n_nchild.get(), nn * 4);
+    cp(depth, t.n_depth.get(), nn * 4);
+    cp(parent, t.n_parent.get(), nn * 4);
+    cp(rope, t.n_next.get(), nn * 4);
+    cp(diameter, t.n_diam.get(), nn * 4);
+    cp(moments64, t.n_m64.get(), nn * 8 * sizeof(double));
+    std::vector<float4> lo, hi, c, nr;
+    if (min_corner) {
+      lo.resize(nn);
+      cp(lo.data(), t.n_lo.get(), nn * sizeof(float4));
+    }
+    if (max_corner) {
+      hi.resize(nn);
+      cp(hi.data(), t.n_hi.get(), nn * sizeof(float4));
+    }
+    if (centroid || mass) {
+      c.resize(nn);
+      cp(c.data(), t.nodeC.get(), nn * sizeof(float4));
+    }
+    if (normal) {
+      nr.resize(nn);
+      cp(nr.data(), t.nodeN.get(), nn * sizeof(float4));
+    }
+    SBH_HIP(hipStreamSynchronize(s));
+    auto put = [&](float* dst, const std::vector<float4>& v) {
+      if (!dst) return;
+      for (size_t i = 0; i < nn; ++i) {
+        dst[i * d] = v[i].x;
+        dst[i * d + 1] = v[i].y;
+        if (d == 3) dst[i * d + 2] = v[i].z;
+      }
+    };
+    put(min_corner, lo);
+    put(max_corner, hi);
+    put(centroid, c);
+    put(normal, nr);
+    if (mass)
+      for (size_t i = 0; i < nn; ++i) mass[i] = c[i].w;
+  });
+}
+
+int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* order,
+                    int32_t flags) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && queries && order, "NULL argument");
+    SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
+    if (q == 0) return;
+    sbh::DeviceTree& t = tree->t;
+    hipStream_t s = t.stream;
+    const bool dev = flags & SBH_EVAL_DEVICE_POINTERS;
+    const size_t nq = (size_t)q;
+    const float* dq = queries;
+    if (!dev) {
+      ensure(tree->q, nq * t.dim);
+      SBH_HIP(hipMemcpyAsync(tree->q.get(), queries, nq * t.dim * 4, hipMemcpyHostToDevice, s));
+      dq = tree->q.get();
+    }
+    ensure(tree->sort_scratch, sbh::sort_scratch_u32((uint32_t)nq));
+    uint32_t* dorder = order;
+    if (!dev) {
+      ensure(tree->order, nq);
+      dorder = tree->order.get();
+    }
+    sbh::query_order(t, dq, (uint32_t)nq, dorder, tree->sort_scratch.get(), s);
+    if (!dev) SBH_HIP(hipMemcpyAsync(order, dorder, nq * 4, hipMemcpyDeviceToHost, s));
+    SBH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_params* params,
+             const float* queries, int64_t q, float* results, int32_t* counts, float* kernel_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && params, "NULL argument");
+    SBH_REQUIRE(method == SBH_BRUTE_FORCE || method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
+                "unknown method");
+    SBH_REQUIRE(kernel == SBH_GRAVITY || kernel == SBH_WINDING_NUMBER || kernel == SBH_SMOOTH_DIST,
+                "unknown kernel");
+    SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
+    if (kernel_ms) *kernel_ms = 0.0f;
+    if (q == 0) return;
+    SBH_REQUIRE(queries && results, "NULL queries or results");
+    const sbh_eval_params& p = *params;
+    if (method == SBH_BARNES_HUT) {
+      SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
+      SBH_REQUIRE(p.vote_group == 1 || p.vote_group == 32 || p.vote_group == 64,
+                  "vote_group must be 1, 32 or 64");
+    }
+    if (method == SBH_STOCHASTIC)
+      SBH_REQUIRE(p.samples_per_subdomain >= 1, "samples_per_subdomain must be >= 1");
+    sbh::DeviceTree& t = tree->t;
+    hipStream_t s = t.stream;
+    const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
+    const uint32_t nq = (uint32_t)q;
+    const size_t qfloats = (size_t)nq * t.dim;
+    const float* dq = queries;
+    float* dout = results;
+    int32_t* dcnt = counts;
+    if (!dev) {
+      SBH_REQUIRE(all_finite(queries, qfloats), "queries must be finite");
+      ensure(tree->q, qfloats);
+      ensure(tree->out, nq);
+      SBH_HIP(hipMemcpyAsync(tree->q.get(), queries, qfloats * 4, hipMemcpyHostToDevice, s));
+      dq = tree->q.get();
+      dout = tree->out.get();
+      if (counts) {
+        ensure(tree->counts, nq);
+        dcnt = tree->counts.get();
+      }
+    }
+    // Counting only exists for the tree walks.
+    if (method == SBH_BRUTE_FORCE) {
+      uint32_t ns = sbh::brute_split_count(t.n, nq);
+      if (ns > 1) ensure(tree->partial, (size_t)ns * nq);
+      const bool sort = false;
+      (void)sort;
+      SBH_HIP(hipEventRecord(tree->ev0, s));
+      sbh::run_brute(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);
+      SBH_HIP(hipEventRecord(tree->ev1, s));
+      if (dcnt) {
+        std::vector<int32_t> c(nq, (int32_t)t.n);
+        SBH_HIP(hipMemcpyAsync(dcnt, c.data(), nq * 4,
+                               dev ? hipMemcpyHostToDevice : hipMemcpyHostToDevice, s));
+        SBH_HIP(hipStreamSynchronize(s));
+      }
+    } else {
+      const bool sort = (p.flags & SBH_EVAL_SORT_QUERIES) != 0;
+      uint32_t* ord = nullptr;
+      if (sort) {
+        ensure(tree->order, nq);
+        ensure(tree->sort_scratch, sbh::sort_scratch_u32(nq));
+        ord = tree->order.get();
+      }
+      SBH_HIP(hipEventRecord(tree->ev0, s));
+      if (sort) sbh::query_order(t, dq, nq, ord, tree->sort_scratch.get(), s);
+      if (method == SBH_BARNES_HUT)
+        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, s);
+      else
+        sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed, dq, nq, ord, dout, dcnt,
+                            s);
+      SBH_HIP(hipEventRecord(tree->ev1, s));
+    }
+    if (!dev) {
+      SBH_HIP(hipMemcpyAsync(results, dout, nq * 4, hipMemcpyDeviceToHost, s));
+      if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, nq * 4, hipMemcpyDeviceToHost, s));
+    }
+    SBH_HIP(hipEventSynchronize(tree->ev1));
+    if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));
+    SBH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q,
+                 double* results, int32_t flags, float* kernel_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(tree, "tree is NULL");
+    SBH_REQUIRE(kernel == SBH_GRAVITY || kernel == SBH_WINDING_NUMBER || kernel == SBH_SMOOTH_DIST,
+                "unknown kernel");
+    SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
+    if (kernel_ms) *kernel_ms = 0.0f;
+    if (q == 0) return;
+    SBH_REQUIRE(queries && results, "NULL queries or results");
+    sbh::DeviceTree& t = tree->t;
+    hipStream_t s = t.stream;
+    const bool dev = flags & SBH_EVAL_DEVICE_POINTERS;
+    const uint32_t nq = (uint32_t)q;
+    const size_t qfloats = (size_t)nq * t.dim;
+    const float* dq = queries;
+    double* dout = results;
+    if (!dev) {
+      SBH_REQUIRE(all_finite(queries, qfloats), "queries must be finite");
+      ensure(tree->q, qfloats);
+      ensure(tree->out64, nq);
+      SBH_HIP(hipMemcpyAsync(tree->q.get(), queries, qfloats * 4, hipMemcpyHostToDevice, s));
+      dq = tree->q.get();
+      dout = tree->out64.get();
+    }
+    uint32_t ns = std::max(1u, sbh::brute_split_count(t.n, nq));
+    ensure(tree->partial, (size_t)ns * nq);
+    SBH_HIP(hipEventRecord(tree->ev0, s));
+    sbh::run_brute_f64(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);
+    SBH_HIP(hipEventRecord(tree->ev1, s));
+    if (!dev) SBH_HIP(hipMemcpyAsync(results, dout, nq * 8, hipMemcpyDeviceToHost, s));
+    SBH_HIP(hipEventSynchronize(tree->ev1));
+    if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));
+    SBH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

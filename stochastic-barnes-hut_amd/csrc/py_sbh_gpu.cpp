@@ -1,0 +1,284 @@
+// pybind11 module `sbh_gpu`: the reference's GPU plugin API
+// (src/sbh_gpu.cu:566-661) over the C ABI of libsbh_hip.so.
+//
+//   QuadtreeGPU1/2/3, OctreeGPU1/2/3 (cpu_tree)       src/sbh_gpu.cu:571-582
+//   eval_brute_force_{gravity,wn,smoothdist}           src/sbh_gpu.cu:584-604
+//   eval_barnes_hut_{gravity,wn,smoothdist}            src/sbh_gpu.cu:606-623
+//   multi_level_prefix_control_variate_{...}           src/sbh_gpu.cu:625-660
+// Each evaluation returns (float32[q], kernel milliseconds), as the reference.
+// Keyword extensions keep the reference defaults: vote_group (32, the
+// reference's warp vote), seed (0), sort_queries (False), return_counts.
+// The GIL is released around device work.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sbh_hip.h"
+
+namespace py = pybind11;
+using namespace py::literals;
+
+namespace {
+
+using F32Array = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+void check(int status, const char* what) {
+  if (status == SBH_OK) return;
+  std::string msg = std::string(what) + ": " + sbh_last_error();
+  switch (status) {
+    case SBH_ERR_INVALID:
+      throw py::value_error(msg);
+    case SBH_ERR_NOMEM:
+      throw std::bad_alloc();
+    default:
+      throw std::runtime_error(msg);
+  }
+}
+
+struct GPUTreeBase {
+  sbh_tree* h = nullptr;
+  int dim = 3;
+  int split_log = 1;
+  float build_ms = 0.0f;
+
+  GPUTreeBase(int d, int k, F32Array pts, F32Array nrm, F32Array m, F32Array mn, F32Array mx,
+              float alpha, int leaf_size, bool moments64, int max_depth) {
+    dim = d;
+    split_log = k;
+    if (pts.ndim() != 2 || pts.shape(1) != d)
+      throw py::value_error("points must have shape (n, " + std::to_string(d) + ")");
+    const int64_t n = pts.shape(0);
+    const float* nptr = nullptr;
+    if (nrm.size() > 0) {
+      if (nrm.ndim() != 2 || nrm.shape(0) != n || nrm.shape(1) != d)
+        throw py::value_error("normals must have the shape of points");
+      nptr = nrm.data();
+    }
+    if (m.ndim() != 1 || m.shape(0) != n) throw py::value_error("masses must have shape (n,)");
+    if (mn.size() != d || mx.size() != d)
+      throw py::value_error("min_corner / max_corner must have " + std::to_string(d) + " values");
+    int status;
+    {
+      py::gil_scoped_release nogil;
+      status = sbh_tree_create(d, k, leaf_size, max_depth, n, pts.data(), nptr, m.data(),
+                               mn.data(), mx.data(), alpha, moments64 ? SBH_TREE_MOMENTS_F64 : 0,
+                               &h, &build_ms);
+    }
+    check(status, "tree build");
+  }
+  ~GPUTreeBase() {
+    if (h) sbh_tree_destroy(h);
+  }
+  GPUTreeBase(const GPUTreeBase&) = delete;
+  GPUTreeBase& operator=(const GPUTreeBase&) = delete;
+
+  py::dict info() const {
+    int64_t np = 0, nn = 0, bytes = 0;
+    int32_t depth = 0;
+    check(sbh_tree_info(h, &np, &nn, &depth, &bytes), "tree info");
+    py::dict d;
+    d["num_points"] = np;
+    d["num_nodes"] = nn;
+    d["depth"] = depth;
+    d["device_bytes"] = bytes;
+    d["build_ms"] = build_ms;
+    d["dim"] = dim;
+    d["split_log"] = split_log;
+    return d;
+  }
+
+  py::array_t<int64_t> leaf_order() const {
+    int64_t np = 0;
+    check(sbh_tree_info(h, &np, nullptr, nullptr, nullptr), "tree info");
+    py::array_t<int64_t> out(np);
+    check(sbh_tree_leaf_order(h, out.mutable_data()), "leaf order");
+    return out;
+  }
+
+  py::dict nodes() const {
+    int64_t nn = 0;
+    check(sbh_tree_info(h, nullptr, &nn, nullptr, nullptr), "tree info");
+    auto u32 = [&] { return py::array_t<uint32_t>(nn); };
+    py::array_t<uint32_t> start = u32(), end = u32(), first = u32(), nch = u32(), parent = u32(),
+                          rope = u32();
+    py::array_t<int32_t> depth(nn);
+    py::array_t<float> bmin({nn, (int64_t)dim}), bmax({nn, (int64_t)dim}), diam(nn),
+        centroid({nn, (int64_t)dim}), mass(nn), normal({nn, (int64_t)dim});
+    py::array_t<double> m64({nn, (int64_t)8});
+    check(sbh_tree_nodes(h, start.mutable_data(), end.mutable_data(), first.mutable_data(),
+                         nch.mutable_data(), depth.mutable_data(), parent.mutable_data(),
+                         rope.mutable_data(), bmin.mutable_data(), bmax.mutable_data(),
+                         diam.mutable_data(), centroid.mutable_data(), mass.mutable_data(),
+                         normal.mutable_data(), m64.mutable_data()),
+          "tree nodes");
+    py::dict d;
+    d["start"] = start;
+    d["end"] = end;
+    d["first_child"] = first;
+    d["nchild"] = nch;
+    d["depth"] = depth;
+    d["parent"] = parent;
+    d["rope"] = rope;
+    d["bmin"] = bmin;
+    d["bmax"] = bmax;
+    d["diam"] = diam;
+    d["centroid"] = centroid;
+    d["mass"] = mass;
+    d["normal"] = normal;
+    d["moments64"] = m64;
+    return d;
+  }
+
+  // (q, dim) row-major -> column-major [x..., y..., z...]
+  std::vector<float> soa(const F32Array& q) const {
+    if (q.ndim() != 2 || q.shape(1) != dim)
+      throw py::value_error("queries must have shape (q, " + std::to_string(dim) + ")");
+    const int64_t nq = q.shape(0);
+    std::vector<float> v((size_t)nq * dim);
+    const float* p = q.data();
+    for (int64_t i = 0; i < nq; ++i)
+      for (int a = 0; a < dim; ++a) v[(size_t)a * nq + i] = p[i * dim + a];
+    return v;
+  }
+};
+
+template <int D, int K>
+struct GPUTree : GPUTreeBase {
+  using GPUTreeBase::GPUTreeBase;
+};
+
+F32Array attr_array(const py::object& o, const char* name) {
+  return F32Array::ensure(o.attr(name));
+}
+
+// Device tree from a host tree of the sbh module (or any object with the same
+// attributes): include/py_tree_gpu.h:39-80.
+template <int D, int K>
+std::unique_ptr<GPUTree<D, K>> from_host(const py::object& cpu, bool moments64) {
+  int d = cpu.attr("dim").cast<int>();
+  int k = cpu.attr("split_log").cast<int>();
+  if (d != D || k != K)
+    throw py::type_error("host tree has dim " + std::to_string(d) + " / split " +
+                         std::to_string(k) + ", expected " + std::to_string(D) + " / " +
+                         std::to_string(K));
+  int leaf = py::hasattr(cpu, "leaf_size") ? cpu.attr("leaf_size").cast<int>() : 1;
+  return std::make_unique<GPUTree<D, K>>(
+      D, K, attr_array(cpu, "points"), attr_array(cpu, "normals"), attr_array(cpu, "masses"),
+      attr_array(cpu, "min_corner"), attr_array(cpu, "max_corner"),
+      cpu.attr("alpha").cast<float>(), leaf, moments64, 100);
+}
+
+template <int D, int K>
+void bind_tree(py::module_& m, const char* name) {
+  py::class_<GPUTree<D, K>, GPUTreeBase>(m, name)
+      .def(py::init([](const py::object& cpu, bool moments64) {
+             return from_host<D, K>(cpu, moments64);
+           }),
+           "cpu_tree"_a, "moments64"_a = false)
+      .def(py::init([](F32Array pts, F32Array nrm, F32Array mass, F32Array mn, F32Array mx,
+                       float alpha, int leaf_size, bool moments64, int max_depth) {
+             return std::make_unique<GPUTree<D, K>>(D, K, pts, nrm, mass, mn, mx, alpha,
+                                                    leaf_size, moments64, max_depth);
+           }),
+           "points"_a, "normals"_a, "masses"_a, "min_corner"_a, "max_corner"_a,
+           "alpha"_a = 1.0f, "leaf_size"_a = 1, "moments64"_a = false, "max_depth"_a = 100);
+}
+
+py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, float beta,
+                    int spl, uint64_t seed, int vote_group, bool sort, bool counts) {
+  std::vector<float> q = t.soa(queries);
+  const int64_t nq = queries.shape(0);
+  py::array_t<float> out(nq);
+  py::array_t<int32_t> cnt(counts ? nq : 0);
+  sbh_eval_params p{};
+  p.beta = beta;
+  p.samples_per_subdomain = spl;
+  p.seed = seed;
+  p.vote_group = vote_group;
+  p.flags = sort ? SBH_EVAL_SORT_QUERIES : 0;
+  float ms = 0.0f;
+  float* op = out.mutable_data();
+  int32_t* cp = counts ? cnt.mutable_data() : nullptr;
+  int status;
+  {
+    py::gil_scoped_release nogil;
+    status = sbh_eval(t.h, method, kernel, &p, q.data(), nq, op, cp, &ms);
+  }
+  check(status, "evaluation");
+  if (counts) return py::make_tuple(out, ms, cnt);
+  return py::make_tuple(out, ms);
+}
+
+void bind_kernel(py::module_& m, const char* suffix, int kernel) {
+  std::string s = suffix;
+  m.def(("eval_brute_force_" + s).c_str(),
+        [kernel](GPUTreeBase& t, F32Array q) {
+          return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false);
+        },
+        "tree"_a, "queries"_a);
+  m.def(("eval_barnes_hut_" + s).c_str(),
+        [kernel](GPUTreeBase& t, F32Array q, float beta, int vote_group, bool sort_queries,
+                 bool return_counts) {
+          return run_eval(t, SBH_BARNES_HUT, kernel, q, beta, 1, 0, vote_group, sort_queries,
+                          return_counts);
+        },
+        "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "sort_queries"_a = false,
+        "return_counts"_a = false);
+  m.def(("multi_level_prefix_control_variate_" + s).c_str(),
+        [kernel](GPUTreeBase& t, F32Array q, int spl, uint64_t seed, bool sort_queries,
+                 bool return_counts) {
+          return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.f, spl, seed, 1, sort_queries,
+                          return_counts);
+        },
+        "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
+        "return_counts"_a = false);
+  m.def(("eval_brute_force_f64_" + s).c_str(),
+        [kernel](GPUTreeBase& t, F32Array queries) {
+          std::vector<float> q = t.soa(queries);
+          const int64_t nq = queries.shape(0);
+          py::array_t<double> out(nq);
+          float ms = 0.0f;
+          double* op = out.mutable_data();
+          int status;
+          {
+            py::gil_scoped_release nogil;
+            status = sbh_eval_f64(t.h, kernel, q.data(), nq, op, 0, &ms);
+          }
+          check(status, "float64 evaluation");
+          return py::make_tuple(out, ms);
+        },
+        "tree"_a, "queries"_a);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(sbh_gpu, m) {
+  m.doc() = "Stochastic Barnes-Hut fast summation on AMD Instinct MI355X (gfx950)";
+  py::class_<GPUTreeBase>(m, "_GPUTree")
+      .def("info", &GPUTreeBase::info)
+      .def("leaf_order", &GPUTreeBase::leaf_order)
+      .def("nodes", &GPUTreeBase::nodes)
+      .def_readonly("dim", &GPUTreeBase::dim)
+      .def_readonly("split_log", &GPUTreeBase::split_log)
+      .def_readonly("build_ms", &GPUTreeBase::build_ms);
+  bind_tree<2, 1>(m, "QuadtreeGPU1");
+  bind_tree<2, 2>(m, "QuadtreeGPU2");
+  bind_tree<2, 3>(m, "QuadtreeGPU3");
+  bind_tree<3, 1>(m, "OctreeGPU1");
+  bind_tree<3, 2>(m, "OctreeGPU2");
+  bind_tree<3, 3>(m, "OctreeGPU3");
+  bind_kernel(m, "gravity", SBH_GRAVITY);
+  bind_kernel(m, "wn", SBH_WINDING_NUMBER);
+  bind_kernel(m, "smoothdist", SBH_SMOOTH_DIST);
+  m.def("device_count", [] {
+    int32_t n = 0;
+    check(sbh_device_count(&n), "device count");
+    return n;
+  });
+  m.attr("fp") = 32;
+}

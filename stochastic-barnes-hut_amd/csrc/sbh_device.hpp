@@ -1,0 +1,177 @@
+// Device-side building blocks shared by the tree build and the evaluation
+// kernels: source/node record layout, the three summation kernels, Philox and
+// the exact-fp32 helpers the parity-critical paths use.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sbh {
+
+constexpr int kWave = 64;
+constexpr uint32_t kEnd = 0xFFFFFFFFu;   // rope terminator
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+enum Kernel : int { GRAVITY = 0, WINDING_NUMBER = 1, SMOOTH_DIST = 2 };  // include/kernel.h:24-28
+
+// Constants of include/kernel.h:31-59 in the reference's float evaluation:
+// 4 * (float)M_PI and 2 * (float)M_PI are exact power-of-two multiples.
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kFourPi = 4.0f * kPiF;
+constexpr float kTwoPi = 2.0f * kPiF;
+
+// ---------------------------------------------------------------------------
+// Exact fp32 kernels: one rounding per operation, operations in the
+// reference's order (Eigen fixed-size reductions sum a0 + (a1 + a2)),
+// correctly rounded sqrt and divide.  Used where results must match the
+// oracle bit for bit: Barnes-Hut and the stochastic estimator.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ float sqnorm_exact(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+  if constexpr (D == 3) {
+    return dx * dx + (dy * dy + dz * dz);
+  } else {
+    return dx * dx + dy * dy;
+  }
+}
+
+// src = {x, y, z, m}, nrm = {nx, ny, nz, *}, q = {qx, qy, qz}
+template <int D, int K>
+__device__ __forceinline__ float kernel_exact(float4 src, float4 nrm, float alpha, float qx,
+                                              float qy, float qz) {
+#pragma clang fp contract(off)
+  float dx = src.x - qx;
+  float dy = src.y - qy;
+  float dz = (D == 3) ? (src.z - qz) : 0.0f;
+  if constexpr (K == GRAVITY) {
+    float dist = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
+    return -src.w / (dist + 1e-1f);
+  } else if constexpr (K == WINDING_NUMBER) {
+    if constexpr (D == 3) {
+      float dist = __builtin_sqrtf(sqnorm_exact<3>(dx, dy, dz));
+      float qdn = nrm.x * dx + (nrm.y * dy + nrm.z * dz);
+      return src.w * qdn / (kFourPi * (dist * dist * dist + 1e-5f));
+    } else {
+      float dsq = sqnorm_exact<2>(dx, dy, dz);
+      float qdn = nrm.x * dx + nrm.y * dy;
+      return src.w * qdn / (kTwoPi * (dsq + 1e-5f));
+    }
+  } else {
+    float dist = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
+    return expf(-alpha * dist);
+  }
+}
+
+// Barnes-Hut opening ratio ||q - c|| / diam (include/tree3d.h:134-140).
+template <int D>
+__device__ __forceinline__ float ratio_exact(float4 c, float diam, float qx, float qy, float qz) {
+#pragma clang fp contract(off)
+  float dx = qx - c.x;
+  float dy = qy - c.y;
+  float dz = (D == 3) ? (qz - c.z) : 0.0f;
+  return __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz)) / diam;
+}
+
+// ---------------------------------------------------------------------------
+// Fast kernels for brute force: hardware sqrt / reciprocal / exp2 (about one
+// ulp each), fused multiply-adds allowed.  Sums stay within 1e-5 relative of
+// the exact formulation.
+// ---------------------------------------------------------------------------
+template <int D, int K>
+__device__ __forceinline__ float kernel_fast(float4 src, float4 nrm, float alpha, float qx,
+                                             float qy, float qz) {
+  float dx = src.x - qx;
+  float dy = src.y - qy;
+  float dz = (D == 3) ? (src.z - qz) : 0.0f;
+  float dsq = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
+                       : __builtin_fmaf(dx, dx, dy * dy);
+  if constexpr (K == GRAVITY) {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    return -src.w * __builtin_amdgcn_rcpf(dist + 1e-1f);
+  } else if constexpr (K == WINDING_NUMBER) {
+    if constexpr (D == 3) {
+      float dist = __builtin_amdgcn_sqrtf(dsq);
+      float qdn = __builtin_fmaf(nrm.x, dx, __builtin_fmaf(nrm.y, dy, nrm.z * dz));
+      float den = kFourPi * __builtin_fmaf(dsq, dist, 1e-5f);
+      return src.w * qdn * __builtin_amdgcn_rcpf(den);
+    } else {
+      float qdn = __builtin_fmaf(nrm.x, dx, nrm.y * dy);
+      return src.w * qdn * __builtin_amdgcn_rcpf(kTwoPi * (dsq + 1e-5f));
+    }
+  } else {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    // exp(-a d) = 2^(-a d log2 e); alpha is pre-scaled by log2(e) by the caller.
+    return __builtin_amdgcn_exp2f(-alpha * dist);
+  }
+}
+
+// float64 formulation of the same kernels (error reporting only).
+template <int D, int K>
+__device__ __forceinline__ double kernel_f64(float4 src, float4 nrm, float alpha, float qx,
+                                             float qy, float qz) {
+  double dx = (double)src.x - (double)qx;
+  double dy = (double)src.y - (double)qy;
+  double dz = (D == 3) ? ((double)src.z - (double)qz) : 0.0;
+  double dsq = dx * dx + dy * dy + dz * dz;
+  double dist = sqrt(dsq);
+  if constexpr (K == GRAVITY) {
+    return -(double)src.w / (dist + (double)1e-1f);
+  } else if constexpr (K == WINDING_NUMBER) {
+    double qdn = (double)nrm.x * dx + (double)nrm.y * dy + (D == 3 ? (double)nrm.z * dz : 0.0);
+    if constexpr (D == 3)
+      return (double)src.w * qdn / ((double)kFourPi * (dist * dsq + (double)1e-5f));
+    else
+      return (double)src.w * qdn / ((double)kTwoPi * (dsq + (double)1e-5f));
+  } else {
+    return exp(-(double)alpha * dist);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel Random Numbers: As
+// Easy as 1, 2, 3", SC'11).  10 rounds, key bumped by the Weyl constants
+// between rounds.
+// ---------------------------------------------------------------------------
+struct Philox4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                 uint32_t c3, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    uint32_t lo0 = M0 * c0, hi0 = __umulhi(M0, c0);
+    uint32_t lo1 = M1 * c2, hi1 = __umulhi(M1, c2);
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  return {c0, c1, c2, c3};
+}
+
+// Russian-roulette uniform in (0, 1] from one 32-bit word.
+__device__ __forceinline__ float rr_uniform(uint32_t w) {
+  return (float)((w >> 8) + 1u) * 0x1p-24f;
+}
+
+// Uniform index in [0, n) from a 64-bit draw (multiply-high).
+__device__ __forceinline__ uint32_t scale_draw(uint32_t lo, uint32_t hi, uint32_t n) {
+  uint64_t u = ((uint64_t)hi << 32) | lo;
+  return (uint32_t)__umul64hi(u, (uint64_t)n);
+}
+
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+}  // namespace sbh

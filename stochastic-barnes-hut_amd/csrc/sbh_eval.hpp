@@ -1,0 +1,36 @@
+// Launchers for the evaluation kernels (eval.hip) and device utilities
+// (tree_build.hip).  All pointers are device pointers; queries are SoA columns
+// [x..., y..., (z...)] of length nq.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sbh_runtime.hpp"
+
+namespace sbh {
+
+uint32_t brute_split_count(uint32_t n, uint32_t nq);
+void run_brute(const DeviceTree& t, int kernel, const float* q, uint32_t nq, float* out,
+               double* partial, uint32_t nsplit, hipStream_t s);
+void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq, double* out,
+                   double* partial, uint32_t nsplit, hipStream_t s);
+void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
+            const uint32_t* order, float* out, int32_t* counts, hipStream_t s);
+void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, const float* q,
+                    uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
+                    hipStream_t s);
+
+// Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)
+// uint32 values.
+size_t sort_scratch_u32(uint32_t nq);
+void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* order,
+                 uint32_t* scratch, hipStream_t s);
+
+// Exclusive scan of n uint32 values; scratch holds exclusive_scan_scratch(n)
+// values; *total_dev (optional) receives the sum.
+size_t exclusive_scan_scratch(uint32_t n);
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev,
+                    uint32_t* scratch, hipStream_t s);
+
+}  // namespace sbh

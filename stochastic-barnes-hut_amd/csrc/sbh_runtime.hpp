@@ -1,0 +1,146 @@
+// Host-side runtime pieces shared by the library's translation units: error
+// handling, device buffers, the device tree object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace sbh {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Status codes shared with include/sbh_hip.h.
+constexpr int kOk = 0;
+constexpr int kErrInvalid = 1;
+constexpr int kErrHip = 2;
+constexpr int kErrNoMemory = 3;
+constexpr int kErrNoDevice = 4;
+constexpr int kErrBudget = 5;
+constexpr int kErrInternal = 6;
+
+#define SBH_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      throw ::sbh::Error(_e == hipErrorOutOfMemory ? ::sbh::kErrNoMemory               \
+                                                   : ::sbh::kErrHip,                   \
+                         std::string(#expr) + ": " + hipGetErrorString(_e));           \
+    }                                                                                  \
+  } while (0)
+
+#define SBH_REQUIRE(cond, msg)                                     \
+  do {                                                             \
+    if (!(cond)) throw ::sbh::Error(::sbh::kErrInvalid, (msg));    \
+  } while (0)
+
+// Owning device allocation.
+template <typename T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) n = 1;
+    SBH_HIP(hipMalloc((void**)&p_, n * sizeof(T)));
+    n_ = n;
+  }
+  // Grow to at least n elements, keeping the first `keep` elements.
+  void grow(size_t n, size_t keep, hipStream_t s) {
+    if (n <= n_) return;
+    T* np = nullptr;
+    SBH_HIP(hipMalloc((void**)&np, n * sizeof(T)));
+    if (p_ && keep) SBH_HIP(hipMemcpyAsync(np, p_, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+    SBH_HIP(hipStreamSynchronize(s));
+    release();
+    p_ = np;
+    n_ = n;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  size_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Device-resident tree.  All node arrays are indexed by the reference's
+// breadth-first node number (include/tree3d.h:338-371); points are stored in
+// the tree's leaf order.
+struct DeviceTree {
+  int dim = 3;
+  int split_log = 1;
+  int leaf_size = 1;
+  int max_depth = 100;
+  int moments64 = 0;  // node moments aggregated in float64 instead of reference fp32 order
+  uint32_t n = 0;
+  uint32_t num_nodes = 0;
+  int depth = 0;  // deepest level
+  float alpha = 1.0f;
+  float lo[3] = {0, 0, 0};
+  float hi[3] = {0, 0, 0};
+  std::vector<uint32_t> level_begin;  // level l holds nodes [level_begin[l], level_begin[l+1])
+  hipStream_t stream = nullptr;
+
+  // points, leaf order
+  DevBuf<float4> pts;   // {x, y, z, mass}
+  DevBuf<float4> nrm;   // {nx, ny, nz, 0}
+  DevBuf<uint32_t> perm;  // input row of each sorted point
+
+  // nodes (breadth-first numbering)
+  DevBuf<uint32_t> n_start, n_end, n_parent, n_first, n_nchild, n_next;
+  DevBuf<int32_t> n_depth;
+  DevBuf<float4> n_lo, n_hi;  // cell bounds (w unused)
+  DevBuf<float> n_diam;
+  DevBuf<float4> nodeC;  // {cx, cy, cz, mass}
+  DevBuf<float4> nodeN;  // {nx, ny, nz, 0}
+  DevBuf<uint4> nodeL;   // {diam bits, first child | first point, rope, info}
+  DevBuf<double> n_m64;  // float64 moments {cx, cy, cz, mass, nx, ny, nz, 0} per node
+  // stochastic descent helpers
+  DevBuf<uint32_t> n_skip;  // node reached by following single-child chains
+
+  size_t device_bytes() const {
+    return pts.bytes() + nrm.bytes() + perm.bytes() + n_start.bytes() + n_end.bytes() +
+           n_parent.bytes() + n_first.bytes() + n_nchild.bytes() + n_next.bytes() +
+           n_depth.bytes() + n_lo.bytes() + n_hi.bytes() + n_diam.bytes() + nodeC.bytes() +
+           nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes();
+  }
+};
+
+// Builds the tree on the device from input rows (row-major n x dim).
+// Host pointers; blocks until the build is complete.
+void build_device_tree(DeviceTree& t, const float* points, const float* normals,
+                       const float* masses, float* build_ms);
+
+}  // namespace sbh

@@ -1,0 +1,831 @@
+// Level-synchronous device tree build.
+//
+// Follows the reference build (include/tree3d.h:238-371, include/tree2d.h:
+// 237-365) exactly: every internal node runs one two-ended std::partition per
+// child cell, cells in raster order, each against the node's own fp32 bounds;
+// children are numbered breadth first; moments are aggregated bottom up.  The
+// leaf order it produces is therefore the reference's leaf order, including
+// the order inside multi-point leaves.
+//
+// One two-ended partition of [s, e) by predicate P moves nothing but this: with
+// t = #P, the i-th element of [s, s+t) failing P swaps with the i-th element
+// of [s+t, e) satisfying P counted from the right.  Each pass is therefore a
+// pair of prefix ranks and a swap, which is what the kernels below compute:
+//   * nodes of up to kSmallMax points: one workgroup, everything in LDS;
+//   * larger nodes: chunked device-wide passes (count, offsets, rank, swap).
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "sbh_device.hpp"
+#include "sbh_eval.hpp"
+#include "sbh_runtime.hpp"
+
+namespace sbh {
+namespace {
+
+constexpr int kSmallMax = 4096;
+constexpr int kSmallThreads = 256;
+constexpr int kChunk = 1024;
+constexpr int kChunkThreads = 256;
+constexpr int kMaxCells = 512;  // 3D with split_log 3
+
+// float -> int with x86 cvttss2si semantics (out of range / NaN -> INT_MIN),
+// the conversion behind the reference's cast<int>() (include/aabb.h:34-41).
+__device__ __forceinline__ int trunc_x86(float v) {
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : INT_MIN;
+}
+
+__device__ __forceinline__ int quant_axis(float p, float lo, float hi, int dsplit) {
+#pragma clang fp contract(off)
+  float num = (float)dsplit * (p - lo);
+  float den = hi - lo;
+  int v = trunc_x86(num / den);
+  v = v < dsplit - 1 ? v : dsplit - 1;
+  v = v > 0 ? v : 0;
+  return v;
+}
+
+// Child cell of point p in box [lo, hi): raster index ((z << k) + y) << k + x
+// (include/tree3d.h:62-65).
+template <int D>
+__device__ __forceinline__ uint32_t cell_of(float4 p, float4 lo, float4 hi, int k) {
+  int ds = 1 << k;
+  uint32_t x = quant_axis(p.x, lo.x, hi.x, ds);
+  uint32_t y = quant_axis(p.y, lo.y, hi.y, ds);
+  if constexpr (D == 3) {
+    uint32_t z = quant_axis(p.z, lo.z, hi.z, ds);
+    return (((z << k) + y) << k) + x;
+  } else {
+    return (y << k) + x;
+  }
+}
+
+// Sub-box of cell `cell` (include/tree3d.h:269-280): min + coord * inc, + inc.
+template <int D>
+__device__ __forceinline__ void cell_box(float4 lo, float4 hi, int k, uint32_t cell, float4& clo,
+                                         float4& chi, float& diam) {
+#pragma clang fp contract(off)
+  int ds = 1 << k;
+  float fds = (float)ds;
+  uint32_t mask = (uint32_t)ds - 1;
+  float ix = (float)(cell & mask);
+  float iy = (float)((cell >> k) & mask);
+  float iz = (float)((cell >> (2 * k)) & mask);
+  float incx = (hi.x - lo.x) / fds;
+  float incy = (hi.y - lo.y) / fds;
+  float incz = (D == 3) ? (hi.z - lo.z) / fds : 0.0f;
+  clo.x = lo.x + ix * incx;
+  clo.y = lo.y + iy * incy;
+  clo.z = (D == 3) ? lo.z + iz * incz : 0.0f;
+  clo.w = 0.0f;
+  chi.x = clo.x + incx;
+  chi.y = clo.y + incy;
+  chi.z = (D == 3) ? clo.z + incz : 0.0f;
+  chi.w = 0.0f;
+  float dx = chi.x - clo.x, dy = chi.y - clo.y, dz = chi.z - clo.z;
+  diam = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
+}
+
+// Exclusive scan of one value per thread over a block of NT threads.
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum,
+                                                         uint32_t& total) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    uint32_t s = wsum[i];
+    base += (i < w) ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return base + x - v;
+}
+
+template <int NT>
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
+  uint32_t total;
+  (void)block_exclusive_scan<NT>(v, wsum, total);
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// Level bookkeeping
+// ---------------------------------------------------------------------------
+__global__ void k_classify(uint32_t lb, uint32_t count, const uint32_t* n_start,
+                           const uint32_t* n_end, const int32_t* n_depth, int leaf_size,
+                           int max_depth, uint32_t* n_nchild, uint32_t* n_first,
+                           uint32_t* small_list, uint32_t* large_list, uint32_t* counters) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t v = lb + i;
+  uint32_t cnt = n_end[v] - n_start[v];
+  // include/tree3d.h:34-36
+  bool leaf = n_depth[v] >= max_depth || cnt <= (uint32_t)leaf_size;
+  n_nchild[v] = 0;
+  n_first[v] = kEnd;
+  if (leaf) return;
+  if (cnt <= (uint32_t)kSmallMax) {
+    small_list[atomicAdd(&counters[0], 1u)] = v;
+  } else {
+    large_list[atomicAdd(&counters[1], 1u)] = v;
+  }
+}
+
+// Zone ranking inside one LDS-resident node: element at position p of [a, b)
+// is flagged when (dig[p] == c) == want_eq; flagged positions are written to
+// out[] in position order (or reversed).  Returns the number flagged.
+template <int NT>
+__device__ uint32_t rank_zone(const uint16_t* dig, uint32_t a, uint32_t b, uint32_t c,
+                              bool want_eq, bool reverse, uint16_t* out, uint32_t* wsum) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t len = b - a;
+  uint32_t per = (len + NT - 1) / NT;
+  uint32_t p0 = a + min(len, tid * per);
+  uint32_t p1 = a + min(len, (tid + 1) * per);
+  uint32_t cnt = 0;
+  for (uint32_t p = p0; p < p1; ++p) cnt += ((dig[p] == c) == want_eq) ? 1u : 0u;
+  uint32_t total;
+  uint32_t r = block_exclusive_scan<NT>(cnt, wsum, total);
+  for (uint32_t p = p0; p < p1; ++p) {
+    if ((dig[p] == c) == want_eq) {
+      out[reverse ? (total - 1 - r) : r] = (uint16_t)p;
+      ++r;
+    }
+  }
+  return total;
+}
+
+// One workgroup per internal node of at most kSmallMax points.
+template <int D>
+__global__ __launch_bounds__(kSmallThreads) void k_partition_small(
+    const uint32_t* list, const uint32_t* n_start, const uint32_t* n_end, const float4* n_lo,
+    const float4* n_hi, uint32_t* perm, const float4* src_pts, int k, uint32_t* n_nchild,
+    uint4* temp_child) {
+  __shared__ uint32_t lperm[kSmallMax];
+  __shared__ uint16_t ldig[kSmallMax];
+  __shared__ uint16_t L[kSmallMax / 2];
+  __shared__ uint16_t R[kSmallMax / 2];
+  __shared__ uint32_t hist[kMaxCells];
+  __shared__ uint32_t wsum[kSmallThreads / 64];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t node = list[blockIdx.x];
+  const uint32_t s0 = n_start[node];
+  const uint32_t n = n_end[node] - s0;
+  const float4 lo = n_lo[node], hi = n_hi[node];
+  const uint32_t B = 1u << (D * k);
+
+  for (uint32_t i = tid; i < B; i += kSmallThreads) hist[i] = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    uint32_t pi = perm[s0 + i];
+    lperm[i] = pi;
+    uint32_t d = cell_of<D>(src_pts[pi], lo, hi, k);
+    ldig[i] = (uint16_t)d;
+    atomicAdd(&hist[d], 1u);
+  }
+  __syncthreads();
+
+  uint32_t s = 0, nch = 0;
+  for (uint32_t c = 0; c < B && s < n; ++c) {
+    uint32_t t = hist[c];
+    if (t == 0) continue;
+    if (tid == 0) temp_child[s0 + nch] = make_uint4(c, s0 + s, s0 + s + t, 0);
+    ++nch;
+    if (s + t < n) {
+      uint32_t m = rank_zone<kSmallThreads>(ldig, s, s + t, c, false, false, L, wsum);
+      if (m > 0) {
+        (void)rank_zone<kSmallThreads>(ldig, s + t, n, c, true, true, R, wsum);
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += kSmallThreads) {
+          uint32_t a = L[i], b = R[i];
+          uint32_t tp = lperm[a];
+          lperm[a] = lperm[b];
+          lperm[b] = tp;
+          uint16_t td = ldig[a];
+          ldig[a] = ldig[b];
+          ldig[b] = td;
+        }
+        __syncthreads();
+      }
+    }
+    s += t;
+  }
+  for (uint32_t i = tid; i < n; i += kSmallThreads) perm[s0 + i] = lperm[i];
+  if (tid == 0) n_nchild[node] = nch;
+}
+
+// ---------------------------------------------------------------------------
+// Large nodes: chunk table entries are {large index, begin, end, node}.
+// ---------------------------------------------------------------------------
+struct LargeCtx {
+  const uint4* chunks;
+  const uint32_t* l_start;  // per large node
+  const uint32_t* l_end;
+  uint32_t* hist;        // [nlarge * B]
+  uint32_t* cellstart;   // [nlarge * B]
+  uint32_t* mcount;      // [nlarge]
+  uint2* ccnt;           // [nchunks]
+  uint2* coff;           // [nchunks]
+  const uint32_t* l_chunk0;  // first chunk of each large node
+  const uint32_t* l_nchunk;
+  uint16_t* digit;       // [N] position-indexed
+  uint32_t* Lg;          // [N]
+  uint32_t* Rg;          // [N]
+};
+
+template <int D>
+__global__ __launch_bounds__(kChunkThreads) void k_large_digits(LargeCtx cx, const float4* n_lo,
+                                                                 const float4* n_hi,
+                                                                 const uint32_t* perm,
+                                                                 const float4* src_pts, int k) {
+  __shared__ uint32_t lh[kMaxCells];
+  const uint4 ch = cx.chunks[blockIdx.x];
+  const uint32_t B = 1u << (D * k);
+  for (uint32_t i = threadIdx.x; i < B; i += kChunkThreads) lh[i] = 0;
+  __syncthreads();
+  const float4 lo = n_lo[ch.w], hi = n_hi[ch.w];
+  for (uint32_t p = ch.y + threadIdx.x; p < ch.z; p += kChunkThreads) {
+    uint32_t d = cell_of<D>(src_pts[perm[p]], lo, hi, k);
+    cx.digit[p] = (uint16_t)d;
+    atomicAdd(&lh[d], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < B; i += kChunkThreads)
+    if (lh[i]) atomicAdd(&cx.hist[ch.x * B + i], lh[i]);
+}
+
+// Per large node: child ranges from the histogram (children of a partitioned
+// node are the runs of equal cell, in cell order).
+__global__ void k_large_cells(LargeCtx cx, const uint32_t* large_list, uint32_t nlarge, uint32_t B,
+                              uint32_t* n_nchild, uint4* temp_child) {
+  uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nlarge) return;
+  uint32_t run = cx.l_start[li], nch = 0;
+  for (uint32_t c = 0; c < B; ++c) {
+    uint32_t h = cx.hist[li * B + c];
+    cx.cellstart[li * B + c] = run;
+    if (h) {
+      temp_child[cx.l_start[li] + nch] = make_uint4(c, run, run + h, 0);
+      ++nch;
+    }
+    run += h;
+  }
+  n_nchild[large_list[li]] = nch;
+}
+
+__device__ __forceinline__ void large_flags(const LargeCtx& cx, uint32_t li, uint32_t B, uint32_t c,
+                                            uint32_t p, bool& lf, bool& rt) {
+  uint32_t s = cx.cellstart[li * B + c];
+  uint32_t t = cx.hist[li * B + c];
+  uint32_t e = cx.l_end[li];
+  uint32_t d = cx.digit[p];
+  lf = (p >= s) && (p < s + t) && (d != c);
+  rt = (p >= s + t) && (p < e) && (d == c);
+}
+
+__global__ __launch_bounds__(kChunkThreads) void k_large_count(LargeCtx cx, uint32_t B, uint32_t c) {
+  __shared__ uint32_t wsum[kChunkThreads / 64];
+  const uint4 ch = cx.chunks[blockIdx.x];
+  uint32_t nl = 0, nr = 0;
+  if (cx.hist[ch.x * B + c] != 0) {
+    for (uint32_t p = ch.y + threadIdx.x; p < ch.z; p += kChunkThreads) {
+      bool lf, rt;
+      large_flags(cx, ch.x, B, c, p, lf, rt);
+      nl += lf;
+      nr += rt;
+    }
+  }
+  uint32_t tot = block_sum<kChunkThreads>(nl | (nr << 16), wsum);
+  if (threadIdx.x == 0) cx.ccnt[blockIdx.x] = make_uint2(tot & 0xFFFFu, tot >> 16);
+}
+
+__global__ void k_large_offsets(LargeCtx cx, uint32_t nlarge) {
+  uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nlarge) return;
+  uint32_t ol = 0, orr = 0;
+  uint32_t c0 = cx.l_chunk0[li], nc = cx.l_nchunk[li];
+  for (uint32_t j = 0; j < nc; ++j) {
+    cx.coff[c0 + j] = make_uint2(ol, orr);
+    uint2 cc = cx.ccnt[c0 + j];
+    ol += cc.x;
+    orr += cc.y;
+  }
+  cx.mcount[li] = ol;
+}
+
+// Chunk of kChunk positions, 4 consecutive positions per thread.
+__global__ __launch_bounds__(kChunkThreads) void k_large_rank(LargeCtx cx, uint32_t B, uint32_t c) {
+  __shared__ uint32_t wsum[kChunkThreads / 64];
+  const uint4 ch = cx.chunks[blockIdx.x];
+  if (cx.hist[ch.x * B + c] == 0) return;  // uniform per block
+  const uint32_t s = cx.cellstart[ch.x * B + c];
+  const uint32_t m = cx.mcount[ch.x];
+  const uint2 off = cx.coff[blockIdx.x];
+  uint32_t p0 = ch.y + threadIdx.x * 4;
+  bool lf[4], rt[4];
+  uint32_t nl = 0, nr = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t p = p0 + j;
+    lf[j] = rt[j] = false;
+    if (p < ch.z) large_flags(cx, ch.x, B, c, p, lf[j], rt[j]);
+    nl += lf[j];
+    nr += rt[j];
+  }
+  uint32_t total;
+  uint32_t pre = block_exclusive_scan<kChunkThreads>(nl | (nr << 16), wsum, total);
+  uint32_t rl = off.x + (pre & 0xFFFFu);
+  uint32_t rr = off.y + (pre >> 16);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t p = p0 + j;
+    if (lf[j]) cx.Lg[s + rl++] = p;
+    if (rt[j]) cx.Rg[s + (m - 1 - rr++)] = p;
+  }
+}
+
+__global__ __launch_bounds__(kChunkThreads) void k_large_swap(LargeCtx cx, uint32_t B, uint32_t c,
+                                                              uint32_t* perm) {
+  const uint4 ch = cx.chunks[blockIdx.x];
+  if (cx.hist[ch.x * B + c] == 0) return;
+  const uint32_t s = cx.cellstart[ch.x * B + c];
+  const uint32_t m = cx.mcount[ch.x];
+  for (uint32_t p = ch.y + threadIdx.x; p < ch.z; p += kChunkThreads) {
+    if (p >= s && p < s + m) {
+      uint32_t a = cx.Lg[p], b = cx.Rg[p];
+      uint32_t tp = perm[a];
+      perm[a] = perm[b];
+      perm[b] = tp;
+      uint16_t td = cx.digit[a];
+      cx.digit[a] = cx.digit[b];
+      cx.digit[b] = td;
+    }
+  }
+}
+
+// Children of the level's internal nodes get consecutive breadth-first
+// numbers: node order, then cell order (include/tree3d.h:400-411).
+template <int D>
+__global__ void k_emit(uint32_t lb, uint32_t count, uint32_t next_base, const uint32_t* cbase, int k,
+                       uint32_t* n_start, uint32_t* n_end, uint32_t* n_parent, uint32_t* n_first,
+                       const uint32_t* n_nchild, uint32_t* n_next, int32_t* n_depth, float4* n_lo,
+                       float4* n_hi, float* n_diam, const uint4* temp_child) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t v = lb + i;
+  uint32_t nch = n_nchild[v];
+  if (nch == 0) return;
+  uint32_t base = next_base + cbase[i];
+  n_first[v] = base;
+  float4 lo = n_lo[v], hi = n_hi[v];
+  int32_t dep = n_depth[v] + 1;
+  uint32_t vnext = n_next[v];
+  uint32_t s0 = n_start[v];
+  for (uint32_t j = 0; j < nch; ++j) {
+    uint4 tc = temp_child[s0 + j];
+    uint32_t cidx = base + j;
+    n_start[cidx] = tc.y;
+    n_end[cidx] = tc.z;
+    n_parent[cidx] = v;
+    n_depth[cidx] = dep;
+    float4 clo, chi;
+    float diam;
+    cell_box<D>(lo, hi, k, tc.x, clo, chi, diam);
+    n_lo[cidx] = clo;
+    n_hi[cidx] = chi;
+    n_diam[cidx] = diam;
+    n_next[cidx] = (j + 1 < nch) ? cidx + 1 : vnext;
+  }
+}
+
+__global__ void k_gather_points(uint32_t n, const uint32_t* perm, const float4* src_pts,
+                                const float4* src_nrm, float4* pts, float4* nrm) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t j = perm[i];
+  pts[i] = src_pts[j];
+  nrm[i] = src_nrm[j];
+}
+
+// Node moments (include/tree3d.h:255-264 for leaves, :310-327 for internal
+// nodes): fp32, array / bitmask order, one rounding per operation.  The
+// float64 aggregation of the same quantities is kept beside them.
+template <int D>
+__global__ void k_moments(uint32_t lb, uint32_t count, const uint32_t* n_start,
+                          const uint32_t* n_end, const uint32_t* n_first,
+                          const uint32_t* n_nchild, const float4* pts, const float4* nrm,
+                          float4* nodeC, float4* nodeN, double* m64, uint32_t* n_skip,
+                          int use64) {
+#pragma clang fp contract(off)
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t v = lb + i;
+  uint32_t nch = n_nchild[v];
+  float mass = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
+  double dm = 0.0, dcx = 0.0, dcy = 0.0, dcz = 0.0, dnx = 0.0, dny = 0.0, dnz = 0.0;
+  if (nch == 0) {
+    for (uint32_t j = n_start[v]; j < n_end[v]; ++j) {
+      float4 p = pts[j];
+      float4 q = nrm[j];
+      mass += p.w;
+      cx += p.w * p.x;
+      cy += p.w * p.y;
+      cz += p.w * p.z;
+      nx += p.w * q.x;
+      ny += p.w * q.y;
+      nz += p.w * q.z;
+      double w = p.w;
+      dm += w;
+      dcx += w * p.x;
+      dcy += w * p.y;
+      dcz += w * p.z;
+      dnx += w * q.x;
+      dny += w * q.y;
+      dnz += w * q.z;
+    }
+    n_skip[v] = v;
+  } else {
+    uint32_t f = n_first[v];
+    for (uint32_t j = 0; j < nch; ++j) {
+      float4 c = nodeC[f + j];
+      float4 q = nodeN[f + j];
+      mass += c.w;
+      cx += c.w * c.x;
+      cy += c.w * c.y;
+      cz += c.w * c.z;
+      nx += c.w * q.x;
+      ny += c.w * q.y;
+      nz += c.w * q.z;
+      const double* cm = m64 + 8ull * (f + j);
+      dm += cm[3];
+      dcx += cm[3] * cm[0];
+      dcy += cm[3] * cm[1];
+      dcz += cm[3] * cm[2];
+      dnx += cm[3] * cm[4];
+      dny += cm[3] * cm[5];
+      dnz += cm[3] * cm[6];
+    }
+    n_skip[v] = (nch == 1) ? n_skip[f] : v;
+  }
+  cx /= mass;
+  cy /= mass;
+  cz /= mass;
+  nx /= mass;
+  ny /= mass;
+  nz /= mass;
+  double* o = m64 + 8ull * v;
+  o[0] = dcx / dm;
+  o[1] = dcy / dm;
+  o[2] = dcz / dm;
+  o[3] = dm;
+  o[4] = dnx / dm;
+  o[5] = dny / dm;
+  o[6] = dnz / dm;
+  o[7] = 0.0;
+  if (use64) {
+    nodeC[v] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)dm);
+    nodeN[v] = make_float4((float)o[4], (float)o[5], (float)o[6], 0.f);
+  } else {
+    nodeC[v] = make_float4(cx, cy, cz, mass);
+    nodeN[v] = make_float4(nx, ny, nz, 0.f);
+  }
+}
+
+__global__ void k_finalize(uint32_t nn, const uint32_t* n_start, const uint32_t* n_end,
+                           const uint32_t* n_first, const uint32_t* n_nchild,
+                           const uint32_t* n_next, const float* n_diam, uint4* nodeL) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  uint32_t nch = n_nchild[v];
+  uint4 r;
+  r.x = __float_as_uint(n_diam[v]);
+  r.z = n_next[v];
+  if (nch == 0) {
+    r.y = n_start[v];
+    r.w = kLeafBit | (n_end[v] - n_start[v]);
+  } else {
+    r.y = n_first[v];
+    r.w = nch;
+  }
+  nodeL[v] = r;
+}
+
+template <int D>
+__global__ void k_set_root(uint32_t n, float4 lo, float4 hi, uint32_t* n_start, uint32_t* n_end,
+                           uint32_t* n_parent, int32_t* n_depth, uint32_t* n_next, float4* n_lo,
+                           float4* n_hi, float* n_diam) {
+#pragma clang fp contract(off)
+  float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  float diam = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));  // include/tree3d.h:249
+  n_start[0] = 0;
+  n_end[0] = n;
+  n_parent[0] = 0;
+  n_depth[0] = 0;
+  n_next[0] = kEnd;
+  n_lo[0] = lo;
+  n_hi[0] = hi;
+  n_diam[0] = diam;
+}
+
+// ---------------------------------------------------------------------------
+// Device exclusive scan (1024 values per block, recursive over block sums).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                     uint32_t* sums) {
+  __shared__ uint32_t wsum[4];
+  uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t v[4];
+  uint32_t loc = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = (base + j < n) ? in[base + j] : 0u;
+    loc += v[j];
+  }
+  uint32_t total;
+  uint32_t pre = block_exclusive_scan<256>(loc, wsum, total);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (base + j < n) out[base + j] = pre;
+    pre += v[j];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / 1024];
+}
+
+__global__ void k_copy_u32(const uint32_t* src, uint32_t* dst) { *dst = *src; }
+
+void scan_rec(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev,
+              uint32_t* scratch, hipStream_t s) {
+  uint32_t nb = (n + 1023) / 1024;
+  uint32_t* sums = scratch;
+  k_scan_blocks<<<nb, 256, 0, s>>>(in, out, n, sums);
+  SBH_HIP(hipGetLastError());
+  if (nb == 1) {
+    if (total_dev) k_copy_u32<<<1, 1, 0, s>>>(sums, total_dev);
+    return;
+  }
+  scan_rec(sums, sums, nb, total_dev, scratch + nb, s);
+  k_scan_add<<<(n + 255) / 256, 256, 0, s>>>(out, n, sums);
+  SBH_HIP(hipGetLastError());
+}
+
+size_t scan_scratch_size(uint32_t n) {
+  size_t tot = 0;
+  do {
+    n = (n + 1023) / 1024;
+    tot += n;
+  } while (n > 1);
+  return tot + 4;
+}
+
+template <typename T>
+void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+  SBH_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
+  SBH_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipStreamSynchronize(s));
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+template <int D>
+void build_impl(DeviceTree& t, const float* points, const float* normals, const float* masses) {
+  hipStream_t s = t.stream;
+  const uint32_t N = t.n;
+  const int k = t.split_log;
+  const uint32_t B = 1u << (D * k);
+
+  // Inputs as {x, y, z, m} / {nx, ny, nz, 0} rows in input order.
+  std::vector<float4> hp(N), hn(N);
+  for (uint32_t i = 0; i < N; ++i) {
+    hp[i] = make_float4(points[(size_t)i * D], points[(size_t)i * D + 1],
+                        D == 3 ? points[(size_t)i * D + 2] : 0.f, masses[i]);
+    if (normals)
+      hn[i] = make_float4(normals[(size_t)i * D], normals[(size_t)i * D + 1],
+                          D == 3 ? normals[(size_t)i * D + 2] : 0.f, 0.f);
+    else
+      hn[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  DevBuf<float4> src_pts(N), src_nrm(N);
+  h2d(src_pts.get(), hp.data(), N, s);
+  h2d(src_nrm.get(), hn.data(), N, s);
+  std::vector<uint32_t> iota(N);
+  for (uint32_t i = 0; i < N; ++i) iota[i] = i;
+  t.perm.alloc(N);
+  h2d(t.perm.get(), iota.data(), N, s);
+
+  // Node arrays with growable capacity.
+  size_t cap = std::max<size_t>(16, (size_t)N * 2 + 2);
+  t.n_start.alloc(cap);
+  t.n_end.alloc(cap);
+  t.n_parent.alloc(cap);
+  t.n_first.alloc(cap);
+  t.n_nchild.alloc(cap);
+  t.n_next.alloc(cap);
+  t.n_depth.alloc(cap);
+  t.n_lo.alloc(cap);
+  t.n_hi.alloc(cap);
+  t.n_diam.alloc(cap);
+
+  auto grow_all = [&](size_t need, size_t keep) {
+    if (need <= cap) return;
+    size_t nc = std::max(need, cap * 2);
+    t.n_start.grow(nc, keep, s);
+    t.n_end.grow(nc, keep, s);
+    t.n_parent.grow(nc, keep, s);
+    t.n_first.grow(nc, keep, s);
+    t.n_nchild.grow(nc, keep, s);
+    t.n_next.grow(nc, keep, s);
+    t.n_depth.grow(nc, keep, s);
+    t.n_lo.grow(nc, keep, s);
+    t.n_hi.grow(nc, keep, s);
+    t.n_diam.grow(nc, keep, s);
+    cap = nc;
+  };
+
+  // Scratch.
+  DevBuf<uint32_t> small_list(N + 1), large_list(N / kSmallMax + 2), counters(4);
+  DevBuf<uint4> temp_child(N + 1);
+  DevBuf<uint32_t> cbase(N + 1), total_dev(1);
+  DevBuf<uint32_t> scan_scratch(scan_scratch_size(N + 1));
+  DevBuf<uint16_t> digit(N + 1);
+  DevBuf<uint32_t> Lg(N + 1), Rg(N + 1);
+
+  float4 rlo = make_float4(t.lo[0], t.lo[1], D == 3 ? t.lo[2] : 0.f, 0.f);
+  float4 rhi = make_float4(t.hi[0], t.hi[1], D == 3 ? t.hi[2] : 0.f, 0.f);
+  k_set_root<D><<<1, 1, 0, s>>>(N, rlo, rhi, t.n_start.get(), t.n_end.get(), t.n_parent.get(),
+                                t.n_depth.get(), t.n_next.get(), t.n_lo.get(), t.n_hi.get(),
+                                t.n_diam.get());
+  SBH_HIP(hipGetLastError());
+
+  t.level_begin.clear();
+  uint32_t lb = 0, lcount = 1, num_nodes = 1;
+  std::vector<uint32_t> h_large, h_lstart, h_lend, h_chunk0, h_nchunk;
+  std::vector<uint4> h_chunks;
+  while (lcount > 0) {
+    t.level_begin.push_back(lb);
+    SBH_HIP(hipMemsetAsync(counters.get(), 0, 4 * sizeof(uint32_t), s));
+    k_classify<<<blocks_for(lcount, 256), 256, 0, s>>>(
+        lb, lcount, t.n_start.get(), t.n_end.get(), t.n_depth.get(), t.leaf_size, t.max_depth,
+        t.n_nchild.get(), t.n_first.get(), small_list.get(), large_list.get(), counters.get());
+    SBH_HIP(hipGetLastError());
+    uint32_t hc[4];
+    d2h(hc, counters.get(), 4, s);
+    const uint32_t nsmall = hc[0], nlarge = hc[1];
+
+    if (nlarge > 0) {
+      h_large.resize(nlarge);
+      d2h(h_large.data(), large_list.get(), nlarge, s);
+      h_lstart.resize(nlarge);
+      h_lend.resize(nlarge);
+      // Few large nodes per level: fetch their ranges one by one.
+      for (uint32_t li = 0; li < nlarge; ++li) {
+        SBH_HIP(hipMemcpyAsync(&h_lstart[li], t.n_start.get() + h_large[li], 4,
+                               hipMemcpyDeviceToHost, s));
+        SBH_HIP(hipMemcpyAsync(&h_lend[li], t.n_end.get() + h_large[li], 4,
+                               hipMemcpyDeviceToHost, s));
+      }
+      SBH_HIP(hipStreamSynchronize(s));
+      h_chunks.clear();
+      h_chunk0.resize(nlarge);
+      h_nchunk.resize(nlarge);
+      for (uint32_t li = 0; li < nlarge; ++li) {
+        h_chunk0[li] = (uint32_t)h_chunks.size();
+        for (uint32_t p = h_lstart[li]; p < h_lend[li]; p += kChunk)
+          h_chunks.push_back(make_uint4(li, p, std::min(p + kChunk, h_lend[li]), h_large[li]));
+        h_nchunk[li] = (uint32_t)h_chunks.size() - h_chunk0[li];
+      }
+      const uint32_t nchunks = (uint32_t)h_chunks.size();
+      DevBuf<uint4> d_chunks(nchunks);
+      DevBuf<uint32_t> d_lstart(nlarge), d_lend(nlarge), d_chunk0(nlarge), d_nchunk(nlarge);
+      DevBuf<uint32_t> d_hist((size_t)nlarge * B), d_cellstart((size_t)nlarge * B),
+          d_m(nlarge);
+      DevBuf<uint2> d_ccnt(nchunks), d_coff(nchunks);
+      h2d(d_chunks.get(), h_chunks.data(), nchunks, s);
+      h2d(d_lstart.get(), h_lstart.data(), nlarge, s);
+      h2d(d_lend.get(), h_lend.data(), nlarge, s);
+      h2d(d_chunk0.get(), h_chunk0.data(), nlarge, s);
+      h2d(d_nchunk.get(), h_nchunk.data(), nlarge, s);
+      SBH_HIP(hipMemsetAsync(d_hist.get(), 0, d_hist.bytes(), s));
+      LargeCtx cx;
+      cx.chunks = d_chunks.get();
+      cx.l_start = d_lstart.get();
+      cx.l_end = d_lend.get();
+      cx.hist = d_hist.get();
+      cx.cellstart = d_cellstart.get();
+      cx.mcount = d_m.get();
+      cx.ccnt = d_ccnt.get();
+      cx.coff = d_coff.get();
+      cx.l_chunk0 = d_chunk0.get();
+      cx.l_nchunk = d_nchunk.get();
+      cx.digit = digit.get();
+      cx.Lg = Lg.get();
+      cx.Rg = Rg.get();
+      k_large_digits<D><<<nchunks, kChunkThreads, 0, s>>>(cx, t.n_lo.get(), t.n_hi.get(),
+                                                          t.perm.get(), src_pts.get(), k);
+      SBH_HIP(hipGetLastError());
+      k_large_cells<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, large_list.get(), nlarge, B,
+                                                         t.n_nchild.get(), temp_child.get());
+      SBH_HIP(hipGetLastError());
+      for (uint32_t c = 0; c < B; ++c) {
+        k_large_count<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c);
+        k_large_offsets<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, nlarge);
+        k_large_rank<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c);
+        k_large_swap<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c, t.perm.get());
+      }
+      SBH_HIP(hipGetLastError());
+      SBH_HIP(hipStreamSynchronize(s));  // scratch buffers of this scope are freed next
+    }
+    if (nsmall > 0) {
+      k_partition_small<D><<<nsmall, kSmallThreads, 0, s>>>(
+          small_list.get(), t.n_start.get(), t.n_end.get(), t.n_lo.get(), t.n_hi.get(),
+          t.perm.get(), src_pts.get(), k, t.n_nchild.get(), temp_child.get());
+      SBH_HIP(hipGetLastError());
+    }
+    // Breadth-first numbering of the next level.
+    scan_rec(t.n_nchild.get() + lb, cbase.get(), lcount, total_dev.get(), scan_scratch.get(), s);
+    uint32_t nnext = 0;
+    d2h(&nnext, total_dev.get(), 1, s);
+    grow_all((size_t)num_nodes + nnext, num_nodes);
+    k_emit<D><<<blocks_for(lcount, 128), 128, 0, s>>>(
+        lb, lcount, num_nodes, cbase.get(), k, t.n_start.get(), t.n_end.get(), t.n_parent.get(),
+        t.n_first.get(), t.n_nchild.get(), t.n_next.get(), t.n_depth.get(), t.n_lo.get(),
+        t.n_hi.get(), t.n_diam.get(), temp_child.get());
+    SBH_HIP(hipGetLastError());
+    lb = num_nodes;
+    lcount = nnext;
+    num_nodes += nnext;
+  }
+  t.level_begin.push_back(num_nodes);
+  t.num_nodes = num_nodes;
+  t.depth = (int)t.level_begin.size() - 2;
+
+  // Points in leaf order.
+  t.pts.alloc(N);
+  t.nrm.alloc(N);
+  k_gather_points<<<blocks_for(N, 256), 256, 0, s>>>(N, t.perm.get(), src_pts.get(),
+                                                     src_nrm.get(), t.pts.get(), t.nrm.get());
+  SBH_HIP(hipGetLastError());
+
+  // Moments, deepest level first.
+  t.nodeC.alloc(num_nodes);
+  t.nodeN.alloc(num_nodes);
+  t.nodeL.alloc(num_nodes);
+  t.n_m64.alloc((size_t)num_nodes * 8);
+  t.n_skip.alloc(num_nodes);
+  for (int l = t.depth; l >= 0; --l) {
+    uint32_t b = t.level_begin[l], cnt = t.level_begin[l + 1] - b;
+    k_moments<D><<<blocks_for(cnt, 128), 128, 0, s>>>(
+        b, cnt, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(), t.pts.get(),
+        t.nrm.get(), t.nodeC.get(), t.nodeN.get(), t.n_m64.get(), t.n_skip.get(), t.moments64);
+    SBH_HIP(hipGetLastError());
+  }
+  k_finalize<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
+      num_nodes, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(),
+      t.n_next.get(), t.n_diam.get(), t.nodeL.get());
+  SBH_HIP(hipGetLastError());
+  SBH_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+size_t exclusive_scan_scratch(uint32_t n) { return scan_scratch_size(n); }
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev,
+                    uint32_t* scratch, hipStream_t s) {
+  scan_rec(in, out, n, total_dev, scratch, s);
+}
+
+void build_device_tree(DeviceTree& t, const float* points, const float* normals,
+                       const float* masses, float* build_ms) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (t.dim == 2)
+    build_impl<2>(t, points, normals, masses);
+  else
+    build_impl<3>(t, points, normals, masses);
+  auto t1 = std::chrono::steady_clock::now();
+  if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+}
+
+}  // namespace sbh

@@ -1,0 +1,30 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "stochastic-barnes-hut_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+def gpu_available() -> bool:
+    try:
+        import torch  # noqa: F401  (only to ask the runtime whether a GPU is visible)
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def sbh_gpu():
+    """The sbh_gpu plugin; on a GPU box it must be the native HIP build."""
+    import sbh_gpu as m
+    assert m.device_count() >= 1, "no GPU visible to libsbh_hip.so"
+    return m

@@ -1,0 +1,110 @@
+"""CPU-only checks of the boundary: libsbh_hip.so loads and exports every
+function include/sbh_hip.h declares; argument validation happens on the host
+before any device work (so it is testable without a GPU); the plugin modules
+import and keep the reference's names."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import PKG, ROOT, gpu_available
+
+HEADER = os.path.join(ROOT, "include", "sbh_hip.h")
+LIB = os.path.join(PKG, "libsbh_hip.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|const char\*)\s+(sbh_\w+)\s*\(", text,
+                                 flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 12
+    lib = ctypes.CDLL(LIB)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_version_and_last_error():
+    lib = ctypes.CDLL(LIB)
+    lib.sbh_version.restype = ctypes.c_int32
+    assert lib.sbh_version() == 100
+    lib.sbh_last_error.restype = ctypes.c_char_p
+    assert isinstance(lib.sbh_last_error(), bytes)
+
+
+def _create(lib, dim=3, k=1, leaf=1, depth=100, pts=None, m=None, lo=None, hi=None, alpha=1.0):
+    n = 0 if pts is None else len(pts)
+    pts = np.zeros((max(n, 1), dim), np.float32) if pts is None else np.asarray(pts, np.float32)
+    m = np.ones(max(n, 1), np.float32) if m is None else np.asarray(m, np.float32)
+    lo = np.zeros(dim, np.float32) if lo is None else np.asarray(lo, np.float32)
+    hi = np.ones(dim, np.float32) if hi is None else np.asarray(hi, np.float32)
+    h = ctypes.c_void_p()
+    f = lib.sbh_tree_create
+    f.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_int64] + [ctypes.c_void_p] * 5 + [
+        ctypes.c_float, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]
+    return f(dim, k, leaf, depth, n, pts.ctypes.data, None, m.ctypes.data, lo.ctypes.data,
+             hi.ctypes.data, alpha, 0, ctypes.byref(h), None)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(dim=4), dict(k=0), dict(k=4), dict(leaf=0), dict(depth=-1),
+    dict(pts=np.zeros((0, 3))),
+    dict(pts=[[0.5, np.nan, 0.5]]),
+    dict(pts=[[0.5, 0.5, 0.5]], m=[0.0]),
+    dict(pts=[[0.5, 0.5, 0.5]], m=[-1.0]),
+    dict(pts=[[0.5, 0.5, 0.5]], lo=[1, 1, 1], hi=[0, 0, 0]),
+    dict(pts=[[0.5, 0.5, 0.5]], alpha=float("inf")),
+])
+def test_invalid_tree_arguments(kw):
+    lib = ctypes.CDLL(LIB)
+    assert _create(lib, **kw) == 1  # SBH_ERR_INVALID, before touching a device
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_valid_tree_without_gpu_reports_no_device():
+    lib = ctypes.CDLL(LIB)
+    assert _create(lib, pts=[[0.5, 0.5, 0.5]]) == 4  # SBH_ERR_NODEVICE
+
+
+def test_plugin_modules_keep_reference_names():
+    import sbh
+    import sbh_gpu
+
+    for c in ("Quadtree1", "Quadtree2", "Quadtree3", "Octree1", "Octree2", "Octree3"):
+        assert hasattr(sbh, c)
+        assert hasattr(sbh_gpu, c.replace("tree", "treeGPU"))
+    for kern in ("gravity", "wn", "smoothdist"):
+        for fn in ("eval_brute_force_", "eval_barnes_hut_", "multi_level_prefix_control_variate_"):
+            assert hasattr(sbh_gpu, fn + kern)
+    assert sbh.fp == 32 and sbh_gpu.fp == 32
+
+
+def test_host_tree_holds_float32_inputs():
+    import sbh
+
+    rng = np.random.default_rng(0)
+    pts = rng.random((50, 3))
+    t = sbh.Octree2(pts, np.zeros_like(pts), np.ones(50), pts.min(0), pts.max(0), 2.5)
+    assert t.dim == 3 and t.split_log == 2 and t.num_points == 50
+    np.testing.assert_array_equal(t.points, pts.astype(np.float32))
+    assert t.alpha == 2.5
+    with pytest.raises(ValueError):
+        sbh.Octree1(pts[:, :2], np.zeros((50, 2)), np.ones(50), [0, 0], [1, 1])
+    with pytest.raises(TypeError):
+        sbh.eval_barnes_hut_gravity(t, pts, 2.0)
+
+
+def test_gpu_tree_validation_errors_on_host():
+    import sbh_gpu
+
+    pts = np.random.default_rng(1).random((20, 3))
+    with pytest.raises(ValueError):
+        sbh_gpu.OctreeGPU1(pts, np.zeros_like(pts), -np.ones(20), [0, 0, 0], [1, 1, 1])
+    with pytest.raises(ValueError):
+        sbh_gpu.OctreeGPU1(pts[:, :2], np.zeros((20, 2)), np.ones(20), [0, 0, 0], [1, 1, 1])

@@ -1,0 +1,226 @@
+"""Parity of the HIP path (through sbh_gpu -> libsbh_hip.so) with the CPU
+oracle on the same seeded inputs.
+
+Bars (from the build's contract):
+  * tree: leaf order, node ranges / numbering / bounds / diameters, and the
+    fp32 node moments are bit-exact;
+  * brute force and Barnes-Hut: within 1e-5 relative of the oracle
+    (denominator max(|y|, 1) for the winding number, |y| otherwise);
+  * stochastic: the same seed gives the same per-query estimate within 1e-5
+    relative (shared Philox paths), and the same evaluation counts.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+KERNELS = ["gravity", "wn", "smoothdist"]
+
+
+def rel_err(got, ref, kernel):
+    ref = np.asarray(ref, np.float64)
+    den = np.maximum(np.abs(ref), 1.0) if kernel == "wn" else np.abs(ref)
+    return np.abs(np.asarray(got, np.float64) - ref) / den
+
+
+def gpu_class(sbh_gpu, dim, k):
+    return getattr(sbh_gpu, ("OctreeGPU%d" if dim == 3 else "QuadtreeGPU%d") % k)
+
+
+def make_pair(sbh_gpu, pts, nrm, m, lo, hi, k, leaf_size=1, alpha=1.0, moments64=False):
+    dim = pts.shape[1]
+    print(f"[case] tree dim={dim} k={k} n={len(pts)} leaf={leaf_size}", flush=True)
+    g = gpu_class(sbh_gpu, dim, k)(pts, nrm, m, lo, hi, alpha=alpha, leaf_size=leaf_size,
+                                   moments64=moments64)
+    o = oracle.Tree(pts, nrm, m, lo, hi, alpha=alpha, split_log=k, leaf_size=leaf_size)
+    return g, o
+
+
+def assert_tree_equal(g, o):
+    go = g.leaf_order()
+    np.testing.assert_array_equal(go, o.leaf_order())
+    gn, on = g.nodes(), o.nodes()
+    assert len(gn["start"]) == len(on["start"])
+    for key in ("start", "end", "nchild", "depth"):
+        np.testing.assert_array_equal(gn[key].astype(np.int64), on[key].astype(np.int64), key)
+    leaf = on["leaf"].astype(bool)
+    np.testing.assert_array_equal(gn["first_child"][~leaf].astype(np.int64),
+                                  on["first_child"][~leaf])
+    np.testing.assert_array_equal(gn["parent"][1:].astype(np.int64), on["parent"][1:])
+    for key in ("bmin", "bmax", "diam", "centroid", "mass", "normal"):
+        np.testing.assert_array_equal(gn[key].view(np.uint32), on[key].view(np.uint32), key)
+    m64 = gn["moments64"]
+    np.testing.assert_allclose(m64[:, 3], on["mass64"], rtol=1e-12)
+    np.testing.assert_allclose(m64[:, :o.dim], on["centroid64"], rtol=1e-9, atol=1e-12)
+
+
+TREE_CASES = [
+    ("cube3d_k1", 3, 1, 1, 3000),
+    ("cube3d_k2", 3, 2, 1, 6000),
+    ("cube3d_k3", 3, 3, 1, 5000),
+    ("cube3d_k1_leaf8", 3, 1, 8, 16384),
+    ("cube2d_k1", 2, 1, 1, 5000),
+    ("cube2d_k2_leaf4", 2, 2, 4, 9000),
+    ("plummer_k1_large", 3, 1, 1, 40000),
+    ("plummer_k2_large", 3, 2, 1, 40000),
+]
+
+
+@pytest.mark.parametrize("name,dim,k,leaf,n", TREE_CASES, ids=[c[0] for c in TREE_CASES])
+def test_tree_bit_exact(sbh_gpu, name, dim, k, leaf, n):
+    if name.startswith("plummer"):
+        pts, nrm, m, _ = W.plummer(n, 1, rmax=50.0)
+        m = np.random.default_rng(5).uniform(0.5, 1.5, n)
+    else:
+        pts, nrm, m, _ = W.uniform_cube(n, 1, dim=dim, seed=n)
+        nrm = np.random.default_rng(9).normal(size=pts.shape)
+    lo, hi = W.bounds(pts)
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, k, leaf_size=leaf)
+    assert_tree_equal(g, o)
+    assert g.info()["num_nodes"] == o.num_nodes
+    assert g.info()["depth"] == o.depth
+
+
+def test_tree_duplicates_and_max_depth(sbh_gpu):
+    """Coincident points never separate: leaves hit max_depth=100 and hold
+    several points, whose internal order comes from the partition passes."""
+    rng = np.random.default_rng(11)
+    base = rng.random((300, 3))
+    pts = np.concatenate([base, base[:40], base[:40], base[5:9]])
+    m = rng.uniform(0.5, 1.5, len(pts))
+    lo, hi = W.bounds(pts)
+    g, o = make_pair(sbh_gpu, pts, np.zeros_like(pts), m, lo, hi, 1)
+    assert o.depth == 100
+    assert_tree_equal(g, o)
+
+
+def test_tree_single_point_and_root_leaf(sbh_gpu):
+    pts = np.array([[0.25, 0.5, 0.75]])
+    g, o = make_pair(sbh_gpu, pts, np.zeros_like(pts), np.ones(1), [0, 0, 0], [1, 1, 1], 1)
+    assert_tree_equal(g, o)
+    pts = np.random.default_rng(2).random((6, 3))
+    g, o = make_pair(sbh_gpu, pts, np.zeros_like(pts), np.ones(6), [0, 0, 0], [1, 1, 1], 2,
+                     leaf_size=8)
+    assert_tree_equal(g, o)
+    q = np.random.default_rng(3).random((70, 3))
+    for kern in KERNELS:
+        fn = getattr(sbh_gpu, "multi_level_prefix_control_variate_" + kern)
+        got, _ = fn(g, q, 4)
+        ref = o.stochastic(kern, q, 4)
+        assert rel_err(got, ref, kern).max() <= TOL
+
+
+def _cube(n, q, dim, seed, kernel):
+    pts, nrm, m, queries = W.uniform_cube(n, q, dim=dim, seed=seed)
+    if kernel == "wn":
+        nrm = np.random.default_rng(seed + 1).normal(size=pts.shape)
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    lo, hi = W.bounds(pts)
+    return pts, nrm, m, queries, lo, hi
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_brute_force(sbh_gpu, dim, kernel):
+    pts, nrm, m, q, lo, hi = _cube(7000, 1001, dim, 21, kernel)
+    alpha = 10.0 if kernel == "smoothdist" else 1.0
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1, alpha=alpha)
+    got, ms = getattr(sbh_gpu, "eval_brute_force_" + kernel)(g, q)
+    ref = o.brute_force(kernel, q)
+    ref64 = o.brute_force(kernel, q, fp64=True)
+    assert got.dtype == np.float32 and got.shape == (len(q),)
+    assert ms > 0
+    assert rel_err(got, ref, kernel).max() <= TOL
+    assert rel_err(got, ref64, kernel).max() <= TOL
+    g64, _ = getattr(sbh_gpu, "eval_brute_force_f64_" + kernel)(g, q)
+    assert rel_err(g64, ref64, kernel).max() <= 1e-12
+
+
+@pytest.mark.parametrize("vote", [1, 32, 64])
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("dim,k", [(3, 1), (3, 2), (2, 1)])
+def test_barnes_hut(sbh_gpu, vote, kernel, dim, k):
+    pts, nrm, m, q, lo, hi = _cube(6000, 777, dim, 31 + k, kernel)
+    alpha = 10.0 if kernel == "smoothdist" else 1.0
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, k, alpha=alpha)
+    fn = getattr(sbh_gpu, "eval_barnes_hut_" + kernel)
+    for beta in (1.5, 2.0, 3.0, np.inf):
+        got, ms, cnt = fn(g, q, beta, vote_group=vote, return_counts=True)
+        ref, rcnt = o.barnes_hut(kernel, q, beta, vote_group=vote, counts=True)
+        np.testing.assert_array_equal(cnt, rcnt)
+        assert rel_err(got, ref, kernel).max() <= TOL, (beta, vote)
+
+
+def test_barnes_hut_sorted_queries_identical(sbh_gpu):
+    pts, nrm, m, q, lo, hi = _cube(20000, 5000, 3, 41, "gravity")
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1)
+    a, _ = sbh_gpu.eval_barnes_hut_gravity(g, q, 2.0, vote_group=1)
+    b, _ = sbh_gpu.eval_barnes_hut_gravity(g, q, 2.0, vote_group=1, sort_queries=True)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("dim,k", [(3, 2), (3, 1), (2, 2)])
+def test_stochastic_same_seed(sbh_gpu, kernel, dim, k):
+    pts, nrm, m, q, lo, hi = _cube(6000, 300, dim, 51 + k, kernel)
+    alpha = 10.0 if kernel == "smoothdist" else 1.0
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, k, alpha=alpha)
+    fn = getattr(sbh_gpu, "multi_level_prefix_control_variate_" + kernel)
+    for spl, seed in ((1, 0), (4, 7), (3, 2**40 + 5)):
+        got, ms, cnt = fn(g, q, spl, seed=seed, return_counts=True)
+        ref, rcnt = o.stochastic(kernel, q, spl, seed=seed, group=64, counts=True)
+        np.testing.assert_array_equal(cnt, rcnt)
+        assert rel_err(got, ref, kernel).max() <= TOL, (spl, seed)
+
+
+def test_stochastic_unbiased_and_variance(sbh_gpu):
+    """Mean over 256 seeds within 3 standard errors of brute force (pooled z
+    test plus a per-query bound with multiple-testing slack), and per-query
+    variance within 10% of the oracle estimator's at the same sample budget."""
+    pts, nrm, m, q, lo, hi = _cube(4096, 128, 3, 61, "gravity")
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 2)
+    exact = o.brute_force("gravity", q, fp64=True)
+    seeds = 256
+    est = np.stack([sbh_gpu.multi_level_prefix_control_variate_gravity(g, q, 1, seed=s)[0]
+                    for s in range(seeds)]).astype(np.float64)
+    mean, sd = est.mean(0), est.std(0, ddof=1)
+    se = sd / np.sqrt(seeds)
+    z = (mean - exact) / np.where(se > 0, se, 1.0)
+    assert abs(z.mean()) * np.sqrt(len(z)) <= 3.0 * max(1.0, z.std())
+    assert np.abs(z).max() <= 4.5  # 128 queries: per-query 3-SE bound with Bonferroni slack
+    # oracle estimator, independent seeds, same budget
+    ests_o = np.stack([o.stochastic("gravity", q, 1, seed=10_000 + s) for s in range(seeds)])
+    var_g = est.var(0, ddof=1).mean()
+    var_o = ests_o.astype(np.float64).var(0, ddof=1).mean()
+    assert abs(var_g / var_o - 1.0) <= 0.10
+
+
+def test_ragged_and_empty_queries(sbh_gpu):
+    pts, nrm, m, q, lo, hi = _cube(3000, 130, 3, 71, "gravity")
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 2)
+    for nq in (0, 1, 63, 65, 130):
+        qq = q[:nq]
+        a, _ = sbh_gpu.eval_brute_force_gravity(g, qq)
+        b, _ = sbh_gpu.eval_barnes_hut_gravity(g, qq, 2.0)
+        c, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, qq, 2)
+        assert a.shape == b.shape == c.shape == (nq,)
+        if nq:
+            assert rel_err(a, o.brute_force("gravity", qq), "gravity").max() <= TOL
+            assert rel_err(b, o.barnes_hut("gravity", qq, 2.0, vote_group=32), "gravity").max() <= TOL
+            assert rel_err(c, o.stochastic("gravity", qq, 2), "gravity").max() <= TOL
+
+
+def test_host_tree_constructor(sbh_gpu):
+    import sbh
+
+    pts, nrm, m, q, lo, hi = _cube(2000, 100, 3, 81, "gravity")
+    cpu = sbh.Octree2(pts, nrm, m, lo, hi)
+    g = sbh_gpu.OctreeGPU2(cpu)
+    o = oracle.Tree(pts, nrm, m, lo, hi, split_log=2)
+    assert_tree_equal(g, o)
+    with pytest.raises(TypeError):
+        sbh_gpu.OctreeGPU1(cpu)
