@@ -140,6 +140,13 @@ class Tree:
                             _ptr(out), nthreads)
         return out if fp64 else out.astype(np.float32)
 
+    def abs_sum(self, kernel, queries, nthreads=0) -> np.ndarray:
+        """float64 sum of |term| per query: the condition of the sum."""
+        q = self._queries(queries)
+        out = np.empty(len(q), np.float64)
+        _lib.ot_brute_force(self._h, kernel_id(kernel), 2, _ptr(q), len(q), _ptr(out), nthreads)
+        return out
+
     def barnes_hut(self, kernel, queries, beta, vote_group=1, nthreads=0, counts=False):
         q = self._queries(queries)
         out = np.empty(len(q), np.float32)

@@ -440,7 +440,8 @@ void build_tree(Tree<D>& t, const Box<D>& root_box) {
 
 // ---------------------------------------------------------------------------
 // Brute force.  mode 0: fp32 Kahan in leaf order, include/brute_force_gpu.h:45-63.
-// mode 1: float64 kernels summed in float64.
+// mode 1: float64 kernels summed in float64.  mode 2: float64 sum of |term|
+// (the condition of the sum, used to bound rounding error).
 // ---------------------------------------------------------------------------
 template <int D>
 void brute_force(const Tree<D>& t, int kernel, int mode, const float* Q, int64_t nq,
@@ -459,9 +460,14 @@ void brute_force(const Tree<D>& t, int kernel, int mode, const float* Q, int64_t
           total = tt;
         }
         out[qi] = total;
-      } else {
+      } else if (mode == 1) {
         double total = 0.0;
         for (size_t j = 0; j < t.recs.size(); ++j) total += k64<D>(kernel, t.rec_src(j), q);
+        out[qi] = total;
+      } else {
+        double total = 0.0;
+        for (size_t j = 0; j < t.recs.size(); ++j)
+          total += std::fabs(k64<D>(kernel, t.rec_src(j), q));
         out[qi] = total;
       }
     }

@@ -114,11 +114,31 @@ def test_tree_single_point_and_root_leaf(sbh_gpu):
         assert rel_err(got, ref, kern).max() <= TOL
 
 
+def _ring(n, q, seed):
+    """2D winding-number workload: dipoles on a jittered circle
+    (python/sampler.py:5-20), queries in the bounding square."""
+    rng = np.random.default_rng(seed)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    r = 0.4 * (1 + 0.05 * rng.normal(size=n))
+    pts = r[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+    nxt = np.roll(pts, -1, axis=0)
+    e = nxt - pts
+    nrm = np.stack([e[:, 1], -e[:, 0]], 1)
+    length = np.linalg.norm(nrm, axis=1)
+    nrm /= length[:, None]
+    queries = rng.uniform(-0.6, 0.6, (q, 2))
+    return pts, nrm, length, queries
+
+
 def _cube(n, q, dim, seed, kernel):
-    pts, nrm, m, queries = W.uniform_cube(n, q, dim=dim, seed=seed)
     if kernel == "wn":
-        nrm = np.random.default_rng(seed + 1).normal(size=pts.shape)
-        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        if dim == 3:
+            level = 3 if n <= 2000 else 4 if n <= 6000 else 5
+            pts, nrm, m, queries = W.noisy_icosphere(level, q, seed=seed)
+        else:
+            pts, nrm, m, queries = _ring(n, q, seed)
+    else:
+        pts, nrm, m, queries = W.uniform_cube(n, q, dim=dim, seed=seed)
     lo, hi = W.bounds(pts)
     return pts, nrm, m, queries, lo, hi
 
@@ -138,6 +158,19 @@ def test_brute_force(sbh_gpu, dim, kernel):
     assert rel_err(got, ref64, kernel).max() <= TOL
     g64, _ = getattr(sbh_gpu, "eval_brute_force_f64_" + kernel)(g, q)
     assert rel_err(g64, ref64, kernel).max() <= 1e-12
+
+
+def test_brute_force_ill_conditioned_wn(sbh_gpu):
+    """Random dipole directions cancel; the error bar is then relative to the
+    sum's condition (sum of |terms|), the bound fp32 terms can meet."""
+    pts, _, m, q = W.uniform_cube(6000, 500, seed=23)
+    nrm = np.random.default_rng(24).normal(size=pts.shape)
+    lo, hi = W.bounds(pts)
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1)
+    got, _ = sbh_gpu.eval_brute_force_wn(g, q)
+    ref64 = o.brute_force("wn", q, fp64=True)
+    cond = o.abs_sum("wn", q)
+    assert (np.abs(got - ref64) / cond).max() <= TOL
 
 
 @pytest.mark.parametrize("vote", [1, 32, 64])
