@@ -57,6 +57,8 @@ typedef struct sbh_tree sbh_tree;
 #define SBH_EVAL_SORT_QUERIES 2    /* evaluate in spatial (Morton) order; results are
                                       returned in input order.  Barnes-Hut with
                                       vote_group 1 gives identical values either way */
+#define SBH_EVAL_SPLIT_COUNTS 4    /* counts holds 2q values: node terms in [0, q),
+                                      point terms in [q, 2q) */
 
 typedef struct sbh_eval_params {
   float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */

@@ -288,7 +288,9 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     sbh::DeviceTree& t = tree->t;
     hipStream_t s = t.stream;
     const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
+    const int split = (p.flags & SBH_EVAL_SPLIT_COUNTS) ? 1 : 0;
     const uint32_t nq = (uint32_t)q;
+    const size_t ncnt = split ? 2ull * nq : nq;
     const size_t qfloats = (size_t)nq * t.dim;
     const float* dq = queries;
     float* dout = results;
@@ -301,7 +303,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       dq = tree->q.get();
       dout = tree->out.get();
       if (counts) {
-        ensure(tree->counts, nq);
+        ensure(tree->counts, ncnt);
         dcnt = tree->counts.get();
       }
     }
@@ -315,9 +317,10 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       sbh::run_brute(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);
       SBH_HIP(hipEventRecord(tree->ev1, s));
       if (dcnt) {
-        std::vector<int32_t> c(nq, (int32_t)t.n);
-        SBH_HIP(hipMemcpyAsync(dcnt, c.data(), nq * 4,
-                               dev ? hipMemcpyHostToDevice : hipMemcpyHostToDevice, s));
+        // every query evaluates every point; split counts have no node terms
+        std::vector<int32_t> c(ncnt, (int32_t)t.n);
+        if (split) std::fill(c.begin(), c.begin() + nq, 0);
+        SBH_HIP(hipMemcpyAsync(dcnt, c.data(), ncnt * 4, hipMemcpyHostToDevice, s));
         SBH_HIP(hipStreamSynchronize(s));
       }
     } else {
@@ -331,15 +334,15 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       SBH_HIP(hipEventRecord(tree->ev0, s));
       if (sort) sbh::query_order(t, dq, nq, ord, tree->sort_scratch.get(), s);
       if (method == SBH_BARNES_HUT)
-        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, s);
+        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, s);
       else
         sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed, dq, nq, ord, dout, dcnt,
-                            s);
+                            split, s);
       SBH_HIP(hipEventRecord(tree->ev1, s));
     }
     if (!dev) {
       SBH_HIP(hipMemcpyAsync(results, dout, nq * 4, hipMemcpyDeviceToHost, s));
-      if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, nq * 4, hipMemcpyDeviceToHost, s));
+      if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, ncnt * 4, hipMemcpyDeviceToHost, s));
     }
     SBH_HIP(hipEventSynchronize(tree->ev1));
     if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
                                                   const float* __restrict__ qz, uint32_t nq,
                                                   const uint32_t* __restrict__ order,
                                                   float* __restrict__ out,
-                                                  int32_t* __restrict__ counts) {
+                                                  int32_t* __restrict__ counts, int split_counts) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
   const bool live = slot < nq;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
     my_group = 1ull << lane;
   }
   float acc = 0.0f;
-  int32_t cnt = 0;
+  int32_t cnt = 0, cnt_nodes = 0;
   bool active = live;
   uint32_t resume = kEnd;
   uint32_t v = 0;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
         cnt += (int32_t)(p1 - p0);
       } else {
         acc += kernel_exact<D, K>(c, nodeN[v], alpha, x, y, z);
-        cnt += 1;
+        cnt_nodes += 1;
       }
     }
     if (rej == 0) {
@@ -227,7 +227,14 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
   }
   if (live) {
     out[qi] = acc;
-    if (counts) counts[qi] = cnt;
+    if (counts) {
+      if (split_counts) {
+        counts[qi] = cnt_nodes;
+        counts[nq + qi] = cnt;
+      } else {
+        counts[qi] = cnt + cnt_nodes;
+      }
+    }
   }
 }
 
@@ -247,7 +254,8 @@ __device__ __forceinline__ float node_term(const float4* __restrict__ nodeC,
                                            const float4* __restrict__ nodeN,
                                            const float4* __restrict__ pts,
                                            const float4* __restrict__ nrm, float alpha,
-                                           uint32_t v, float x, float y, float z, int32_t& cnt) {
+                                           uint32_t v, float x, float y, float z, int32_t& cnt,
+                                           int32_t& cnt_nodes) {
   const uint4 l = nodeL[v];
   float s = 0.0f;
   if (l.w & kLeafBit) {
@@ -256,7 +264,7 @@ __device__ __forceinline__ float node_term(const float4* __restrict__ nodeC,
     cnt += (int32_t)(p1 - p0);
   } else {
     s = kernel_exact<D, K>(nodeC[v], nodeN[v], alpha, x, y, z);
-    cnt += 1;
+    cnt_nodes += 1;
   }
   return s;
 }
@@ -269,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
     const float4* __restrict__ pts, const float4* __restrict__ nrm, float alpha, int spl,
     uint32_t key0, uint32_t key1, const float* __restrict__ qx, const float* __restrict__ qy,
     const float* __restrict__ qz, uint32_t nq, const uint32_t* __restrict__ order,
-    float* __restrict__ out, int32_t* __restrict__ counts) {
+    float* __restrict__ out, int32_t* __restrict__ counts, int split_counts) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
@@ -279,17 +287,17 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
   (void)lane;
-  int32_t cnt = 0;
+  int32_t cnt = 0, cnt_nodes = 0;
   const uint4 rl = nodeL[0];
   float result;
   if (rl.w & kLeafBit) {
-    result = node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, 0, x, y, z, cnt);
+    result = node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, 0, x, y, z, cnt, cnt_nodes);
   } else {
     float bh = 0.0f, mlcv = 0.0f;
     const uint32_t nsub = rl.w, first = rl.y;
     for (uint32_t si = 0; si < nsub; ++si) {
       const uint32_t S = uniform_u32(first + si);
-      bh += node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, S, x, y, z, cnt);
+      bh += node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, S, x, y, z, cnt, cnt_nodes);
       const uint4 Sl = nodeL[S];
       const uint32_t sstart = n_start[S];
       const uint32_t nS = n_end[S] - sstart;
@@ -354,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
               float cur_result = 0.0f;
               const uint32_t f = prevL.y, nch = prevL.w;
               for (uint32_t j = 0; j < nch; ++j)
-                cur_result += node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, f + j, x, y, z, cnt);
+                cur_result += node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, f + j, x, y, z, cnt, cnt_nodes);
               const float diff = cur_result - prev_result;
               const float wd = (float)nS / (event_prob * (float)(n_end[prev] - n_start[prev])) * diff;
               wd_all += wd;
@@ -369,7 +377,14 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
   }
   if (live) {
     out[qi] = result;
-    if (counts) counts[qi] = cnt;
+    if (counts) {
+      if (split_counts) {
+        counts[qi] = cnt_nodes;
+        counts[nq + qi] = cnt;
+      } else {
+        counts[qi] = cnt + cnt_nodes;
+      }
+    }
   }
 }
 
@@ -497,31 +512,31 @@ void launch_brute_f64(const DeviceTree& t, const float* qx, const float* qy, con
 template <int D, int K>
 void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, const float* qy,
                const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
-               hipStream_t s) {
+               int split, hipStream_t s) {
   uint32_t grid = nblk(nq, kThreads);
   if (group == 64)
     k_bh<D, K, 64><<<grid, kThreads, 0, s>>>(t.nodeC.get(), t.nodeL.get(), t.nodeN.get(),
                                               t.pts.get(), t.nrm.get(), t.alpha, beta, qx, qy, qz,
-                                              nq, order, out, counts);
+                                              nq, order, out, counts, split);
   else if (group == 32)
     k_bh<D, K, 32><<<grid, kThreads, 0, s>>>(t.nodeC.get(), t.nodeL.get(), t.nodeN.get(),
                                               t.pts.get(), t.nrm.get(), t.alpha, beta, qx, qy, qz,
-                                              nq, order, out, counts);
+                                              nq, order, out, counts, split);
   else
     k_bh<D, K, 1><<<grid, kThreads, 0, s>>>(t.nodeC.get(), t.nodeL.get(), t.nodeN.get(),
                                              t.pts.get(), t.nrm.get(), t.alpha, beta, qx, qy, qz,
-                                             nq, order, out, counts);
+                                             nq, order, out, counts, split);
   SBH_HIP(hipGetLastError());
 }
 
 template <int D, int K>
 void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, const float* qx, const float* qy,
                   const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
-                  hipStream_t s) {
+                  int split, hipStream_t s) {
   k_stochastic<D, K><<<nblk(nq, kThreads), kThreads, 0, s>>>(
       t.nodeC.get(), t.nodeL.get(), t.nodeN.get(), t.n_start.get(), t.n_end.get(),
       t.n_skip.get(), t.pts.get(), t.nrm.get(), t.alpha, spl, (uint32_t)seed,
-      (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out, counts);
+      (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out, counts, split);
   SBH_HIP(hipGetLastError());
 }
 
@@ -584,23 +599,23 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
 }
 
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
-            const uint32_t* order, float* out, int32_t* counts, hipStream_t s) {
+            const uint32_t* order, float* out, int32_t* counts, int split, hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
-    launch_bh<D, K>(t, beta, group, qx, qy, qz, nq, order, out, counts, s);
+    launch_bh<D, K>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, s);
   });
 }
 
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, const float* q,
-                    uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
+                    uint32_t nq, const uint32_t* order, float* out, int32_t* counts, int split,
                     hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
-    launch_stoch<D, K>(t, spl, seed, qx, qy, qz, nq, order, out, counts, s);
+    launch_stoch<D, K>(t, spl, seed, qx, qy, qz, nq, order, out, counts, split, s);
   });
 }
 
