@@ -60,6 +60,13 @@ def main():
                     help="samples per subdomain swept for the stochastic estimator")
     ap.add_argument("--rmax", type=float, default=None,
                     help="truncate the Plummer sphere at this radius (default: untruncated)")
+    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute"], default=None,
+                    help="profiling mode: build the trees, run only this method --steps times "
+                         "(stochastic at --spl[0], Barnes-Hut at --theta) and exit")
+    ap.add_argument("--theta", type=float, default=1.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain device-to-device copy of this many MiB (a counter "
+                         "calibration kernel of known traffic)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,6 +109,31 @@ def main():
     qe = torch.from_numpy(np.ascontiguousarray(q32[:ne].T)).to(dev)
     ref = torch.empty(ne, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+
+    if args.copy_mib:
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_mib} MiB x3 (calibration)")
+        del a, b
+
+    if args.profile_kernel:
+        which = args.profile_kernel
+        for _ in range(args.steps):
+            if which == "stochastic":
+                ms = t2.eval_device(C.STOCHASTIC, "gravity", qd.data_ptr(), nq, out.data_ptr(),
+                                    samples_per_subdomain=args.spl[0], seed=1)
+            elif which == "bh":
+                ms = t1.eval_device(C.BARNES_HUT, "gravity", qd.data_ptr(), nq, out.data_ptr(),
+                                    beta=1.0 / args.theta, vote_group=1,
+                                    flags=C.EVAL_SORT_QUERIES)
+            else:
+                ms = t1.eval_device(C.BRUTE_FORCE, "gravity", qd.data_ptr(), nq, out.data_ptr())
+            log(f"{which}: {ms:.3f} ms")
+        torch.cuda.synchronize()
+        return
 
     log(f"float64 reference on {ne} queries")
     ref_ms = t1.eval_f64_device("gravity", qe.data_ptr(), ne, ref.data_ptr())

@@ -59,6 +59,13 @@ typedef struct sbh_tree sbh_tree;
                                       vote_group 1 gives identical values either way */
 #define SBH_EVAL_SPLIT_COUNTS 4    /* counts holds 2q values: node terms in [0, q),
                                       point terms in [q, 2q) */
+#define SBH_EVAL_EXACT_TERMS 8     /* Barnes-Hut / stochastic: every term with correctly
+                                      rounded fp32 operations in the reference CPU
+                                      plugin's order (bit-reproducible); default is the
+                                      hardware sqrt / reciprocal / exp2 the reference's
+                                      fast-math GPU build uses (CMakeLists.txt:27).
+                                      Opening tests, roulette and weights are exact in
+                                      both modes */
 
 typedef struct sbh_eval_params {
   float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */

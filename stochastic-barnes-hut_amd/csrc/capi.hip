@@ -289,6 +289,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     hipStream_t s = t.stream;
     const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
     const int split = (p.flags & SBH_EVAL_SPLIT_COUNTS) ? 1 : 0;
+    const bool exact = (p.flags & SBH_EVAL_EXACT_TERMS) != 0;
     const uint32_t nq = (uint32_t)q;
     const size_t ncnt = split ? 2ull * nq : nq;
     const size_t qfloats = (size_t)nq * t.dim;
@@ -334,10 +335,10 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       SBH_HIP(hipEventRecord(tree->ev0, s));
       if (sort) sbh::query_order(t, dq, nq, ord, tree->sort_scratch.get(), s);
       if (method == SBH_BARNES_HUT)
-        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, s);
+        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, exact, s);
       else
         sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed, dq, nq, ord, dout, dcnt,
-                            split, s);
+                            split, exact, s);
       SBH_HIP(hipEventRecord(tree->ev1, s));
     }
     if (!dev) {

@@ -148,6 +148,17 @@ __global__ void k_reduce_splits_f64(const double* partial, uint32_t nsplit, uint
   out[i] = s;
 }
 
+// One term of a sum: exact (correctly rounded, reference operation order) or
+// fast (hardware sqrt / reciprocal / exp2; alpha_k = alpha * log2(e)).
+template <int D, int K, bool EXACT>
+__device__ __forceinline__ float term(float4 src, float4 nrm, float alpha, float alpha_k,
+                                      float x, float y, float z) {
+  if constexpr (EXACT)
+    return kernel_exact<D, K>(src, nrm, alpha, x, y, z);
+  else
+    return kernel_fast<D, K>(src, nrm, alpha_k, x, y, z);
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic Barnes-Hut (include/barnes_hut.h:23-78 per query,
 // include/barnes_hut_gpu.h:19-72 vote).
@@ -160,15 +171,19 @@ __global__ void k_reduce_splits_f64(const double* partial, uint32_t nsplit, uint
 // sleeps until the wave reaches v's rope target, which is where its own walk
 // resumes.  Each lane therefore adds exactly the terms of its own walk, in the
 // same order.  The node index is wave-uniform, so node records are scalar
-// loads.
+// loads, and the records are stored in depth-first order so that the walk
+// reads forward through memory.  The opening test is always evaluated with
+// correctly rounded operations, so every lane opens exactly the nodes the
+// reference walk opens.
 // ---------------------------------------------------------------------------
-template <int D, int K, int GROUP>
+template <int D, int K, int GROUP, bool EXACT>
 __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ nodeC,
                                                   const uint4* __restrict__ nodeL,
                                                   const float4* __restrict__ nodeN,
                                                   const float4* __restrict__ pts,
                                                   const float4* __restrict__ nrm, float alpha,
-                                                  float beta, const float* __restrict__ qx,
+                                                  float alpha_k, float beta,
+                                                  const float* __restrict__ qx,
                                                   const float* __restrict__ qy,
                                                   const float* __restrict__ qz, uint32_t nq,
                                                   const uint32_t* __restrict__ order,
@@ -206,12 +221,11 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
     if (accept) {
       if (leaf) {
         const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
-        for (uint32_t j = p0; j < p1; ++j) {
-          acc += kernel_exact<D, K>(pts[j], nrm[j], alpha, x, y, z);
-        }
+        for (uint32_t j = p0; j < p1; ++j)
+          acc += term<D, K, EXACT>(pts[j], nrm[j], alpha, alpha_k, x, y, z);
         cnt += (int32_t)(p1 - p0);
       } else {
-        acc += kernel_exact<D, K>(c, nodeN[v], alpha, x, y, z);
+        acc += term<D, K, EXACT>(c, nodeN[v], alpha, alpha_k, x, y, z);
         cnt_nodes += 1;
       }
     }
@@ -245,59 +259,78 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
 // path of sample `ss` in subdomain `si` of wave `g` is drawn from Philox block
 // {ss, si, g, b} under key = seed: words 0,1 give the sampled point index,
 // word 2 + l the Russian-roulette uniform at descent level l.  Each lane runs
-// its own roulette against the shared uniforms.  Arithmetic follows the
-// reference's fp32 order exactly.
+// its own roulette against the shared uniforms; the roulette, the weights and
+// the opening ratios use the reference's fp32 operations exactly.
+//
+// At every level the wavefront stages the children of the current node in
+// LDS with one coalesced load per lane (children are contiguous in
+// breadth-first order) and every lane then reads them as broadcasts.
 // ---------------------------------------------------------------------------
-template <int D, int K>
+constexpr uint32_t kMulti = 1u;  // leaf with more than one point
+
+template <int D, int K, bool EXACT>
 __device__ __forceinline__ float node_term(const float4* __restrict__ nodeC,
                                            const uint4* __restrict__ nodeL,
                                            const float4* __restrict__ nodeN,
                                            const float4* __restrict__ pts,
                                            const float4* __restrict__ nrm, float alpha,
-                                           uint32_t v, float x, float y, float z, int32_t& cnt,
-                                           int32_t& cnt_nodes) {
+                                           float alpha_k, uint32_t v, float x, float y, float z,
+                                           int32_t& cnt, int32_t& cnt_nodes) {
   const uint4 l = nodeL[v];
   float s = 0.0f;
   if (l.w & kLeafBit) {
     const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
-    for (uint32_t j = p0; j < p1; ++j) s += kernel_exact<D, K>(pts[j], nrm[j], alpha, x, y, z);
+    for (uint32_t j = p0; j < p1; ++j) s += term<D, K, EXACT>(pts[j], nrm[j], alpha, alpha_k, x, y, z);
     cnt += (int32_t)(p1 - p0);
   } else {
-    s = kernel_exact<D, K>(nodeC[v], nodeN[v], alpha, x, y, z);
+    s = term<D, K, EXACT>(nodeC[v], nodeN[v], alpha, alpha_k, x, y, z);
     cnt_nodes += 1;
   }
   return s;
 }
 
-template <int D, int K>
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int D, int K, bool EXACT>
 __global__ __launch_bounds__(kThreads) void k_stochastic(
     const float4* __restrict__ nodeC, const uint4* __restrict__ nodeL,
     const float4* __restrict__ nodeN, const uint32_t* __restrict__ n_start,
     const uint32_t* __restrict__ n_end, const uint32_t* __restrict__ n_skip,
-    const float4* __restrict__ pts, const float4* __restrict__ nrm, float alpha, int spl,
-    uint32_t key0, uint32_t key1, const float* __restrict__ qx, const float* __restrict__ qy,
-    const float* __restrict__ qz, uint32_t nq, const uint32_t* __restrict__ order,
-    float* __restrict__ out, int32_t* __restrict__ counts, int split_counts) {
+    const float4* __restrict__ pts, const float4* __restrict__ nrm, float alpha, float alpha_k,
+    int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
+    const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
+    const uint32_t* __restrict__ order, float* __restrict__ out, int32_t* __restrict__ counts,
+    int split_counts) {
 #pragma clang fp contract(off)
+  constexpr bool kWN = (K == WINDING_NUMBER);
+  __shared__ float4 sSrc[kThreads];
+  __shared__ float4 sNrm[kWN ? kThreads : 1];
+  __shared__ uint4 sInfo[kThreads];  // {flags, first point, point count, 0}
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wbase = threadIdx.x & ~63u;
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
   const bool live = slot < nq;
   const uint32_t g = uniform_u32(slot >> 6);
   uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  (void)lane;
   int32_t cnt = 0, cnt_nodes = 0;
   const uint4 rl = nodeL[0];
   float result;
   if (rl.w & kLeafBit) {
-    result = node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, 0, x, y, z, cnt, cnt_nodes);
+    result = node_term<D, K, EXACT>(nodeC, nodeL, nodeN, pts, nrm, alpha, alpha_k, 0, x, y, z,
+                                    cnt, cnt_nodes);
   } else {
     float bh = 0.0f, mlcv = 0.0f;
     const uint32_t nsub = rl.w, first = rl.y;
     for (uint32_t si = 0; si < nsub; ++si) {
       const uint32_t S = uniform_u32(first + si);
-      bh += node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, S, x, y, z, cnt, cnt_nodes);
+      bh += node_term<D, K, EXACT>(nodeC, nodeL, nodeN, pts, nrm, alpha, alpha_k, S, x, y, z,
+                                   cnt, cnt_nodes);
       const uint4 Sl = nodeL[S];
       const uint32_t sstart = n_start[S];
       const uint32_t nS = n_end[S] - sstart;
@@ -339,30 +372,66 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
               }
             }
             if (__ballot(alive) == 0) break;
-            // Child of cur holding sample idx (children are contiguous in
-            // cell order), then down any single-child chain.
             const uint32_t prev = cur;
-            const uint4 prevL = curL;
-            const float4 prevC = curC;
-            {
-              const uint32_t f = prevL.y, nch = prevL.w;
-              uint32_t pick = 0;
-              for (uint32_t b = 0; b < nch; b += 64) {
-                const uint32_t ci = b + (threadIdx.x & 63);
-                const bool le = ci < nch && n_start[f + ci] <= idx;
-                pick += (uint32_t)__popcll(__ballot(le));
+            const uint32_t f = curL.y, nch = curL.w;
+            float cur_result = 0.0f;
+            uint32_t pick = 0;
+            for (uint32_t b = 0; b < nch; b += 64) {
+              // stage children b .. b+63 of prev
+              const uint32_t cj = b + lane;
+              const bool valid = cj < nch;
+              uint32_t cstart = 0xFFFFFFFFu;
+              if (valid) {
+                const uint32_t c = f + cj;
+                const uint4 l = nodeL[c];
+                cstart = n_start[c];
+                float4 src, nv = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint4 info = make_uint4(0u, 0u, 0u, 0u);
+                if (l.w & kLeafBit) {
+                  const uint32_t np = l.w & ~kLeafBit;
+                  src = pts[l.y];
+                  if (kWN) nv = nrm[l.y];
+                  info = make_uint4(np > 1 ? kMulti : 0u, l.y, np, 0u);
+                } else {
+                  src = nodeC[c];
+                  if (kWN) nv = nodeN[c];
+                  info = make_uint4(0u, 0u, 0u, 0u);
+                }
+                sSrc[threadIdx.x] = src;
+                if (kWN) sNrm[threadIdx.x] = nv;
+                sInfo[threadIdx.x] = info;
               }
-              cur = uniform_u32(n_skip[f + pick - 1]);
+              pick += (uint32_t)__popcll(__ballot(valid && cstart <= idx));
+              wave_lds_sync();
+              if (alive) {
+                const uint32_t m = min(64u, nch - b);
+                for (uint32_t j = 0; j < m; ++j) {
+                  const uint4 info = sInfo[wbase + j];
+                  if (info.x & kMulti) {
+                    for (uint32_t pp = info.y; pp < info.y + info.z; ++pp)
+                      cur_result += term<D, K, EXACT>(pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
+                    cnt += (int32_t)info.z;
+                  } else {
+                    const float4 src = sSrc[wbase + j];
+                    const float4 nv = kWN ? sNrm[wbase + j] : src;
+                    cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
+                    if (info.z)
+                      cnt += 1;
+                    else
+                      cnt_nodes += 1;
+                  }
+                }
+              }
+              wave_lds_sync();
             }
+            // child holding the sample, then down any single-child chain
+            cur = uniform_u32(n_skip[f + pick - 1]);
             curL = nodeL[cur];
             curC = nodeC[cur];
             cur_beta = fmaxf(cur_beta, ffr);
             if (alive) {
-              const float prev_result = kernel_exact<D, K>(prevC, nodeN[prev], alpha, x, y, z);
-              float cur_result = 0.0f;
-              const uint32_t f = prevL.y, nch = prevL.w;
-              for (uint32_t j = 0; j < nch; ++j)
-                cur_result += node_term<D, K>(nodeC, nodeL, nodeN, pts, nrm, alpha, f + j, x, y, z, cnt, cnt_nodes);
+              const float prev_result =
+                  term<D, K, EXACT>(nodeC[prev], nodeN[prev], alpha, alpha_k, x, y, z);
               const float diff = cur_result - prev_result;
               const float wd = (float)nS / (event_prob * (float)(n_end[prev] - n_start[prev])) * diff;
               wd_all += wd;
@@ -509,33 +578,35 @@ void launch_brute_f64(const DeviceTree& t, const float* qx, const float* qy, con
   SBH_HIP(hipGetLastError());
 }
 
-template <int D, int K>
+template <int D, int K, bool EXACT>
 void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, const float* qy,
                const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
                int split, hipStream_t s) {
   uint32_t grid = nblk(nq, kThreads);
+  const float alpha_k = t.alpha * 1.4426950408889634f;
+#define SBH_LAUNCH_BH(G)                                                                      \
+  k_bh<D, K, G, EXACT><<<grid, kThreads, 0, s>>>(t.dfsC.get(), t.dfsL.get(), t.dfsN.get(),   \
+                                                  t.pts.get(), t.nrm.get(), t.alpha, alpha_k, \
+                                                  beta, qx, qy, qz, nq, order, out, counts,   \
+                                                  split)
   if (group == 64)
-    k_bh<D, K, 64><<<grid, kThreads, 0, s>>>(t.nodeC.get(), t.nodeL.get(), t.nodeN.get(),
-                                              t.pts.get(), t.nrm.get(), t.alpha, beta, qx, qy, qz,
-                                              nq, order, out, counts, split);
+    SBH_LAUNCH_BH(64);
   else if (group == 32)
-    k_bh<D, K, 32><<<grid, kThreads, 0, s>>>(t.nodeC.get(), t.nodeL.get(), t.nodeN.get(),
-                                              t.pts.get(), t.nrm.get(), t.alpha, beta, qx, qy, qz,
-                                              nq, order, out, counts, split);
+    SBH_LAUNCH_BH(32);
   else
-    k_bh<D, K, 1><<<grid, kThreads, 0, s>>>(t.nodeC.get(), t.nodeL.get(), t.nodeN.get(),
-                                             t.pts.get(), t.nrm.get(), t.alpha, beta, qx, qy, qz,
-                                             nq, order, out, counts, split);
+    SBH_LAUNCH_BH(1);
+#undef SBH_LAUNCH_BH
   SBH_HIP(hipGetLastError());
 }
 
-template <int D, int K>
+template <int D, int K, bool EXACT>
 void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, const float* qx, const float* qy,
                   const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
                   int split, hipStream_t s) {
-  k_stochastic<D, K><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+  const float alpha_k = t.alpha * 1.4426950408889634f;
+  k_stochastic<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
       t.nodeC.get(), t.nodeL.get(), t.nodeN.get(), t.n_start.get(), t.n_end.get(),
-      t.n_skip.get(), t.pts.get(), t.nrm.get(), t.alpha, spl, (uint32_t)seed,
+      t.n_skip.get(), t.pts.get(), t.nrm.get(), t.alpha, alpha_k, spl, (uint32_t)seed,
       (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out, counts, split);
   SBH_HIP(hipGetLastError());
 }
@@ -599,23 +670,30 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
 }
 
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
-            const uint32_t* order, float* out, int32_t* counts, int split, hipStream_t s) {
+            const uint32_t* order, float* out, int32_t* counts, int split, bool exact,
+            hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
-    launch_bh<D, K>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, s);
+    if (exact)
+      launch_bh<D, K, true>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, s);
+    else
+      launch_bh<D, K, false>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, s);
   });
 }
 
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, const float* q,
                     uint32_t nq, const uint32_t* order, float* out, int32_t* counts, int split,
-                    hipStream_t s) {
+                    bool exact, hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
-    launch_stoch<D, K>(t, spl, seed, qx, qy, qz, nq, order, out, counts, split, s);
+    if (exact)
+      launch_stoch<D, K, true>(t, spl, seed, qx, qy, qz, nq, order, out, counts, split, s);
+    else
+      launch_stoch<D, K, false>(t, spl, seed, qx, qy, qz, nq, order, out, counts, split, s);
   });
 }
 

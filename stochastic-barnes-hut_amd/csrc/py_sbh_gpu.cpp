@@ -7,7 +7,9 @@
 //   multi_level_prefix_control_variate_{...}           src/sbh_gpu.cu:625-660
 // Each evaluation returns (float32[q], kernel milliseconds), as the reference.
 // Keyword extensions keep the reference defaults: vote_group (32, the
-// reference's warp vote), seed (0), sort_queries (False), return_counts.
+// reference's warp vote), seed (0), sort_queries (False), return_counts,
+// exact_terms (False: hardware sqrt / reciprocal / exp2 as the reference's
+// fast-math GPU build; True: correctly rounded, bit-reproducible terms).
 // The GIL is released around device work.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -190,7 +192,8 @@ void bind_tree(py::module_& m, const char* name) {
 }
 
 py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, float beta,
-                    int spl, uint64_t seed, int vote_group, bool sort, bool counts) {
+                    int spl, uint64_t seed, int vote_group, bool sort, bool counts,
+                    bool exact = false) {
   std::vector<float> q = t.soa(queries);
   const int64_t nq = queries.shape(0);
   py::array_t<float> out(nq);
@@ -200,7 +203,7 @@ py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, fl
   p.samples_per_subdomain = spl;
   p.seed = seed;
   p.vote_group = vote_group;
-  p.flags = sort ? SBH_EVAL_SORT_QUERIES : 0;
+  p.flags = (sort ? SBH_EVAL_SORT_QUERIES : 0) | (exact ? SBH_EVAL_EXACT_TERMS : 0);
   float ms = 0.0f;
   float* op = out.mutable_data();
   int32_t* cp = counts ? cnt.mutable_data() : nullptr;
@@ -223,20 +226,20 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
         "tree"_a, "queries"_a);
   m.def(("eval_barnes_hut_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, float beta, int vote_group, bool sort_queries,
-                 bool return_counts) {
+                 bool return_counts, bool exact_terms) {
           return run_eval(t, SBH_BARNES_HUT, kernel, q, beta, 1, 0, vote_group, sort_queries,
-                          return_counts);
+                          return_counts, exact_terms);
         },
         "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "sort_queries"_a = false,
-        "return_counts"_a = false);
+        "return_counts"_a = false, "exact_terms"_a = false);
   m.def(("multi_level_prefix_control_variate_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, int spl, uint64_t seed, bool sort_queries,
-                 bool return_counts) {
+                 bool return_counts, bool exact_terms) {
           return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.f, spl, seed, 1, sort_queries,
-                          return_counts);
+                          return_counts, exact_terms);
         },
         "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
-        "return_counts"_a = false);
+        "return_counts"_a = false, "exact_terms"_a = false);
   m.def(("eval_brute_force_f64_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array queries) {
           std::vector<float> q = t.soa(queries);

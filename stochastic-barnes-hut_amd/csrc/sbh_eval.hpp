@@ -16,12 +16,15 @@ void run_brute(const DeviceTree& t, int kernel, const float* q, uint32_t nq, flo
 void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq, double* out,
                    double* partial, uint32_t nsplit, hipStream_t s);
 // counts: total terms per query, or with split != 0 node terms in counts[0, nq)
-// and point terms in counts[nq, 2 nq).
+// and point terms in counts[nq, 2 nq).  exact: every term with correctly
+// rounded fp32 operations in the reference's order (otherwise hardware
+// sqrt / reciprocal / exp2).
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
-            const uint32_t* order, float* out, int32_t* counts, int split, hipStream_t s);
+            const uint32_t* order, float* out, int32_t* counts, int split, bool exact,
+            hipStream_t s);
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, const float* q,
                     uint32_t nq, const uint32_t* order, float* out, int32_t* counts, int split,
-                    hipStream_t s);
+                    bool exact, hipStream_t s);
 
 // Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)
 // uint32 values.

@@ -129,12 +129,20 @@ struct DeviceTree {
   DevBuf<double> n_m64;  // float64 moments {cx, cy, cz, mass, nx, ny, nz, 0} per node
   // stochastic descent helpers
   DevBuf<uint32_t> n_skip;  // node reached by following single-child chains
+  // Depth-first copy of the walk records for Barnes-Hut: node v's record sits
+  // at dfs position n_dfs[v]; its first child is the next record and its rope
+  // is the position after its subtree, so a walk reads forward in memory.
+  DevBuf<uint32_t> n_size;  // nodes in the subtree
+  DevBuf<uint32_t> n_dfs;
+  DevBuf<float4> dfsC, dfsN;
+  DevBuf<uint4> dfsL;
 
   size_t device_bytes() const {
     return pts.bytes() + nrm.bytes() + perm.bytes() + n_start.bytes() + n_end.bytes() +
            n_parent.bytes() + n_first.bytes() + n_nchild.bytes() + n_next.bytes() +
            n_depth.bytes() + n_lo.bytes() + n_hi.bytes() + n_diam.bytes() + nodeC.bytes() +
-           nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes();
+           nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
+           n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes();
   }
 };
 

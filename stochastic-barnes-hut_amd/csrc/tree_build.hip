@@ -314,18 +314,24 @@ __global__ __launch_bounds__(kChunkThreads) void k_large_count(LargeCtx cx, uint
   if (threadIdx.x == 0) cx.ccnt[blockIdx.x] = make_uint2(tot & 0xFFFFu, tot >> 16);
 }
 
-__global__ void k_large_offsets(LargeCtx cx, uint32_t nlarge) {
-  uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= nlarge) return;
-  uint32_t ol = 0, orr = 0;
-  uint32_t c0 = cx.l_chunk0[li], nc = cx.l_nchunk[li];
-  for (uint32_t j = 0; j < nc; ++j) {
-    cx.coff[c0 + j] = make_uint2(ol, orr);
-    uint2 cc = cx.ccnt[c0 + j];
-    ol += cc.x;
-    orr += cc.y;
+// One workgroup per large node: exclusive scans of its chunks' left-false and
+// right-true counts, in chunk (position) order.
+__global__ __launch_bounds__(kChunkThreads) void k_large_offsets(LargeCtx cx) {
+  __shared__ uint32_t wsum[kChunkThreads / 64];
+  const uint32_t li = blockIdx.x;
+  const uint32_t c0 = cx.l_chunk0[li], nc = cx.l_nchunk[li];
+  uint32_t carry_l = 0, carry_r = 0;
+  for (uint32_t b = 0; b < nc; b += kChunkThreads) {
+    const uint32_t j = b + threadIdx.x;
+    const uint2 cc = j < nc ? cx.ccnt[c0 + j] : make_uint2(0u, 0u);
+    uint32_t tl, tr;
+    const uint32_t pl = block_exclusive_scan<kChunkThreads>(cc.x, wsum, tl);
+    const uint32_t pr = block_exclusive_scan<kChunkThreads>(cc.y, wsum, tr);
+    if (j < nc) cx.coff[c0 + j] = make_uint2(carry_l + pl, carry_r + pr);
+    carry_l += tl;
+    carry_r += tr;
   }
-  cx.mcount[li] = ol;
+  if (threadIdx.x == 0) cx.mcount[li] = carry_l;
 }
 
 // Chunk of kChunk positions, 4 consecutive positions per thread.
@@ -430,7 +436,7 @@ __global__ void k_moments(uint32_t lb, uint32_t count, const uint32_t* n_start,
                           const uint32_t* n_end, const uint32_t* n_first,
                           const uint32_t* n_nchild, const float4* pts, const float4* nrm,
                           float4* nodeC, float4* nodeN, double* m64, uint32_t* n_skip,
-                          int use64) {
+                          uint32_t* n_size, int use64) {
 #pragma clang fp contract(off)
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -459,6 +465,7 @@ __global__ void k_moments(uint32_t lb, uint32_t count, const uint32_t* n_start,
       dnz += w * q.z;
     }
     n_skip[v] = v;
+    n_size[v] = 1;
   } else {
     uint32_t f = n_first[v];
     for (uint32_t j = 0; j < nch; ++j) {
@@ -481,6 +488,9 @@ __global__ void k_moments(uint32_t lb, uint32_t count, const uint32_t* n_start,
       dnz += cm[3] * cm[6];
     }
     n_skip[v] = (nch == 1) ? n_skip[f] : v;
+    uint32_t sz = 1;
+    for (uint32_t j = 0; j < nch; ++j) sz += n_size[f + j];
+    n_size[v] = sz;
   }
   cx /= mass;
   cy /= mass;
@@ -524,6 +534,41 @@ __global__ void k_finalize(uint32_t nn, const uint32_t* n_start, const uint32_t*
   }
   nodeL[v] = r;
 }
+
+// Depth-first positions, one level at a time from the root: a node's first
+// child follows it, each further child follows its elder sibling's subtree.
+__global__ void k_dfs_positions(uint32_t lb, uint32_t count, const uint32_t* n_first,
+                                const uint32_t* n_nchild, const uint32_t* n_size,
+                                uint32_t* n_dfs) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t v = lb + i;
+  uint32_t nch = n_nchild[v];
+  if (nch == 0) return;
+  uint32_t p = n_dfs[v] + 1;
+  uint32_t f = n_first[v];
+  for (uint32_t j = 0; j < nch; ++j) {
+    n_dfs[f + j] = p;
+    p += n_size[f + j];
+  }
+}
+
+__global__ void k_dfs_scatter(uint32_t nn, const uint32_t* n_dfs, const uint32_t* n_size,
+                              const float4* nodeC, const float4* nodeN, const uint4* nodeL,
+                              float4* dfsC, float4* dfsN, uint4* dfsL) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  uint32_t d = n_dfs[v];
+  uint4 l = nodeL[v];
+  if (!(l.w & kLeafBit)) l.y = d + 1;
+  uint32_t rope = d + n_size[v];
+  l.z = rope < nn ? rope : kEnd;
+  dfsC[d] = nodeC[v];
+  dfsN[d] = nodeN[v];
+  dfsL[d] = l;
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
 template <int D>
 __global__ void k_set_root(uint32_t n, float4 lo, float4 hi, uint32_t* n_start, uint32_t* n_end,
@@ -750,7 +795,7 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
       SBH_HIP(hipGetLastError());
       for (uint32_t c = 0; c < B; ++c) {
         k_large_count<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c);
-        k_large_offsets<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, nlarge);
+        k_large_offsets<<<nlarge, kChunkThreads, 0, s>>>(cx);
         k_large_rank<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c);
         k_large_swap<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c, t.perm.get());
       }
@@ -794,16 +839,35 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
   t.nodeL.alloc(num_nodes);
   t.n_m64.alloc((size_t)num_nodes * 8);
   t.n_skip.alloc(num_nodes);
+  t.n_size.alloc(num_nodes);
   for (int l = t.depth; l >= 0; --l) {
     uint32_t b = t.level_begin[l], cnt = t.level_begin[l + 1] - b;
     k_moments<D><<<blocks_for(cnt, 128), 128, 0, s>>>(
         b, cnt, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(), t.pts.get(),
-        t.nrm.get(), t.nodeC.get(), t.nodeN.get(), t.n_m64.get(), t.n_skip.get(), t.moments64);
+        t.nrm.get(), t.nodeC.get(), t.nodeN.get(), t.n_m64.get(), t.n_skip.get(), t.n_size.get(),
+        t.moments64);
     SBH_HIP(hipGetLastError());
   }
   k_finalize<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
       num_nodes, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(),
       t.n_next.get(), t.n_diam.get(), t.nodeL.get());
+  SBH_HIP(hipGetLastError());
+
+  // Depth-first walk records.
+  t.n_dfs.alloc(num_nodes);
+  t.dfsC.alloc(num_nodes);
+  t.dfsN.alloc(num_nodes);
+  t.dfsL.alloc(num_nodes);
+  k_zero_u32<<<1, 1, 0, s>>>(t.n_dfs.get());
+  for (int l = 0; l <= t.depth; ++l) {
+    uint32_t b = t.level_begin[l], cnt = t.level_begin[l + 1] - b;
+    k_dfs_positions<<<blocks_for(cnt, 128), 128, 0, s>>>(b, cnt, t.n_first.get(),
+                                                          t.n_nchild.get(), t.n_size.get(),
+                                                          t.n_dfs.get());
+  }
+  k_dfs_scatter<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
+      num_nodes, t.n_dfs.get(), t.n_size.get(), t.nodeC.get(), t.nodeN.get(), t.nodeL.get(),
+      t.dfsC.get(), t.dfsN.get(), t.dfsL.get());
   SBH_HIP(hipGetLastError());
   SBH_HIP(hipStreamSynchronize(s));
 }
