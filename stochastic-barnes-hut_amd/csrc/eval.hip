@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kThreads) void k_bh(const float4* __restrict__ node
     const uint4 l = nodeL[v];
     const bool leaf = (l.w & kLeafBit) != 0;
     bool reject = false;
-    if (active && !leaf) reject = !(ratio_exact<D>(c, __uint_as_float(l.x), x, y, z) >= beta);
+    if (active && !leaf) reject = !opens_far<D>(c, __uint_as_float(l.x), beta, x, y, z);
     const uint64_t rej = __ballot(reject);
     const bool accept = active && ((rej & my_group) == 0);
     if (accept) {
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
                   const uint32_t np = l.w & ~kLeafBit;
                   src = pts[l.y];
                   if (kWN) nv = nrm[l.y];
-                  info = make_uint4(np > 1 ? kMulti : 0u, l.y, np, 0u);
+                  info = make_uint4(np > 1 ? kMulti : 0u, l.y, np, 0u);  // z: points
                 } else {
                   src = nodeC[c];
                   if (kWN) nv = nodeN[c];
@@ -402,24 +402,44 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(
                 sInfo[threadIdx.x] = info;
               }
               pick += (uint32_t)__popcll(__ballot(valid && cstart <= idx));
+              const uint32_t np_lane = valid ? sInfo[threadIdx.x].z : 0u;
+              const uint64_t multi = __ballot(np_lane > 1);
+              const uint32_t m = min(64u, nch - b);
               wave_lds_sync();
               if (alive) {
-                const uint32_t m = min(64u, nch - b);
-                for (uint32_t j = 0; j < m; ++j) {
-                  const uint4 info = sInfo[wbase + j];
-                  if (info.x & kMulti) {
-                    for (uint32_t pp = info.y; pp < info.y + info.z; ++pp)
-                      cur_result += term<D, K, EXACT>(pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
-                    cnt += (int32_t)info.z;
-                  } else {
+                if (multi == 0) {
+                  // every child is one source: a node moment or a one-point leaf
+#pragma unroll 4
+                  for (uint32_t j = 0; j < m; ++j) {
                     const float4 src = sSrc[wbase + j];
                     const float4 nv = kWN ? sNrm[wbase + j] : src;
                     cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
-                    if (info.z)
-                      cnt += 1;
-                    else
-                      cnt_nodes += 1;
                   }
+                } else {
+                  for (uint32_t j = 0; j < m; ++j) {
+                    const uint4 info = sInfo[wbase + j];
+                    if (info.x & kMulti) {
+                      for (uint32_t pp = info.y; pp < info.y + info.z; ++pp)
+                        cur_result += term<D, K, EXACT>(pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
+                    } else {
+                      const float4 src = sSrc[wbase + j];
+                      const float4 nv = kWN ? sNrm[wbase + j] : src;
+                      cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
+                    }
+                  }
+                }
+              }
+              // term counts of this batch, the same for every lane that evaluated it
+              {
+                uint32_t pt = np_lane, nd = (valid && np_lane == 0) ? 1u : 0u;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                  pt += __shfl_xor(pt, off, 64);
+                  nd += __shfl_xor(nd, off, 64);
+                }
+                if (alive) {
+                  cnt += (int32_t)pt;
+                  cnt_nodes += (int32_t)nd;
                 }
               }
               wave_lds_sync();

@@ -130,19 +130,30 @@ __global__ void k_classify(uint32_t lb, uint32_t count, const uint32_t* n_start,
                            int max_depth, uint32_t* n_nchild, uint32_t* n_first,
                            uint32_t* small_list, uint32_t* large_list, uint32_t* counters) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+  const uint32_t lane = threadIdx.x & 63;
+  bool small = false, large = false;
   uint32_t v = lb + i;
-  uint32_t cnt = n_end[v] - n_start[v];
-  // include/tree3d.h:34-36
-  bool leaf = n_depth[v] >= max_depth || cnt <= (uint32_t)leaf_size;
-  n_nchild[v] = 0;
-  n_first[v] = kEnd;
-  if (leaf) return;
-  if (cnt <= (uint32_t)kSmallMax) {
-    small_list[atomicAdd(&counters[0], 1u)] = v;
-  } else {
-    large_list[atomicAdd(&counters[1], 1u)] = v;
+  if (i < count) {
+    uint32_t cnt = n_end[v] - n_start[v];
+    // include/tree3d.h:34-36
+    bool leaf = n_depth[v] >= max_depth || cnt <= (uint32_t)leaf_size;
+    n_nchild[v] = 0;
+    n_first[v] = kEnd;
+    small = !leaf && cnt <= (uint32_t)kSmallMax;
+    large = !leaf && cnt > (uint32_t)kSmallMax;
   }
+  // one atomic per wavefront and list
+  const uint64_t ms = __ballot(small), ml = __ballot(large);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t bs = 0, bl = 0;
+  if (lane == 0) {
+    if (ms) bs = atomicAdd(&counters[0], (uint32_t)__popcll(ms));
+    if (ml) bl = atomicAdd(&counters[1], (uint32_t)__popcll(ml));
+  }
+  bs = __shfl(bs, 0, 64);
+  bl = __shfl(bl, 0, 64);
+  if (small) small_list[bs + __popcll(ms & below)] = v;
+  if (large) large_list[bl + __popcll(ml & below)] = v;
 }
 
 // Zone ranking inside one LDS-resident node: element at position p of [a, b)
