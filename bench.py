@@ -64,6 +64,10 @@ def main():
                     help="profiling mode: build the trees, run only this method --steps times "
                          "(stochastic at --spl[0], Barnes-Hut at --theta) and exit")
     ap.add_argument("--theta", type=float, default=1.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-load", type=float, default=0.5,
+                    help="Poisson offered load as a fraction of the measured burst capacity")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (a counter "
                          "calibration kernel of known traffic)")
@@ -240,6 +244,12 @@ def main():
                 "node_terms": node_terms, "point_terms": point_terms,
                 "flops_per_term": FLOPS_PER_PAIR_GRAVITY_3D, "kernel_ms": kernel_ms}
 
+    serving = None
+    if not args.no_serve:
+        del t1, t2
+        torch.cuda.empty_cache()
+        serving = serving_bench(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pts, m, lo, hi, q32, head["samples_per_subdomain"], args.cpu_seconds,
@@ -262,8 +272,14 @@ def main():
             "err_queries": ne, "tree": "octree k=2, leaf size 1",
             "target_err_met": head_ok,
         },
-        "latency_ms": {"p50": percentile(wall, 50), "p99": percentile(wall, 99),
-                       "kind": "per 2^20-query evaluation, wall clock"},
+        "latency_ms": ({"p50": serving["poisson"]["p50_ms"], "p99": serving["poisson"]["p99_ms"],
+                        "kind": "serving request latency, submit to results in host memory, "
+                                "Poisson arrivals at {:.0%} of burst capacity".format(
+                                    args.serve_load)}
+                       if serving else None),
+        "step_latency_ms": {"p50": percentile(wall, 50), "p99": percentile(wall, 99),
+                            "kind": "per 2^20-query evaluation, wall clock"},
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_brute_force": value / results["brute_force"]["queries_per_s"],
@@ -274,6 +290,105 @@ def main():
         "fp64_reference_ms": ref_ms,
     }
     print(json.dumps(line), flush=True)
+
+
+SERVE_TREES = [  # name, dim, kernel, points, generator (as in the 1M-point cases)
+    ("plummer_r10_3d_gravity_64k", 3, "gravity", 1 << 16),
+    ("gmm_2d_gravity_128k", 2, "gravity", 1 << 17),
+    ("icosphere7_3d_wn_328k", 3, "wn", 327680),
+    ("cube_3d_smoothdist_512k", 3, "smoothdist", 1 << 19),
+    ("plummer_r10_3d_gravity_1m", 3, "gravity", 1 << 20),
+    ("gmm_2d_smoothdist_2m", 2, "smoothdist", 1 << 21),
+    ("cube_3d_gravity_4m", 3, "gravity", 1 << 22),
+    ("plummer_r10_3d_smoothdist_4m", 3, "smoothdist", 1 << 22),
+]
+
+
+def _serve_sources(name, n):
+    import workloads as W
+
+    if name.startswith("plummer"):
+        return W.plummer(n, 1, seed_sources=11, rmax=10.0)[:3]
+    if name.startswith("gmm"):
+        return W.gmm2d(n, 1, seed=12)[:3]
+    if name.startswith("icosphere"):
+        return W.noisy_icosphere(7, 1, seed=13)[:3]
+    return W.uniform_cube(n, 1, seed=14)[:3]
+
+
+def serving_bench(args):
+    """Eight resident trees (64k-4M sources, 2D/3D, all three kernels); requests
+    of log-uniform size 1e3-1e6 queries, each picking a tree, Barnes-Hut
+    (theta 0.5) or the stochastic estimator (128 samples per query) and a
+    seed.  A burst of all requests measures capacity; then a Poisson stream at
+    --serve-load of that capacity measures latency."""
+    import sbh_gpu
+
+    rng = np.random.default_rng(21)
+    srv = sbh_gpu.Server(4 << 30, 4)
+    trees = []
+    for name, dim, kern, n in SERVE_TREES:
+        pts, nrm, m = _serve_sources(name, n)
+        lo, hi = pts.min(0) - 0.01, pts.max(0) + 0.01
+        cls = sbh_gpu.OctreeGPU2 if dim == 3 else sbh_gpu.QuadtreeGPU2
+        tr = cls(pts, nrm, m, lo, hi, alpha=10.0 if kern == "smoothdist" else 1.0)
+        nsub = int(tr.nodes()["nchild"][0])
+        trees.append(dict(name=name, dim=dim, kernel=kern, n=len(pts), lo=lo, hi=hi,
+                          id=srv.add_tree(tr), tree=tr, subdomains=nsub))
+        log(f"serve tree {name}: {len(pts)} points, {nsub} subdomains, build {tr.build_ms:.1f} ms")
+    nreq = args.serve_requests
+    reqs = []
+    for _ in range(nreq):
+        t = trees[int(rng.integers(len(trees)))]
+        q = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+        pts = t["lo"] + (t["hi"] - t["lo"]) * rng.random((q, t["dim"]))
+        method = "stochastic" if rng.random() < 0.5 else "barnes_hut"
+        spl = max(1, 128 // max(1, t["subdomains"]))
+        reqs.append((t, method, pts.astype(np.float32), spl, int(rng.integers(1 << 40))))
+    total_q = sum(len(r[2]) for r in reqs)
+
+    def run(arrivals):
+        hs = []
+        t0 = time.perf_counter()
+        for (t, method, pts, spl, seed), at in zip(reqs, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            hs.append(srv.submit(t["id"], method, t["kernel"], pts, beta=2.0,
+                                 samples_per_subdomain=spl, seed=seed))
+        lats = [h.result(600.0)[1] for h in hs]
+        el = time.perf_counter() - t0
+        return np.asarray(lats), el
+
+    # warm-up (first launches compile nothing, but touch every code path)
+    for h in [srv.submit(t["id"], m_, t["kernel"], p[:2000], beta=2.0, samples_per_subdomain=s_,
+                         seed=1) for (t, m_, p, s_, _) in reqs[:16]]:
+        h.result(600.0)
+    lat_b, el_b = run(np.zeros(nreq))
+    cap = total_q / el_b
+    log(f"serve burst: {nreq} requests, {total_q} queries in {el_b:.2f} s -> {cap / 1e6:.1f} M q/s")
+    mean_q = total_q / nreq
+    rate = args.serve_load * cap / mean_q
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    lat_p, el_p = run(arrivals)
+    log(f"serve poisson: rate {rate:.1f} req/s, p50 {np.percentile(lat_p, 50):.1f} ms, "
+        f"p99 {np.percentile(lat_p, 99):.1f} ms")
+    st = srv.stats()
+    return {
+        "trees": [{k: t[k] for k in ("name", "dim", "kernel", "n", "subdomains")} for t in trees],
+        "requests": nreq, "queries": int(total_q),
+        "request_sizes": "log-uniform 1e3..1e6 queries",
+        "methods": "barnes_hut theta=0.5 / stochastic 128 samples per query, 50/50",
+        "burst": {"queries_per_s": cap, "elapsed_s": el_b,
+                  "p50_ms": float(np.percentile(lat_b, 50)),
+                  "p99_ms": float(np.percentile(lat_b, 99))},
+        "poisson": {"rate_req_per_s": rate, "offered_load": args.serve_load,
+                    "queries_per_s": total_q / el_p, "elapsed_s": el_p,
+                    "p50_ms": float(np.percentile(lat_p, 50)),
+                    "p99_ms": float(np.percentile(lat_p, 99))},
+        "launches": st["launches"], "peak_request_bytes": st["peak_bytes"],
+        "budget_bytes": st["budget_bytes"], "resident_tree_bytes": st["tree_bytes"],
+    }
 
 
 def cpu_baseline(pts, m, lo, hi, q32, spl, budget_s, gpu_qps):

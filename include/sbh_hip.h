@@ -150,6 +150,60 @@ int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* o
 /* Stream the tree's work is issued on (hipStream_t). */
 int sbh_tree_stream(sbh_tree* tree, void** stream);
 
+/*
+ * Serving.  No counterpart in the reference (it re-copies queries per call
+ * and runs synchronously, src/sbh_gpu.cu:212-248): resident trees, requests
+ * queued and admitted in arrival order while their working sets (queries +
+ * results, (dim + 1) * 4 bytes per query) fit in the server's device budget,
+ * each admitted batch run as one work-queue launch, up to max_inflight
+ * launches overlapping on separate streams.  Barnes-Hut runs per query
+ * (vote_group is ignored), the stochastic estimator keys paths on (seed,
+ * 64-query block of the request): a request's values are identical alone and
+ * in any mix.  Terms use the fast kernels.
+ *
+ * Trees must outlive the server.  `queries` (column-major dim x q) and
+ * `results` (q floats) are host memory that must stay valid until the request
+ * is done, failed or cancelled.
+ */
+typedef struct sbh_server sbh_server;
+
+#define SBH_REQ_QUEUED 0
+#define SBH_REQ_RUNNING 1
+#define SBH_REQ_DONE 2
+#define SBH_REQ_FAILED 3
+#define SBH_REQ_CANCELLED 4
+
+typedef struct sbh_request {
+  int32_t tree_id;
+  int32_t method; /* SBH_BARNES_HUT or SBH_STOCHASTIC */
+  int32_t kernel;
+  sbh_eval_params params;
+  const float* queries;
+  int64_t q;
+  float* results;
+} sbh_request;
+
+typedef struct sbh_server_stats {
+  int64_t completed, launches, queued;
+  int64_t bytes_in_use, peak_bytes, budget_bytes, tree_bytes;
+} sbh_server_stats;
+
+int sbh_server_create(int64_t budget_bytes, int32_t max_inflight, sbh_server** out);
+int sbh_server_destroy(sbh_server* server); /* waits for in-flight launches */
+int sbh_server_add_tree(sbh_server* server, sbh_tree* tree, int32_t* tree_id);
+/* SBH_ERR_BUDGET if the request alone exceeds the budget. */
+int sbh_server_submit(sbh_server* server, const sbh_request* request, int64_t* request_id);
+/* state: SBH_REQ_*; latency_ms (optional): submit -> results in place, once done. */
+int sbh_server_poll(sbh_server* server, int64_t request_id, int32_t* state, double* latency_ms);
+/* timeout_s < 0 waits without limit. */
+int sbh_server_wait(sbh_server* server, int64_t request_id, double timeout_s, int32_t* state,
+                    double* latency_ms);
+/* Cancels a request that is still queued; *cancelled = 0 if it already started. */
+int sbh_server_cancel(sbh_server* server, int64_t request_id, int32_t* cancelled);
+/* Drops the bookkeeping of a finished request. */
+int sbh_server_release(sbh_server* server, int64_t request_id);
+int sbh_server_stats_get(sbh_server* server, sbh_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

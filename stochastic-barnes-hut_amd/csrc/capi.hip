@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -13,6 +14,11 @@
 #include "../../include/sbh_hip.h"
 #include "sbh_eval.hpp"
 #include "sbh_runtime.hpp"
+#include "sbh_serve.hpp"
+
+struct sbh_server {
+  std::unique_ptr<sbh::Server> s;
+};
 
 struct sbh_tree {
   sbh::DeviceTree t;
@@ -385,6 +391,81 @@ int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q
     SBH_HIP(hipEventSynchronize(tree->ev1));
     if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));
     SBH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int sbh_server_create(int64_t budget_bytes, int32_t max_inflight, sbh_server** out) {
+  return guarded([&] {
+    SBH_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    SBH_REQUIRE(budget_bytes > 0, "budget must be positive");
+    require_device();
+    auto h = std::make_unique<sbh_server>();
+    h->s = std::make_unique<sbh::Server>((size_t)budget_bytes, max_inflight);
+    *out = h.release();
+  });
+}
+
+int sbh_server_destroy(sbh_server* server) {
+  return guarded([&] { delete server; });
+}
+
+int sbh_server_add_tree(sbh_server* server, sbh_tree* tree, int32_t* tree_id) {
+  return guarded([&] {
+    SBH_REQUIRE(server && tree && tree_id, "NULL argument");
+    *tree_id = server->s->add_tree(&tree->t);
+  });
+}
+
+int sbh_server_submit(sbh_server* server, const sbh_request* request, int64_t* request_id) {
+  return guarded([&] {
+    SBH_REQUIRE(server && request && request_id, "NULL argument");
+    const sbh_request& r = *request;
+    *request_id = server->s->submit(r.tree_id, r.method, r.kernel, r.params, r.queries, r.q,
+                                    r.results);
+  });
+}
+
+int sbh_server_poll(sbh_server* server, int64_t request_id, int32_t* state, double* latency_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(server && state, "NULL argument");
+    *state = server->s->state(request_id, latency_ms);
+  });
+}
+
+int sbh_server_wait(sbh_server* server, int64_t request_id, double timeout_s, int32_t* state,
+                    double* latency_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(server && state, "NULL argument");
+    *state = server->s->wait(request_id, timeout_s, latency_ms);
+  });
+}
+
+int sbh_server_cancel(sbh_server* server, int64_t request_id, int32_t* cancelled) {
+  return guarded([&] {
+    SBH_REQUIRE(server && cancelled, "NULL argument");
+    *cancelled = server->s->cancel(request_id) ? 1 : 0;
+  });
+}
+
+int sbh_server_release(sbh_server* server, int64_t request_id) {
+  return guarded([&] {
+    SBH_REQUIRE(server, "NULL argument");
+    server->s->release(request_id);
+  });
+}
+
+int sbh_server_stats_get(sbh_server* server, sbh_server_stats* stats) {
+  return guarded([&] {
+    SBH_REQUIRE(server && stats, "NULL argument");
+    sbh::ServerStats st = server->s->stats();
+    stats->completed = st.completed;
+    stats->launches = st.launches;
+    stats->queued = st.queued;
+    stats->bytes_in_use = st.bytes_in_use;
+    stats->peak_bytes = st.peak_bytes;
+    stats->budget_bytes = st.budget_bytes;
+    stats->tree_bytes = st.tree_bytes;
   });
 }
 

@@ -18,6 +18,8 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <memory>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sbh_hip.h"
@@ -258,6 +260,129 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
         "tree"_a, "queries"_a);
 }
 
+// Serving (sbh_server_* in include/sbh_hip.h).  Query and result buffers of a
+// request are owned here and kept alive until the engine has finished with
+// them, whether or not the caller still holds the request object.
+struct RequestBufs {
+  std::vector<float> queries;  // column-major dim x q
+  py::array_t<float> results;
+};
+
+struct PyServer;
+
+struct PyRequest {
+  std::shared_ptr<RequestBufs> bufs;
+  sbh_server* srv = nullptr;
+  int64_t id = 0;
+  int state() const {
+    int32_t st = 0;
+    check(sbh_server_poll(srv, id, &st, nullptr), "poll");
+    return st;
+  }
+  int wait(double timeout) const {
+    int32_t st = 0;
+    double lat = 0;
+    {
+      py::gil_scoped_release nogil;
+      check(sbh_server_wait(srv, id, timeout, &st, &lat), "wait");
+    }
+    return st;
+  }
+  py::tuple result(double timeout) const {
+    int st = wait(timeout);
+    if (st != SBH_REQ_DONE)
+      throw std::runtime_error("request " + std::to_string(id) + " not done (state " +
+                               std::to_string(st) + ")");
+    int32_t s2 = 0;
+    double lat = 0;
+    check(sbh_server_poll(srv, id, &s2, &lat), "poll");
+    return py::make_tuple(bufs->results, lat);
+  }
+};
+
+struct PyServer {
+  sbh_server* h = nullptr;
+  std::unordered_map<int64_t, std::shared_ptr<RequestBufs>> live;
+  std::vector<py::object> trees;  // resident trees outlive the server
+
+  PyServer(int64_t budget, int max_inflight) {
+    check(sbh_server_create(budget, max_inflight, &h), "server");
+  }
+  ~PyServer() {
+    if (h) {
+      py::gil_scoped_release nogil;
+      sbh_server_destroy(h);
+    }
+  }
+  int add_tree(py::object tree) {
+    GPUTreeBase& t = tree.cast<GPUTreeBase&>();
+    int32_t id = 0;
+    check(sbh_server_add_tree(h, t.h, &id), "add tree");
+    trees.push_back(tree);
+    return id;
+  }
+  void sweep() {
+    for (auto it = live.begin(); it != live.end();) {
+      int32_t st = 0;
+      if (it->second.use_count() == 1 &&
+          sbh_server_poll(h, it->first, &st, nullptr) == SBH_OK && st >= SBH_REQ_DONE) {
+        sbh_server_release(h, it->first);
+        it = live.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  PyRequest submit(int tree_id, const std::string& method, const std::string& kernel,
+                   F32Array queries, float beta, int spl, uint64_t seed) {
+    sweep();
+    if (tree_id < 0 || tree_id >= (int)trees.size()) throw py::value_error("unknown tree id");
+    GPUTreeBase& t = trees[tree_id].cast<GPUTreeBase&>();
+    auto b = std::make_shared<RequestBufs>();
+    b->queries = t.soa(queries);
+    const int64_t nq = queries.shape(0);
+    b->results = py::array_t<float>(nq);
+    sbh_request r{};
+    r.tree_id = tree_id;
+    r.method = method == "stochastic" ? SBH_STOCHASTIC : SBH_BARNES_HUT;
+    if (method != "stochastic" && method != "barnes_hut")
+      throw py::value_error("method must be 'barnes_hut' or 'stochastic'");
+    r.kernel = kernel == "gravity" ? SBH_GRAVITY : kernel == "wn" ? SBH_WINDING_NUMBER
+                                                                   : SBH_SMOOTH_DIST;
+    if (kernel != "gravity" && kernel != "wn" && kernel != "smoothdist")
+      throw py::value_error("kernel must be 'gravity', 'wn' or 'smoothdist'");
+    r.params.beta = beta;
+    r.params.samples_per_subdomain = spl;
+    r.params.seed = seed;
+    r.params.vote_group = 1;
+    r.queries = b->queries.data();
+    r.q = nq;
+    r.results = b->results.mutable_data();
+    int64_t id = 0;
+    check(sbh_server_submit(h, &r, &id), "submit");
+    live[id] = b;
+    return PyRequest{b, h, id};
+  }
+  bool cancel(const PyRequest& r) {
+    int32_t c = 0;
+    check(sbh_server_cancel(h, r.id, &c), "cancel");
+    return c != 0;
+  }
+  py::dict stats() {
+    sbh_server_stats st{};
+    check(sbh_server_stats_get(h, &st), "stats");
+    py::dict d;
+    d["completed"] = st.completed;
+    d["launches"] = st.launches;
+    d["queued"] = st.queued;
+    d["bytes_in_use"] = st.bytes_in_use;
+    d["peak_bytes"] = st.peak_bytes;
+    d["budget_bytes"] = st.budget_bytes;
+    d["tree_bytes"] = st.tree_bytes;
+    return d;
+  }
+};
+
 }  // namespace
 
 PYBIND11_MODULE(sbh_gpu, m) {
@@ -278,6 +403,24 @@ PYBIND11_MODULE(sbh_gpu, m) {
   bind_kernel(m, "gravity", SBH_GRAVITY);
   bind_kernel(m, "wn", SBH_WINDING_NUMBER);
   bind_kernel(m, "smoothdist", SBH_SMOOTH_DIST);
+  py::class_<PyRequest>(m, "Request")
+      .def_readonly("id", &PyRequest::id)
+      .def("state", &PyRequest::state)
+      .def("wait", &PyRequest::wait, "timeout"_a = -1.0)
+      .def("result", &PyRequest::result, "timeout"_a = -1.0,
+           "(values, latency_ms) once done; latency is submit -> results in place");
+  py::class_<PyServer>(m, "Server")
+      .def(py::init<int64_t, int>(), "budget_bytes"_a, "max_inflight"_a = 4)
+      .def("add_tree", &PyServer::add_tree, "tree"_a)
+      .def("submit", &PyServer::submit, "tree_id"_a, "method"_a, "kernel"_a, "queries"_a,
+           "beta"_a = 2.0f, "samples_per_subdomain"_a = 1, "seed"_a = 0)
+      .def("cancel", &PyServer::cancel, "request"_a)
+      .def("stats", &PyServer::stats);
+  m.attr("REQ_QUEUED") = SBH_REQ_QUEUED;
+  m.attr("REQ_RUNNING") = SBH_REQ_RUNNING;
+  m.attr("REQ_DONE") = SBH_REQ_DONE;
+  m.attr("REQ_FAILED") = SBH_REQ_FAILED;
+  m.attr("REQ_CANCELLED") = SBH_REQ_CANCELLED;
   m.def("device_count", [] {
     int32_t n = 0;
     check(sbh_device_count(&n), "device count");

@@ -7,8 +7,15 @@
 #include <stdint.h>
 
 #include "sbh_runtime.hpp"
+#include "sbh_walk.hpp"
 
 namespace sbh {
+
+inline TreeView view_of(const DeviceTree& t) {
+  return TreeView{t.nodeC.get(), t.nodeL.get(), t.nodeN.get(), t.dfsC.get(), t.dfsL.get(),
+                  t.dfsN.get(), t.n_start.get(), t.n_end.get(), t.n_skip.get(), t.pts.get(),
+                  t.nrm.get(), t.alpha, t.alpha * 1.4426950408889634f};
+}
 
 uint32_t brute_split_count(uint32_t n, uint32_t nq);
 void run_brute(const DeviceTree& t, int kernel, const float* q, uint32_t nq, float* out,

@@ -1,0 +1,331 @@
+// Per-wavefront tree walks shared by the one-shot evaluation kernels
+// (eval.hip) and the serving kernel (serve.hip): deterministic Barnes-Hut and
+// the stochastic control-variate estimator over a device-resident tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sbh_device.hpp"
+
+namespace sbh {
+
+// Device pointers of one resident tree.
+struct TreeView {
+  const float4* nodeC;  // breadth-first records
+  const uint4* nodeL;
+  const float4* nodeN;
+  const float4* dfsC;   // depth-first records (Barnes-Hut walk)
+  const uint4* dfsL;
+  const float4* dfsN;
+  const uint32_t* n_start;
+  const uint32_t* n_end;
+  const uint32_t* n_skip;
+  const float4* pts;
+  const float4* nrm;
+  float alpha;    // smooth-distance exponent
+  float alpha_k;  // alpha * log2(e), for exp2
+};
+
+struct LaneResult {
+  float value;
+  int32_t points;  // point terms evaluated
+  int32_t nodes;   // node terms evaluated
+};
+
+__device__ __forceinline__ void load_query(const float* qx, const float* qy, const float* qz,
+                                           uint32_t i, int dim, float& x, float& y, float& z) {
+  x = qx[i];
+  y = qy[i];
+  z = dim == 3 ? qz[i] : 0.0f;
+}
+
+// One term of a sum: exact (correctly rounded, reference operation order) or
+// fast (hardware sqrt / reciprocal / exp2; alpha_k = alpha * log2(e)).
+template <int D, int K, bool EXACT>
+__device__ __forceinline__ float term(float4 src, float4 nrm, float alpha, float alpha_k,
+                                      float x, float y, float z) {
+  if constexpr (EXACT)
+    return kernel_exact<D, K>(src, nrm, alpha, x, y, z);
+  else
+    return kernel_fast<D, K>(src, nrm, alpha_k, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic Barnes-Hut (include/barnes_hut.h:23-78 per query,
+// include/barnes_hut_gpu.h:19-72 vote).
+//
+// One wavefront walks the union of its lanes' traversals along the tree's
+// ropes (depth-first order, children in cell order): at node v the wave
+// descends to v's first child if any active lane (GROUP == 1) or any lane of
+// an undecided vote group (GROUP > 1) rejects v, and otherwise follows v's
+// rope.  A lane that accepts v while the wave descends adds v's term and
+// sleeps until the wave reaches v's rope target, which is where its own walk
+// resumes.  Each lane therefore adds exactly the terms of its own walk, in the
+// same order.  The node index is wave-uniform, so node records are scalar
+// loads, and the records are stored in depth-first order so that the walk
+// reads forward through memory.  The opening test is always evaluated with
+// correctly rounded operations, so every lane opens exactly the nodes the
+// reference walk opens.
+// ---------------------------------------------------------------------------
+template <int D, int K, int GROUP, bool EXACT>
+__device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, bool live, float x,
+                                              float y, float z) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint64_t my_group;
+  if constexpr (GROUP == 64) {
+    my_group = ~0ull;
+  } else if constexpr (GROUP == 32) {
+    my_group = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
+  } else {
+    my_group = 1ull << lane;
+  }
+  const float4* __restrict__ nodeC = t.dfsC;
+  const uint4* __restrict__ nodeL = t.dfsL;
+  const float4* __restrict__ nodeN = t.dfsN;
+  float acc = 0.0f;
+  int32_t cnt = 0, cnt_nodes = 0;
+  bool active = live;
+  uint32_t resume = kEnd;
+  uint32_t v = 0;
+  while (v != kEnd) {
+    v = uniform_u32(v);
+    if (!active && live && resume == v) active = true;
+    const float4 c = nodeC[v];
+    const uint4 l = nodeL[v];
+    const bool leaf = (l.w & kLeafBit) != 0;
+    bool reject = false;
+    if (active && !leaf) reject = !opens_far<D>(c, __uint_as_float(l.x), beta, x, y, z);
+    const uint64_t rej = __ballot(reject);
+    const bool accept = active && ((rej & my_group) == 0);
+    if (accept) {
+      if (leaf) {
+        const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
+        for (uint32_t j = p0; j < p1; ++j)
+          acc += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+        cnt += (int32_t)(p1 - p0);
+      } else {
+        acc += term<D, K, EXACT>(c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
+        cnt_nodes += 1;
+      }
+    }
+    if (rej == 0) {
+      v = l.z;
+    } else {
+      if (accept) {
+        active = false;
+        resume = l.z;
+      }
+      v = l.y;
+    }
+  }
+  return {acc, cnt, cnt_nodes};
+}
+
+// ---------------------------------------------------------------------------
+// Stochastic control-variate estimator (include/tree_sampler.h:21-113,
+// include/tree_sampler_gpu.h:25-140).  The 64 queries of a wavefront share
+// their random sample paths (the reference shares them per thread block); the
+// path of sample `ss` in subdomain `si` of wave `g` is drawn from Philox block
+// {ss, si, g, b} under key = seed: words 0,1 give the sampled point index,
+// word 2 + l the Russian-roulette uniform at descent level l.  Each lane runs
+// its own roulette against the shared uniforms; the roulette, the weights and
+// the opening ratios use the reference's fp32 operations exactly.
+//
+// At every level the wavefront stages the children of the current node in
+// LDS with one coalesced load per lane (children are contiguous in
+// breadth-first order) and every lane then reads them as broadcasts.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMulti = 1u;  // leaf with more than one point
+
+template <int D, int K, bool EXACT>
+__device__ __forceinline__ float node_term(const TreeView& t, uint32_t v, float x, float y,
+                                           float z, int32_t& cnt, int32_t& cnt_nodes) {
+  const uint4 l = t.nodeL[v];
+  float s = 0.0f;
+  if (l.w & kLeafBit) {
+    const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
+    for (uint32_t j = p0; j < p1; ++j)
+      s += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+    cnt += (int32_t)(p1 - p0);
+  } else {
+    s = term<D, K, EXACT>(t.nodeC[v], t.nodeN[v], t.alpha, t.alpha_k, x, y, z);
+    cnt_nodes += 1;
+  }
+  return s;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sSrc / sNrm / sInfo: this wavefront's 64 LDS slots.  g: wavefront index
+// within the request (the path key).
+template <int D, int K, bool EXACT>
+__device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int spl,
+                                                          uint32_t key0, uint32_t key1,
+                                                          uint32_t g, bool live, float x, float y,
+                                                          float z, float4* sSrc, float4* sNrm,
+                                                          uint4* sInfo) {
+#pragma clang fp contract(off)
+  constexpr bool kWN = (K == WINDING_NUMBER);
+  const uint32_t lane = threadIdx.x & 63;
+  const float4* __restrict__ nodeC = t.nodeC;
+  const uint4* __restrict__ nodeL = t.nodeL;
+  const float4* __restrict__ nodeN = t.nodeN;
+  const uint32_t* __restrict__ n_start = t.n_start;
+  const uint32_t* __restrict__ n_end = t.n_end;
+  const uint32_t* __restrict__ n_skip = t.n_skip;
+  const float4* __restrict__ pts = t.pts;
+  const float4* __restrict__ nrm = t.nrm;
+  const float alpha = t.alpha, alpha_k = t.alpha_k;
+  int32_t cnt = 0, cnt_nodes = 0;
+  const uint4 rl = nodeL[0];
+  float result;
+  if (rl.w & kLeafBit) {
+    result = node_term<D, K, EXACT>(t, 0, x, y, z, cnt, cnt_nodes);
+  } else {
+    float bh = 0.0f, mlcv = 0.0f;
+    const uint32_t nsub = rl.w, first = rl.y;
+    for (uint32_t si = 0; si < nsub; ++si) {
+      const uint32_t S = uniform_u32(first + si);
+      bh += node_term<D, K, EXACT>(t, S, x, y, z, cnt, cnt_nodes);
+      const uint4 Sl = nodeL[S];
+      const uint32_t sstart = n_start[S];
+      const uint32_t nS = n_end[S] - sstart;
+      float sub_total = 0.0f;
+      if (!(Sl.w & kLeafBit)) {
+        const float cS = ratio_exact<D>(nodeC[S], __uint_as_float(Sl.x), x, y, z);
+        for (int ss = 0; ss < spl; ++ss) {
+          Philox4 r0 = philox4x32_10((uint32_t)ss, si, g, 0u, key0, key1);
+          const uint32_t idx = uniform_u32(sstart + scale_draw(r0.x, r0.y, nS));
+          Philox4 blk = r0;
+          uint32_t blk_id = 0;
+          float event_prob = 1.0f;
+          float wd_all = 0.0f;
+          float cur_beta = fmaxf(1.0f, cS);
+          bool alive = live;
+          uint32_t cur = S;
+          uint32_t level = 0;
+          uint4 curL = Sl;
+          float4 curC = nodeC[S];
+          while (!(curL.w & kLeafBit)) {
+            if (__ballot(alive) == 0) break;
+            const uint32_t wi = level + 2;
+            if ((wi >> 2) != blk_id) {
+              blk_id = wi >> 2;
+              blk = philox4x32_10((uint32_t)ss, si, g, blk_id, key0, key1);
+            }
+            const uint32_t wsel = wi & 3;
+            const uint32_t word = wsel == 0 ? blk.x : wsel == 1 ? blk.y : wsel == 2 ? blk.z : blk.w;
+            const float p = rr_uniform(word);
+            ++level;
+            const float ffr = ratio_exact<D>(curC, __uint_as_float(curL.x), x, y, z);
+            const float tp = fminf(cur_beta / ffr, 1.0f);
+            if (alive) {
+              if (p > tp) {
+                event_prob *= 1.0f - tp;
+                alive = false;
+              } else {
+                event_prob *= tp;
+              }
+            }
+            if (__ballot(alive) == 0) break;
+            const uint32_t prev = cur;
+            const uint32_t f = curL.y, nch = curL.w;
+            float cur_result = 0.0f;
+            uint32_t pick = 0;
+            for (uint32_t b = 0; b < nch; b += 64) {
+              // stage children b .. b+63 of prev
+              const uint32_t cj = b + lane;
+              const bool valid = cj < nch;
+              uint32_t cstart = 0xFFFFFFFFu;
+              if (valid) {
+                const uint32_t c = f + cj;
+                const uint4 l = nodeL[c];
+                cstart = n_start[c];
+                float4 src, nv = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint4 info = make_uint4(0u, 0u, 0u, 0u);
+                if (l.w & kLeafBit) {
+                  const uint32_t np = l.w & ~kLeafBit;
+                  src = pts[l.y];
+                  if (kWN) nv = nrm[l.y];
+                  info = make_uint4(np > 1 ? kMulti : 0u, l.y, np, 0u);  // z: points
+                } else {
+                  src = nodeC[c];
+                  if (kWN) nv = nodeN[c];
+                  info = make_uint4(0u, 0u, 0u, 0u);
+                }
+                sSrc[lane] = src;
+                if (kWN) sNrm[lane] = nv;
+                sInfo[lane] = info;
+              }
+              pick += (uint32_t)__popcll(__ballot(valid && cstart <= idx));
+              const uint32_t np_lane = valid ? sInfo[lane].z : 0u;
+              const uint64_t multi = __ballot(np_lane > 1);
+              const uint32_t m = min(64u, nch - b);
+              wave_lds_sync();
+              if (alive) {
+                if (multi == 0) {
+                  // every child is one source: a node moment or a one-point leaf
+#pragma unroll 4
+                  for (uint32_t j = 0; j < m; ++j) {
+                    const float4 src = sSrc[j];
+                    const float4 nv = kWN ? sNrm[j] : src;
+                    cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
+                  }
+                } else {
+                  for (uint32_t j = 0; j < m; ++j) {
+                    const uint4 info = sInfo[j];
+                    if (info.x & kMulti) {
+                      for (uint32_t pp = info.y; pp < info.y + info.z; ++pp)
+                        cur_result += term<D, K, EXACT>(pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
+                    } else {
+                      const float4 src = sSrc[j];
+                      const float4 nv = kWN ? sNrm[j] : src;
+                      cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
+                    }
+                  }
+                }
+              }
+              // term counts of this batch, the same for every lane that evaluated it
+              {
+                uint32_t pt = np_lane, nd = (valid && np_lane == 0) ? 1u : 0u;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                  pt += __shfl_xor(pt, off, 64);
+                  nd += __shfl_xor(nd, off, 64);
+                }
+                if (alive) {
+                  cnt += (int32_t)pt;
+                  cnt_nodes += (int32_t)nd;
+                }
+              }
+              wave_lds_sync();
+            }
+            // child holding the sample, then down any single-child chain
+            cur = uniform_u32(n_skip[f + pick - 1]);
+            curL = nodeL[cur];
+            curC = nodeC[cur];
+            cur_beta = fmaxf(cur_beta, ffr);
+            if (alive) {
+              const float prev_result =
+                  term<D, K, EXACT>(nodeC[prev], nodeN[prev], alpha, alpha_k, x, y, z);
+              const float diff = cur_result - prev_result;
+              const float wd = (float)nS / (event_prob * (float)(n_end[prev] - n_start[prev])) * diff;
+              wd_all += wd;
+            }
+          }
+          sub_total += wd_all;
+        }
+      }
+      mlcv += sub_total / (float)spl;
+    }
+    result = bh + mlcv;
+  }
+  return {result, cnt, cnt_nodes};
+}
+
+}  // namespace sbh
