@@ -1,18 +1,21 @@
 """Benchmark: queries/s at matched 1e-2 mean relative error on one MI355X.
 
-Workload (default): 3D gravity, N = 2^20 sources drawn from a Plummer sphere
-(scale radius 1, equal masses summing to 1) and Q = 2^20 queries from the same
-distribution; fp32; synthetic data generated here.
+Headline workload (3D gravity, the 1M-source GPU case): N = 2^20 sources drawn
+from a Plummer sphere (scale radius 1, equal masses summing to 1) and
+Q = 2^20 queries from the same distribution; fp32; synthetic data generated
+here with seeded numpy.
 
-A step is one evaluation of all Q queries by the headline method: the
-stochastic control-variate estimator at the smallest sample budget whose mean
-relative error (against a float64 all-pairs reference on the first
---err-queries queries) is <= 1e-2.  Brute force and Barnes-Hut (theta 0.3-0.7)
-are swept and reported beside it at their own matched error.
+A step is one evaluation of all Q queries by the stochastic control-variate
+estimator at the smallest sample budget whose mean relative error (against a
+float64 all-pairs reference on the first --err-queries queries) is <= 1e-2.
+Brute force and Barnes-Hut (theta 0.3-1.0) are swept beside it.  Inputs are
+resident in HBM before timing (torch tensors on cuda:0, passed to the C ABI as
+device pointers); kernel times are hipEvent-timed on the tree's stream.
 
-Inputs are resident in HBM before timing (torch tensors on cuda:0, passed to
-the C ABI as device pointers).  One JSON line is printed on stdout; progress
-goes to stderr.
+Also measured (unless disabled): variants (truncated Plummer, 2D Gaussian
+mixture, winding number and smooth distance on a noisy icosphere), the
+serving loop (8 resident trees, Poisson request stream, p50/p99 latency) and
+the CPU baseline.  One JSON line goes to stdout; progress to stderr.
 """
 from __future__ import annotations
 
@@ -31,8 +34,9 @@ sys.path[:0] = [ROOT, PKG]
 METRIC = "queries/s at 1e-2 rel-err (stoch-BH vs BH vs brute, 1 GPU); p50/p99 latency"
 TARGET_ERR = 1e-2
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
-FLOPS_PER_PAIR_GRAVITY_3D = 13  # add/sub/mul = 1, sqrt/rcp = 1, divide = 2
+# algorithmic flops per pair (add/sub/mul = 1, sqrt/rcp/exp = 1, divide = 2)
+FLOPS_PER_PAIR = {(3, "gravity"): 13, (3, "wn"): 22, (3, "smoothdist"): 12,
+                  (2, "gravity"): 10, (2, "wn"): 14, (2, "smoothdist"): 9}
 
 
 def log(*a):
@@ -41,6 +45,149 @@ def log(*a):
 
 def percentile(x, p):
     return float(np.percentile(np.asarray(x), p)) if len(x) else None
+
+
+# --------------------------------------------------------------------------
+# workloads
+# --------------------------------------------------------------------------
+def make_workload(name, n, q, rmax=None):
+    import workloads as W
+
+    kernel, alpha = "gravity", 1.0
+    if name == "plummer3d":
+        pts, nrm, m, qs = W.plummer(n, q, seed_sources=1, seed_queries=2, rmax=rmax)
+        desc = ("3D gravity, Plummer sphere N=%d sources (equal masses 1/N), Q=%d Plummer "
+                "queries%s" % (n, q, "" if rmax is None else f", truncated at r<={rmax}"))
+    elif name == "gmm2d":
+        pts, nrm, m, qs = W.gmm2d(n, q, seed=3)
+        desc = f"2D gravity, 8-component Gaussian mixture N={n}, Q={q}"
+    elif name in ("icosphere_wn", "icosphere_smoothdist"):
+        level = {1310720: 8, 327680: 7, 81920: 6}.get(n, 8)
+        pts, nrm, m, qs = W.noisy_icosphere(level, q, seed=4)
+        kernel = "wn" if name.endswith("wn") else "smoothdist"
+        alpha = 10.0 if kernel == "smoothdist" else 1.0
+        desc = (f"3D {'winding number' if kernel == 'wn' else 'smooth distance (alpha=10)'}, "
+                f"noisy icosphere level {level} ({len(pts)} area-weighted dipoles), "
+                f"Q={q} queries in the 10%-expanded bbox")
+    else:
+        raise ValueError(name)
+    lo, hi = W.bounds(pts)
+    return dict(name=name, desc=desc, dim=pts.shape[1], kernel=kernel, alpha=alpha, pts=pts,
+                nrm=nrm if kernel == "wn" else None, m=m, queries=qs, lo=lo, hi=hi)
+
+
+def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=False):
+    """Sweep brute force, Barnes-Hut and the stochastic estimator on one
+    workload; optionally time the stochastic configuration at the target
+    error for headline_steps steps."""
+    import torch
+
+    import sbh_capi as C
+
+    dev = torch.device("cuda:0")
+    dim, kern = wl["dim"], wl["kernel"]
+    t1 = C.Tree(wl["pts"], wl["nrm"], wl["m"], wl["lo"], wl["hi"], alpha=wl["alpha"], split_log=1)
+    t2 = C.Tree(wl["pts"], wl["nrm"], wl["m"], wl["lo"], wl["hi"], alpha=wl["alpha"], split_log=2)
+    i1, i2 = t1.info(), t2.info()
+    nch = np.empty(i2["num_nodes"], np.uint32)
+    C.check(C.lib().sbh_tree_nodes(t2.h, None, None, None, nch.ctypes.data, *([None] * 10)))
+    nsub = int(nch[0])
+    log(f"[{wl['name']}] octree1 {i1['num_nodes']} nodes depth {i1['depth']} "
+        f"{t1.build_ms:.1f} ms; octree2 {i2['num_nodes']} nodes depth {i2['depth']} "
+        f"{t2.build_ms:.1f} ms; {nsub} subdomains")
+
+    q32 = wl["queries"].astype(np.float32)
+    nq = len(q32)
+    qd = torch.from_numpy(np.ascontiguousarray(q32.T)).to(dev)
+    out = torch.empty(nq, dtype=torch.float32, device=dev)
+    cnt = torch.empty(2 * nq, dtype=torch.int32, device=dev)
+    ne = min(args.err_queries, nq)
+    qe = torch.from_numpy(np.ascontiguousarray(q32[:ne].T)).to(dev)
+    ref = torch.empty(ne, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ref_ms = t1.eval_f64_device(kern, qe.data_ptr(), ne, ref.data_ptr())
+    ref_np = ref.cpu().numpy()
+    den = np.maximum(np.abs(ref_np), 1.0) if kern == "wn" else np.abs(ref_np)
+
+    def err_of():
+        r = out[:ne].double().cpu().numpy()
+        return float(np.mean(np.abs(r - ref_np) / den))
+
+    def run(tree, method, steps, warm, **kw):
+        for _ in range(warm):
+            tree.eval_device(method, kern, qd.data_ptr(), nq, out.data_ptr(), **kw)
+        torch.cuda.synchronize()
+        wall, kms = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            kms.append(tree.eval_device(method, kern, qd.data_ptr(), nq, out.data_ptr(), **kw))
+            torch.cuda.synchronize()
+            wall.append((time.perf_counter() - t0) * 1e3)
+        return wall, kms
+
+    F = FLOPS_PER_PAIR[(dim, kern)]
+    res = {"workload": wl["desc"], "n_sources": len(wl["pts"]), "n_queries": nq,
+           "subdomains": nsub, "err_queries": ne, "fp64_reference_ms": ref_ms,
+           "build_ms": {"octree1": t1.build_ms, "octree2": t2.build_ms},
+           "trees": {"octree1": i1, "octree2": i2}, "barnes_hut": [], "stochastic": []}
+
+    _, kms = run(t1, C.BRUTE_FORCE, args.brute_steps, 1)
+    ms = float(np.mean(kms))
+    res["brute_force"] = {"kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
+                          "mean_rel_err": err_of(),
+                          "tflops": len(wl["pts"]) * nq * F / (ms / 1e3) / 1e12}
+    log(f"  brute force {ms:.1f} ms err {res['brute_force']['mean_rel_err']:.2e} "
+        f"{res['brute_force']['tflops']:.1f} TFLOP/s")
+    for theta in thetas:
+        _, kms = run(t1, C.BARNES_HUT, args.sweep_steps, 1, beta=1.0 / theta, vote_group=1,
+                     flags=C.EVAL_SORT_QUERIES)
+        ms = float(np.mean(kms))
+        res["barnes_hut"].append({"theta": theta, "sorted": True, "kernel_ms": ms,
+                                  "queries_per_s": nq / (ms / 1e3), "mean_rel_err": err_of()})
+        log(f"  BH theta={theta}: {ms:.2f} ms err {res['barnes_hut'][-1]['mean_rel_err']:.2e}")
+    for spl in spls:
+        for sort in (False, True):
+            flags = C.EVAL_SORT_QUERIES if sort else 0
+            _, kms = run(t2, C.STOCHASTIC, args.sweep_steps, 1, samples_per_subdomain=spl,
+                         seed=1, flags=flags)
+            ms = float(np.mean(kms))
+            res["stochastic"].append({"samples_per_subdomain": spl,
+                                      "samples_per_query": spl * nsub, "sorted": sort,
+                                      "kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
+                                      "mean_rel_err": err_of()})
+            log(f"  SBH spl={spl} sort={sort}: {ms:.2f} ms "
+                f"err {res['stochastic'][-1]['mean_rel_err']:.2e}")
+
+    def at_target(rows):
+        ok = [r for r in rows if r["mean_rel_err"] <= TARGET_ERR]
+        if ok:
+            return min(ok, key=lambda r: r["kernel_ms"]), True
+        return min(rows, key=lambda r: r["mean_rel_err"]), False
+
+    head, head_ok = at_target(res["stochastic"])
+    bh, bh_ok = at_target(res["barnes_hut"])
+    res["stochastic_at_target"] = dict(head, target_met=head_ok)
+    res["barnes_hut_at_target"] = dict(bh, target_met=bh_ok)
+    res["stochastic_over_brute_force"] = head["queries_per_s"] / res["brute_force"]["queries_per_s"]
+    res["barnes_hut_over_brute_force"] = bh["queries_per_s"] / res["brute_force"]["queries_per_s"]
+
+    hflags = C.EVAL_SORT_QUERIES if head["sorted"] else 0
+    hkw = dict(samples_per_subdomain=head["samples_per_subdomain"], seed=1, flags=hflags)
+    if headline_steps:
+        wall, kms = run(t2, C.STOCHASTIC, headline_steps, warmup, **hkw)
+        res["headline"] = {"wall_ms": wall, "kernel_ms": kms, "mean_rel_err": err_of()}
+    if want_counts:
+        t2.eval_device(C.STOCHASTIC, kern, qd.data_ptr(), nq, out.data_ptr(),
+                       counts_ptr=cnt.data_ptr(), samples_per_subdomain=hkw["samples_per_subdomain"],
+                       seed=1, flags=hflags | C.EVAL_SPLIT_COUNTS)
+        c = cnt.cpu().numpy().astype(np.int64)
+        res["terms"] = {"node_terms": int(c[:nq].sum()), "point_terms": int(c[nq:].sum()),
+                        "flops_per_term": F}
+    t1.close()
+    t2.close()
+    del qd, out, cnt, qe, ref
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -53,24 +200,25 @@ def main():
     ap.add_argument("--err-queries", type=int, default=16384)
     ap.add_argument("--sweep-steps", type=int, default=3)
     ap.add_argument("--brute-steps", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="approximate CPU work of the baseline leg")
-    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spl", type=int, nargs="+", default=[1, 4, 16, 64, 256, 512],
                     help="samples per subdomain swept for the stochastic estimator")
     ap.add_argument("--rmax", type=float, default=None,
-                    help="truncate the Plummer sphere at this radius (default: untruncated)")
-    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute"], default=None,
-                    help="profiling mode: build the trees, run only this method --steps times "
-                         "(stochastic at --spl[0], Barnes-Hut at --theta) and exit")
-    ap.add_argument("--theta", type=float, default=1.0)
+                    help="truncate the headline Plummer sphere at this radius")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate CPU work of the baseline leg")
+    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-variants", action="store_true", help="skip the variant workloads")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="Poisson offered load as a fraction of the measured burst capacity")
+    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute"], default=None,
+                    help="profiling mode: build the headline trees, run only this method "
+                         "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
+    ap.add_argument("--theta", type=float, default=1.0)
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="also run a plain device-to-device copy of this many MiB (a counter "
-                         "calibration kernel of known traffic)")
+                    help="with --profile-kernel: also run a plain read+write kernel over this "
+                         "many MiB (a counter calibration kernel of known traffic)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,180 +228,58 @@ def main():
 
     import torch
 
-    import sbh_capi as C
-    import workloads as W
-
     assert torch.cuda.is_available(), "no GPU visible"
-    dev = torch.device("cuda:0")
-
     log(f"generating Plummer N={args.n} Q={args.q}")
-    pts, nrm, m, queries = W.plummer(args.n, args.q, seed_sources=1, seed_queries=2,
-                                     rmax=args.rmax)
-    lo, hi = W.bounds(pts)
-
-    log("building trees on the GPU")
-    t1 = C.Tree(pts, None, m, lo, hi, split_log=1)   # Barnes-Hut (Octree1)
-    t2 = C.Tree(pts, None, m, lo, hi, split_log=2)   # stochastic (Octree2)
-    i1, i2 = t1.info(), t2.info()
-    # a second build of each, timed warm
-    t1b = C.Tree(pts, None, m, lo, hi, split_log=1)
-    t2b = C.Tree(pts, None, m, lo, hi, split_log=2)
-    build_ms = {"octree1": t1b.build_ms, "octree2": t2b.build_ms}
-    t1b.close()
-    t2b.close()
-    log(f"octree1 nodes={i1['num_nodes']} depth={i1['depth']} build={build_ms['octree1']:.1f} ms; "
-        f"octree2 nodes={i2['num_nodes']} depth={i2['depth']} build={build_ms['octree2']:.1f} ms")
-
-    nq = args.q
-    q32 = queries.astype(np.float32)
-    qd = torch.from_numpy(np.ascontiguousarray(q32.T)).to(dev)           # SoA [3, nq]
-    out = torch.empty(nq, dtype=torch.float32, device=dev)
-    cnt = torch.empty(2 * nq, dtype=torch.int32, device=dev)
-    ne = min(args.err_queries, nq)
-    qe = torch.from_numpy(np.ascontiguousarray(q32[:ne].T)).to(dev)
-    ref = torch.empty(ne, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    if args.copy_mib:
-        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"copied {args.copy_mib} MiB x3 (calibration)")
-        del a, b
+    wl = make_workload("plummer3d", args.n, args.q, rmax=args.rmax)
 
     if args.profile_kernel:
-        which = args.profile_kernel
-        for _ in range(args.steps):
-            if which == "stochastic":
-                ms = t2.eval_device(C.STOCHASTIC, "gravity", qd.data_ptr(), nq, out.data_ptr(),
-                                    samples_per_subdomain=args.spl[0], seed=1)
-            elif which == "bh":
-                ms = t1.eval_device(C.BARNES_HUT, "gravity", qd.data_ptr(), nq, out.data_ptr(),
-                                    beta=1.0 / args.theta, vote_group=1,
-                                    flags=C.EVAL_SORT_QUERIES)
-            else:
-                ms = t1.eval_device(C.BRUTE_FORCE, "gravity", qd.data_ptr(), nq, out.data_ptr())
-            log(f"{which}: {ms:.3f} ms")
-        torch.cuda.synchronize()
+        profile(wl, args)
         return
 
-    log(f"float64 reference on {ne} queries")
-    ref_ms = t1.eval_f64_device("gravity", qe.data_ptr(), ne, ref.data_ptr())
-    ref_np = ref.cpu().numpy()
-
-    def err_of(res):
-        r = res[:ne].double().cpu().numpy()
-        return float(np.mean(np.abs(r - ref_np) / np.abs(ref_np)))
-
-    def run(tree, method, steps, warmup, **kw):
-        for _ in range(warmup):
-            tree.eval_device(method, "gravity", qd.data_ptr(), nq, out.data_ptr(), **kw)
-        torch.cuda.synchronize()
-        wall, kern = [], []
-        for _ in range(steps):
-            t0 = time.perf_counter()
-            kern.append(tree.eval_device(method, "gravity", qd.data_ptr(), nq, out.data_ptr(), **kw))
-            torch.cuda.synchronize()
-            wall.append((time.perf_counter() - t0) * 1e3)
-        return wall, kern
-
-    results = {"brute_force": None, "barnes_hut": [], "stochastic": []}
-
-    log("brute force")
-    wall, kern = run(t1, C.BRUTE_FORCE, args.brute_steps, 1)
-    bf_err = err_of(out)
-    bf_ms = float(np.mean(kern))
-    pairs = float(args.n) * nq
-    results["brute_force"] = {
-        "queries_per_s": nq / (bf_ms / 1e3), "kernel_ms": bf_ms, "mean_rel_err": bf_err,
-        "tflops": pairs * FLOPS_PER_PAIR_GRAVITY_3D / (bf_ms / 1e3) / 1e12}
-    log(f"  {bf_ms:.1f} ms  err {bf_err:.2e}  {results['brute_force']['tflops']:.1f} TFLOP/s")
-
-    for theta in (0.3, 0.4, 0.5, 0.6, 0.7, 1.0):
-        for sort in (True,):
-            flags = C.EVAL_SORT_QUERIES if sort else 0
-            wall, kern = run(t1, C.BARNES_HUT, args.sweep_steps, 1, beta=1.0 / theta,
-                             vote_group=1, flags=flags)
-            e = err_of(out)
-            ms = float(np.mean(kern))
-            results["barnes_hut"].append({"theta": theta, "sorted": sort, "kernel_ms": ms,
-                                          "queries_per_s": nq / (ms / 1e3), "mean_rel_err": e})
-            log(f"  BH theta={theta} sort={sort}: {ms:.2f} ms err {e:.2e}")
-
-    nsub = None
-    for spl in args.spl:
-        for sort in (False, True):
-            flags = C.EVAL_SORT_QUERIES if sort else 0
-            wall, kern = run(t2, C.STOCHASTIC, args.sweep_steps, 1, samples_per_subdomain=spl,
-                             seed=1, flags=flags)
-            e = err_of(out)
-            ms = float(np.mean(kern))
-            results["stochastic"].append({"samples_per_subdomain": spl, "sorted": sort,
-                                          "kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
-                                          "mean_rel_err": e})
-            log(f"  SBH spl={spl} sort={sort}: {ms:.2f} ms err {e:.2e}")
-
-    # root children = subdomains of the stochastic tree
-    import sbh_capi
-    nchild = np.empty(i2["num_nodes"], np.uint32)
-    sbh_capi.check(sbh_capi.lib().sbh_tree_nodes(t2.h, None, None, None, nchild.ctypes.data,
-                                                 *([None] * 10)))
-    nsub = int(nchild[0])
-
-    def at_target(rows):
-        ok = [r for r in rows if r["mean_rel_err"] <= TARGET_ERR]
-        if ok:
-            return min(ok, key=lambda r: r["kernel_ms"]), True
-        return min(rows, key=lambda r: r["mean_rel_err"]), False
-
-    head, head_ok = at_target(results["stochastic"])
-    bh_best, _ = at_target(results["barnes_hut"])
-
-    log(f"headline: stochastic spl={head['samples_per_subdomain']} sort={head['sorted']}")
-    hflags = C.EVAL_SORT_QUERIES if head["sorted"] else 0
-    hkw = dict(samples_per_subdomain=head["samples_per_subdomain"], seed=1, flags=hflags)
-    wall, kern = run(t2, C.STOCHASTIC, args.steps, args.warmup, **hkw)
-    head_err = err_of(out)
+    head = measure(wl, args, (0.3, 0.4, 0.5, 0.6, 0.7, 1.0), args.spl,
+                   headline_steps=args.steps, warmup=args.warmup, want_counts=True)
+    hs = head["stochastic_at_target"]
+    wall, kms = head["headline"]["wall_ms"], head["headline"]["kernel_ms"]
     ms_per_step = float(np.mean(wall))
-    value = nq / (ms_per_step / 1e3)
-
-    # roofline of the estimator kernel: FP32 work of the terms it evaluates
-    t2.eval_device(C.STOCHASTIC, "gravity", qd.data_ptr(), nq, out.data_ptr(),
-                   counts_ptr=cnt.data_ptr(), flags=hflags | C.EVAL_SPLIT_COUNTS,
-                   samples_per_subdomain=head["samples_per_subdomain"], seed=1)
-    c = cnt.cpu().numpy().astype(np.int64)
-    node_terms, point_terms = int(c[:nq].sum()), int(c[nq:].sum())
-    # each term is one pair evaluation; the control variate adds one more
-    # node term per accepted level (prev_result), not counted in num_contribs
-    terms = node_terms + point_terms
-    flops = terms * FLOPS_PER_PAIR_GRAVITY_3D
-    kernel_ms = float(np.mean(kern))
-    achieved = flops / (kernel_ms / 1e3) / 1e12
+    value = head["n_queries"] / (ms_per_step / 1e3)
+    kernel_ms = float(np.mean(kms))
+    terms = head["terms"]["node_terms"] + head["terms"]["point_terms"]
+    achieved = terms * head["terms"]["flops_per_term"] / (kernel_ms / 1e3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("stochastic_gravity_plummer_1m")
+            traffic = json.load(open(tpath)).get("k_stochastic_plummer_1m_spl512")
         except Exception:
             traffic = None
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                "kernel": "k_stochastic<3,GRAVITY>", "terms_per_launch": terms,
-                "node_terms": node_terms, "point_terms": point_terms,
-                "flops_per_term": FLOPS_PER_PAIR_GRAVITY_3D, "kernel_ms": kernel_ms}
+                "kernel": "k_stochastic<3,GRAVITY,fast>", "compute": "FP32 VALU (no matrix cores)",
+                "terms_per_launch": terms, "node_terms": head["terms"]["node_terms"],
+                "point_terms": head["terms"]["point_terms"],
+                "flops_per_term": head["terms"]["flops_per_term"], "kernel_ms": kernel_ms}
 
-    serving = None
-    if not args.no_serve:
-        del t1, t2
-        torch.cuda.empty_cache()
-        serving = serving_bench(args)
+    variants = None
+    if not args.no_variants:
+        variants = []
+        specs = [("plummer3d", 1 << 20, 1 << 20, 10.0, (0.5, 1.0), (1, 4, 8, 16)),
+                 ("gmm2d", 1 << 20, 1 << 20, None, (0.5, 1.0), (1, 4, 16, 64)),
+                 ("icosphere_wn", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16)),
+                 ("icosphere_smoothdist", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16))]
+        vargs = argparse.Namespace(**vars(args))
+        vargs.sweep_steps, vargs.brute_steps = 2, 1
+        for name, n, q, rmax, thetas, spls in specs:
+            log(f"variant {name}")
+            v = measure(make_workload(name, n, q, rmax=rmax), vargs, thetas, spls)
+            variants.append({k: v[k] for k in (
+                "workload", "n_sources", "n_queries", "subdomains", "build_ms", "brute_force",
+                "stochastic_at_target", "barnes_hut_at_target", "stochastic_over_brute_force",
+                "barnes_hut_over_brute_force", "barnes_hut", "stochastic")})
 
+    serving = None if args.no_serve else serving_bench(args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(pts, m, lo, hi, q32, head["samples_per_subdomain"], args.cpu_seconds,
-                           value)
+        cpu = cpu_baseline(wl, hs["samples_per_subdomain"], args.cpu_seconds, value)
 
     line = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1,
@@ -261,16 +287,14 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "3D gravity, Plummer sphere N=2^20 sources (equal masses 1/N), "
-                        "Q=2^20 Plummer queries, stochastic Barnes-Hut at <=1e-2 mean rel err"
-                        + ("" if args.rmax is None else f", truncated at r<={args.rmax}"),
-            "plummer_rmax": args.rmax,
-            "n_sources": args.n, "n_queries": nq, "method": "stochastic",
-            "samples_per_subdomain": head["samples_per_subdomain"], "subdomains": nsub,
-            "samples_per_query": head["samples_per_subdomain"] * nsub,
-            "sorted_queries": head["sorted"], "mean_rel_err": head_err,
-            "err_queries": ne, "tree": "octree k=2, leaf size 1",
-            "target_err_met": head_ok,
+            "workload": head["workload"] + ", stochastic Barnes-Hut at <=1e-2 mean rel err",
+            "n_sources": head["n_sources"], "n_queries": head["n_queries"],
+            "plummer_rmax": args.rmax, "method": "stochastic",
+            "samples_per_subdomain": hs["samples_per_subdomain"],
+            "subdomains": head["subdomains"], "samples_per_query": hs["samples_per_query"],
+            "sorted_queries": hs["sorted"], "mean_rel_err": head["headline"]["mean_rel_err"],
+            "target_err_met": hs["target_met"], "err_queries": head["err_queries"],
+            "tree": "octree k=2, leaf size 1",
         },
         "latency_ms": ({"p50": serving["poisson"]["p50_ms"], "p99": serving["poisson"]["p99_ms"],
                         "kind": "serving request latency, submit to results in host memory, "
@@ -279,20 +303,56 @@ def main():
                        if serving else None),
         "step_latency_ms": {"p50": percentile(wall, 50), "p99": percentile(wall, 99),
                             "kind": "per 2^20-query evaluation, wall clock"},
-        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "speedup_vs_brute_force": value / results["brute_force"]["queries_per_s"],
-        "barnes_hut_at_target": bh_best,
-        "methods": results,
-        "build_ms": build_ms,
-        "trees": {"octree1": i1, "octree2": i2},
-        "fp64_reference_ms": ref_ms,
+        "speedup_vs_brute_force": value / head["brute_force"]["queries_per_s"],
+        "brute_force": head["brute_force"],
+        "barnes_hut_at_target": head["barnes_hut_at_target"],
+        "barnes_hut_over_brute_force": head["barnes_hut_over_brute_force"],
+        "methods": {"barnes_hut": head["barnes_hut"], "stochastic": head["stochastic"]},
+        "build_ms": head["build_ms"], "trees": head["trees"],
+        "variants": variants,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 
 
-SERVE_TREES = [  # name, dim, kernel, points, generator (as in the 1M-point cases)
+def profile(wl, args):
+    """Profiling mode: only the requested method on the headline trees."""
+    import torch
+
+    import sbh_capi as C
+
+    dev = torch.device("cuda:0")
+    if args.copy_mib:
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)  # one read and one write of copy_mib MiB
+        torch.cuda.synchronize()
+        log(f"read+write {args.copy_mib} MiB x3 (calibration)")
+        del a, b
+    t = C.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"],
+               split_log=2 if args.profile_kernel == "stochastic" else 1)
+    q32 = wl["queries"].astype(np.float32)
+    nq = len(q32)
+    qd = torch.from_numpy(np.ascontiguousarray(q32.T)).to(dev)
+    out = torch.empty(nq, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        if args.profile_kernel == "stochastic":
+            ms = t.eval_device(C.STOCHASTIC, "gravity", qd.data_ptr(), nq, out.data_ptr(),
+                               samples_per_subdomain=args.spl[0], seed=1)
+        elif args.profile_kernel == "bh":
+            ms = t.eval_device(C.BARNES_HUT, "gravity", qd.data_ptr(), nq, out.data_ptr(),
+                               beta=1.0 / args.theta, vote_group=1, flags=C.EVAL_SORT_QUERIES)
+        else:
+            ms = t.eval_device(C.BRUTE_FORCE, "gravity", qd.data_ptr(), nq, out.data_ptr())
+        log(f"{args.profile_kernel}: {ms:.3f} ms")
+    torch.cuda.synchronize()
+
+
+SERVE_TREES = [  # name, dim, kernel, points
     ("plummer_r10_3d_gravity_64k", 3, "gravity", 1 << 16),
     ("gmm_2d_gravity_128k", 2, "gravity", 1 << 17),
     ("icosphere7_3d_wn_328k", 3, "wn", 327680),
@@ -360,15 +420,13 @@ def serving_bench(args):
         el = time.perf_counter() - t0
         return np.asarray(lats), el
 
-    # warm-up (first launches compile nothing, but touch every code path)
     for h in [srv.submit(t["id"], m_, t["kernel"], p[:2000], beta=2.0, samples_per_subdomain=s_,
                          seed=1) for (t, m_, p, s_, _) in reqs[:16]]:
         h.result(600.0)
     lat_b, el_b = run(np.zeros(nreq))
     cap = total_q / el_b
     log(f"serve burst: {nreq} requests, {total_q} queries in {el_b:.2f} s -> {cap / 1e6:.1f} M q/s")
-    mean_q = total_q / nreq
-    rate = args.serve_load * cap / mean_q
+    rate = args.serve_load * cap / (total_q / nreq)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     lat_p, el_p = run(arrivals)
     log(f"serve poisson: rate {rate:.1f} req/s, p50 {np.percentile(lat_p, 50):.1f} ms, "
@@ -391,16 +449,17 @@ def serving_bench(args):
     }
 
 
-def cpu_baseline(pts, m, lo, hi, q32, spl, budget_s, gpu_qps):
-    """The oracle (a C++ restatement of the reference's CPU estimator with the
-    same threading model) timed on this machine's host cores on a bounded
-    sample of the same workload."""
+def cpu_baseline(wl, spl, budget_s, gpu_qps):
+    """CPU baseline leg: the oracle (a C++ restatement of the reference's CPU
+    estimator with the same threading model) timed on this machine's host
+    cores on a bounded sample of the headline workload."""
     import oracle
 
     threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
-    ot = oracle.Tree(pts, None, m, lo, hi, split_log=2)
+    ot = oracle.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"], split_log=2)
     build_s = time.perf_counter() - t0
+    q32 = wl["queries"].astype(np.float32)
     probe = q32[:2048]
     t0 = time.perf_counter()
     ot.stochastic("gravity", probe, spl, seed=1, nthreads=threads)
@@ -414,8 +473,8 @@ def cpu_baseline(pts, m, lo, hi, q32, spl, budget_s, gpu_qps):
         f"oracle tree build {build_s:.1f} s")
     return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"stochastic estimator, samples_per_subdomain={spl}, first {nsample} of the "
-                      f"2^20 Plummer queries against the 2^20-source octree (k=2); tree build "
-                      f"excluded ({build_s:.1f} s single-threaded)",
+                      f"{len(q32)} Plummer queries against the {len(wl['pts'])}-source octree "
+                      f"(k=2); tree build excluded ({build_s:.1f} s single-threaded)",
             "gpu_over_cpu": gpu_qps / qps}
 
 

@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
   load_query(qx, qy, qz, qi, D, x, y, z);
   const LaneResult r = stochastic_estimate<D, K, EXACT>(
       t, spl, key0, key1, g, live, x, y, z, sSrc + wbase,
-      sNrm + (K == WINDING_NUMBER ? wbase : 0), sInfo + wbase);
+      sNrm + (K == WINDING_NUMBER ? wbase : 0), sInfo + wbase, counts != nullptr);
   if (live) {
     out[qi] = r.value;
     if (counts) {

@@ -142,16 +142,20 @@ template <int D, int K, bool EXACT>
 __device__ __forceinline__ float node_term(const TreeView& t, uint32_t v, float x, float y,
                                            float z, int32_t& cnt, int32_t& cnt_nodes) {
   const uint4 l = t.nodeL[v];
+  const bool leaf = (l.w & kLeafBit) != 0;
   float s = 0.0f;
-  if (l.w & kLeafBit) {
+  uint32_t np = 0;
+  if (leaf) {
     const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
     for (uint32_t j = p0; j < p1; ++j)
       s += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
-    cnt += (int32_t)(p1 - p0);
+    np = p1 - p0;
   } else {
     s = term<D, K, EXACT>(t.nodeC[v], t.nodeN[v], t.alpha, t.alpha_k, x, y, z);
-    cnt_nodes += 1;
   }
+  // value selects, not a selected reference: both counters stay in registers
+  cnt += (int32_t)np;
+  cnt_nodes += leaf ? 0 : 1;
   return s;
 }
 
@@ -168,7 +172,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
                                                           uint32_t key0, uint32_t key1,
                                                           uint32_t g, bool live, float x, float y,
                                                           float z, float4* sSrc, float4* sNrm,
-                                                          uint4* sInfo) {
+                                                          uint4* sInfo, bool count = true) {
 #pragma clang fp contract(off)
   constexpr bool kWN = (K == WINDING_NUMBER);
   const uint32_t lane = threadIdx.x & 63;
@@ -291,7 +295,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
                 }
               }
               // term counts of this batch, the same for every lane that evaluated it
-              {
+              if (count) {
                 uint32_t pt = np_lane, nd = (valid && np_lane == 0) ? 1u : 0u;
 #pragma unroll
                 for (int off = 32; off >= 1; off >>= 1) {

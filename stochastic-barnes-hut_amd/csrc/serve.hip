@@ -64,7 +64,7 @@ __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, ui
                                            float4* sNrm, uint4* sInfo) {
   if (stochastic)
     return stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
-                                            sSrc, sNrm, sInfo)
+                                            sSrc, sNrm, sInfo, false)
         .value;
   return bh_walk<D, K, 1, false>(r.tree, r.beta, live, x, y, z).value;
 }
