@@ -66,6 +66,11 @@ typedef struct sbh_tree sbh_tree;
                                       fast-math GPU build uses (CMakeLists.txt:27).
                                       Opening tests, roulette and weights are exact in
                                       both modes */
+#define SBH_EVAL_MFMA 16           /* brute force, 3D gravity / smooth distance: squared
+                                      distances |q'|^2 + |p'|^2 - 2 q'.p' on the matrix
+                                      cores (V_MFMA_F32_16X16X4_F32) in coordinates
+                                      centred per 64-query wavefront, near-field pairs
+                                      recomputed directly */
 
 typedef struct sbh_eval_params {
   float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */

@@ -316,12 +316,24 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     }
     // Counting only exists for the tree walks.
     if (method == SBH_BRUTE_FORCE) {
-      uint32_t ns = sbh::brute_split_count(t.n, nq);
+      const bool mfma = (p.flags & SBH_EVAL_MFMA) != 0;
+      if (mfma)
+        SBH_REQUIRE(t.dim == 3 && (kernel == SBH_GRAVITY || kernel == SBH_SMOOTH_DIST),
+                    "SBH_EVAL_MFMA covers 3D gravity and smooth distance");
+      uint32_t ns = mfma ? sbh::brute_mfma_split_count(t.n, nq) : sbh::brute_split_count(t.n, nq);
       if (ns > 1) ensure(tree->partial, (size_t)ns * nq);
-      const bool sort = false;
-      (void)sort;
+      if (mfma) {
+        ensure(tree->order, nq);
+        ensure(tree->sort_scratch, sbh::sort_scratch_u32(nq));
+      }
       SBH_HIP(hipEventRecord(tree->ev0, s));
-      sbh::run_brute(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);
+      if (mfma) {
+        sbh::query_order(t, dq, nq, tree->order.get(), tree->sort_scratch.get(), s);
+        sbh::run_brute_mfma(t, kernel, dq, nq, tree->order.get(), dout, tree->partial.get(), ns,
+                            s);
+      } else {
+        sbh::run_brute(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);
+      }
       SBH_HIP(hipEventRecord(tree->ev1, s));
       if (dcnt) {
         // every query evaluates every point; split counts have no node terms

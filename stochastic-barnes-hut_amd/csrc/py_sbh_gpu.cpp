@@ -195,7 +195,7 @@ void bind_tree(py::module_& m, const char* name) {
 
 py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, float beta,
                     int spl, uint64_t seed, int vote_group, bool sort, bool counts,
-                    bool exact = false) {
+                    bool exact = false, bool mfma = false) {
   std::vector<float> q = t.soa(queries);
   const int64_t nq = queries.shape(0);
   py::array_t<float> out(nq);
@@ -205,7 +205,8 @@ py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, fl
   p.samples_per_subdomain = spl;
   p.seed = seed;
   p.vote_group = vote_group;
-  p.flags = (sort ? SBH_EVAL_SORT_QUERIES : 0) | (exact ? SBH_EVAL_EXACT_TERMS : 0);
+  p.flags = (sort ? SBH_EVAL_SORT_QUERIES : 0) | (exact ? SBH_EVAL_EXACT_TERMS : 0) |
+            (mfma ? SBH_EVAL_MFMA : 0);
   float ms = 0.0f;
   float* op = out.mutable_data();
   int32_t* cp = counts ? cnt.mutable_data() : nullptr;
@@ -222,10 +223,10 @@ py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, fl
 void bind_kernel(py::module_& m, const char* suffix, int kernel) {
   std::string s = suffix;
   m.def(("eval_brute_force_" + s).c_str(),
-        [kernel](GPUTreeBase& t, F32Array q) {
-          return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false);
+        [kernel](GPUTreeBase& t, F32Array q, bool mfma) {
+          return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false, false, mfma);
         },
-        "tree"_a, "queries"_a);
+        "tree"_a, "queries"_a, "mfma"_a = false);
   m.def(("eval_barnes_hut_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, float beta, int vote_group, bool sort_queries,
                  bool return_counts, bool exact_terms) {

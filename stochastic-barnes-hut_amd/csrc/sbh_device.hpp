@@ -77,15 +77,16 @@ __device__ __forceinline__ float ratio_exact(float4 c, float diam, float qx, flo
 // hardware sqrt and reciprocal (within about 1 ulp each) settle every case
 // farther than 1e-6 from the threshold; the rest take the correctly rounded
 // path.
+// inv_diam = hardware reciprocal of diam, computed by the caller.
 template <int D>
-__device__ __forceinline__ bool opens_far(float4 c, float diam, float beta, float qx, float qy,
-                                          float qz) {
+__device__ __forceinline__ bool opens_far(float4 c, float diam, float inv_diam, float beta,
+                                          float qx, float qy, float qz) {
 #pragma clang fp contract(off)
   float dx = qx - c.x;
   float dy = qy - c.y;
   float dz = (D == 3) ? (qz - c.z) : 0.0f;
   float d2 = sqnorm_exact<D>(dx, dy, dz);
-  float r = __builtin_amdgcn_sqrtf(d2) * __builtin_amdgcn_rcpf(diam);
+  float r = __builtin_amdgcn_sqrtf(d2) * inv_diam;
   if (d2 >= 1e-30f) {
     if (r >= beta * 1.000001f) return true;
     if (r <= beta * 0.999999f) return false;

@@ -94,8 +94,12 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
     const float4 c = nodeC[v];
     const uint4 l = nodeL[v];
     const bool leaf = (l.w & kLeafBit) != 0;
+    // every field of the record is consumed here, so its scalar loads issue
+    // together at the top of the step
+    const float diam = __uint_as_float(l.x);
+    const float inv_diam = __builtin_amdgcn_rcpf(diam);
     bool reject = false;
-    if (active && !leaf) reject = !opens_far<D>(c, __uint_as_float(l.x), beta, x, y, z);
+    if (active && !leaf) reject = !opens_far<D>(c, diam, inv_diam, beta, x, y, z);
     const uint64_t rej = __ballot(reject);
     const bool accept = active && ((rej & my_group) == 0);
     if (accept) {

@@ -160,6 +160,27 @@ def test_brute_force(sbh_gpu, dim, kernel):
     assert rel_err(g64, ref64, kernel).max() <= 1e-12
 
 
+@pytest.mark.parametrize("kernel", ["gravity", "smoothdist"])
+@pytest.mark.parametrize("workload", ["cube", "plummer"])
+def test_brute_force_mfma(sbh_gpu, kernel, workload):
+    """Matrix-core squared distances with the near field recomputed: within
+    1e-5 of the oracle and of float64, for every query (ragged count)."""
+    if workload == "cube":
+        pts, nrm, m, q = W.uniform_cube(9000, 3001, seed=27)
+    else:
+        pts, nrm, m, q = W.plummer(9000, 3001)
+    lo, hi = W.bounds(pts)
+    alpha = 10.0 if kernel == "smoothdist" else 1.0
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1, alpha=alpha)
+    got, ms = getattr(sbh_gpu, "eval_brute_force_" + kernel)(g, q, mfma=True)
+    ref = o.brute_force(kernel, q)
+    ref64 = o.brute_force(kernel, q, fp64=True)
+    err = rel_err(got, ref64, kernel)
+    print(f"[mfma] {kernel} {workload}: max {err.max():.2e} mean {err.mean():.2e}")
+    assert rel_err(got, ref, kernel).max() <= TOL
+    assert err.max() <= TOL
+
+
 def test_brute_force_ill_conditioned_wn(sbh_gpu):
     """Random dipole directions cancel; the error bar is then relative to the
     sum's condition (sum of |terms|), the bound fp32 terms can meet."""
