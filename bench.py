@@ -34,6 +34,12 @@ sys.path[:0] = [ROOT, PKG]
 METRIC = "queries/s at 1e-2 rel-err (stoch-BH vs BH vs brute, 1 GPU); p50/p99 latency"
 TARGET_ERR = 1e-2
 FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+# record bytes per term for 3D gravity: node = centroid 12 + mass 4 + diameter 4 +
+# child range 8, padded to 32; point = position 12 + mass 4
+NODE_RECORD_BYTES, POINT_RECORD_BYTES = 32, 16
+# V_MFMA_F32_16X16X4_F32: 16*16*4 multiply-adds; one per 16 x 16 query-source block
+MFMA_FLOPS_PER_PAIR = 8
 # algorithmic flops per pair (add/sub/mul = 1, sqrt/rcp/exp = 1, divide = 2)
 FLOPS_PER_PAIR = {(3, "gravity"): 13, (3, "wn"): 22, (3, "smoothdist"): 12,
                   (2, "gravity"): 10, (2, "wn"): 14, (2, "smoothdist"): 9}
@@ -69,6 +75,12 @@ def make_workload(name, n, q, rmax=None):
         desc = (f"3D {'winding number' if kernel == 'wn' else 'smooth distance (alpha=10)'}, "
                 f"noisy icosphere level {level} ({len(pts)} area-weighted dipoles), "
                 f"Q={q} queries in the 10%-expanded bbox")
+    elif name in ("cube3d_gravity", "cube3d_smoothdist"):
+        pts, nrm, m, qs = W.uniform_cube(n, q, dim=3, seed=0)
+        kernel = "gravity" if name.endswith("gravity") else "smoothdist"
+        alpha = 10.0 if kernel == "smoothdist" else 1.0
+        desc = (f"3D {'gravity' if kernel == 'gravity' else 'smooth distance (alpha=10)'}, "
+                f"uniform unit cube N={n} (masses U[0.5,1.5]), Q={q} queries in [-0.1,1.1]^3")
     else:
         raise ValueError(name)
     lo, hi = W.bounds(pts)
@@ -89,12 +101,15 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
     t1 = C.Tree(wl["pts"], wl["nrm"], wl["m"], wl["lo"], wl["hi"], alpha=wl["alpha"], split_log=1)
     t2 = C.Tree(wl["pts"], wl["nrm"], wl["m"], wl["lo"], wl["hi"], alpha=wl["alpha"], split_log=2)
     i1, i2 = t1.info(), t2.info()
-    nch = np.empty(i2["num_nodes"], np.uint32)
-    C.check(C.lib().sbh_tree_nodes(t2.h, None, None, None, nch.ctypes.data, *([None] * 10)))
-    nsub = int(nch[0])
+    def root_children(t, info):
+        nch = np.empty(info["num_nodes"], np.uint32)
+        C.check(C.lib().sbh_tree_nodes(t.h, None, None, None, nch.ctypes.data, *([None] * 10)))
+        return int(nch[0])
+
+    nsub, nsub1 = root_children(t2, i2), root_children(t1, i1)
     log(f"[{wl['name']}] octree1 {i1['num_nodes']} nodes depth {i1['depth']} "
         f"{t1.build_ms:.1f} ms; octree2 {i2['num_nodes']} nodes depth {i2['depth']} "
-        f"{t2.build_ms:.1f} ms; {nsub} subdomains")
+        f"{t2.build_ms:.1f} ms; {nsub} subdomains (k=2), {nsub1} (k=1)")
 
     q32 = wl["queries"].astype(np.float32)
     nq = len(q32)
@@ -108,10 +123,21 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
     ref_ms = t1.eval_f64_device(kern, qe.data_ptr(), ne, ref.data_ptr())
     ref_np = ref.cpu().numpy()
     den = np.maximum(np.abs(ref_np), 1.0) if kern == "wn" else np.abs(ref_np)
+    if kern == "smoothdist":
+        # the distance form -log(w)/alpha, relative to max(|d|, 1/alpha)
+        ref_d = -np.log(ref_np) / wl["alpha"]
+        den_d = np.maximum(np.abs(ref_d), 1.0 / wl["alpha"])
 
     def err_of():
         r = out[:ne].double().cpu().numpy()
         return float(np.mean(np.abs(r - ref_np) / den))
+
+    def dist_err_of():
+        if kern != "smoothdist":
+            return None
+        r = out[:ne].double().cpu().numpy()
+        d = -np.log(np.maximum(r, 1e-300)) / wl["alpha"]
+        return float(np.mean(np.abs(d - ref_d) / den_d))
 
     def run(tree, method, steps, warm, **kw):
         for _ in range(warm):
@@ -134,29 +160,47 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
     _, kms = run(t1, C.BRUTE_FORCE, args.brute_steps, 1)
     ms = float(np.mean(kms))
     res["brute_force"] = {"kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
-                          "mean_rel_err": err_of(),
+                          "mean_rel_err": err_of(), "distance_rel_err": dist_err_of(),
                           "tflops": len(wl["pts"]) * nq * F / (ms / 1e3) / 1e12}
     log(f"  brute force {ms:.1f} ms err {res['brute_force']['mean_rel_err']:.2e} "
         f"{res['brute_force']['tflops']:.1f} TFLOP/s")
+    if dim == 3 and kern in ("gravity", "smoothdist"):
+        # matrix-core distances; the timed region includes its Morton query sort
+        _, kms = run(t1, C.BRUTE_FORCE, args.brute_steps, 1, flags=C.EVAL_MFMA)
+        ms = float(np.mean(kms))
+        res["brute_force_mfma"] = {"kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
+                                   "mean_rel_err": err_of(),
+                                   "tflops": len(wl["pts"]) * nq * F / (ms / 1e3) / 1e12}
+        log(f"  brute force (MFMA) {ms:.1f} ms err "
+            f"{res['brute_force_mfma']['mean_rel_err']:.2e} "
+            f"{res['brute_force_mfma']['tflops']:.1f} TFLOP/s equivalent")
     for theta in thetas:
         _, kms = run(t1, C.BARNES_HUT, args.sweep_steps, 1, beta=1.0 / theta, vote_group=1,
                      flags=C.EVAL_SORT_QUERIES)
         ms = float(np.mean(kms))
         res["barnes_hut"].append({"theta": theta, "sorted": True, "kernel_ms": ms,
-                                  "queries_per_s": nq / (ms / 1e3), "mean_rel_err": err_of()})
-        log(f"  BH theta={theta}: {ms:.2f} ms err {res['barnes_hut'][-1]['mean_rel_err']:.2e}")
-    for spl in spls:
+                                  "queries_per_s": nq / (ms / 1e3), "mean_rel_err": err_of(),
+                                  "distance_rel_err": dist_err_of()})
+        log(f"  BH theta={theta}: {ms:.2f} ms err {res['barnes_hut'][-1]['mean_rel_err']:.2e}"
+            + ("" if kern != "smoothdist"
+               else f" (distance {res['barnes_hut'][-1]['distance_rel_err']:.2e})"))
+    trees = {2: (t2, nsub), 1: (t1, nsub1)}
+    for k, spl in [(2, s) for s in spls] + [(1, s) for s in spls]:
+        tk, nsk = trees[k]
         for sort in (False, True):
             flags = C.EVAL_SORT_QUERIES if sort else 0
-            _, kms = run(t2, C.STOCHASTIC, args.sweep_steps, 1, samples_per_subdomain=spl,
+            _, kms = run(tk, C.STOCHASTIC, args.sweep_steps, 1, samples_per_subdomain=spl,
                          seed=1, flags=flags)
             ms = float(np.mean(kms))
-            res["stochastic"].append({"samples_per_subdomain": spl,
-                                      "samples_per_query": spl * nsub, "sorted": sort,
+            res["stochastic"].append({"tree_k": k, "samples_per_subdomain": spl,
+                                      "samples_per_query": spl * nsk, "sorted": sort,
                                       "kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
-                                      "mean_rel_err": err_of()})
-            log(f"  SBH spl={spl} sort={sort}: {ms:.2f} ms "
-                f"err {res['stochastic'][-1]['mean_rel_err']:.2e}")
+                                      "mean_rel_err": err_of(),
+                                      "distance_rel_err": dist_err_of()})
+            log(f"  SBH k={k} spl={spl} sort={sort}: {ms:.2f} ms "
+                f"err {res['stochastic'][-1]['mean_rel_err']:.2e}"
+                + ("" if kern != "smoothdist"
+                   else f" (distance {res['stochastic'][-1]['distance_rel_err']:.2e})"))
 
     def at_target(rows):
         ok = [r for r in rows if r["mean_rel_err"] <= TARGET_ERR]
@@ -173,11 +217,12 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
 
     hflags = C.EVAL_SORT_QUERIES if head["sorted"] else 0
     hkw = dict(samples_per_subdomain=head["samples_per_subdomain"], seed=1, flags=hflags)
+    th = trees[head["tree_k"]][0]
     if headline_steps:
-        wall, kms = run(t2, C.STOCHASTIC, headline_steps, warmup, **hkw)
+        wall, kms = run(th, C.STOCHASTIC, headline_steps, warmup, **hkw)
         res["headline"] = {"wall_ms": wall, "kernel_ms": kms, "mean_rel_err": err_of()}
     if want_counts:
-        t2.eval_device(C.STOCHASTIC, kern, qd.data_ptr(), nq, out.data_ptr(),
+        th.eval_device(C.STOCHASTIC, kern, qd.data_ptr(), nq, out.data_ptr(),
                        counts_ptr=cnt.data_ptr(), samples_per_subdomain=hkw["samples_per_subdomain"],
                        seed=1, flags=hflags | C.EVAL_SPLIT_COUNTS)
         c = cnt.cpu().numpy().astype(np.int64)
@@ -212,7 +257,12 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="Poisson offered load as a fraction of the measured burst capacity")
-    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute"], default=None,
+    ap.add_argument("--tree-k", type=int, default=2,
+                    help="profiling mode: split_log of the stochastic estimator's tree")
+    ap.add_argument("--no-sort", action="store_true",
+                    help="profiling mode: evaluate the stochastic estimator in input order")
+    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute", "mfma"],
+                    default=None,
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
     ap.add_argument("--theta", type=float, default=1.0)
@@ -243,8 +293,11 @@ def main():
     ms_per_step = float(np.mean(wall))
     value = head["n_queries"] / (ms_per_step / 1e3)
     kernel_ms = float(np.mean(kms))
-    terms = head["terms"]["node_terms"] + head["terms"]["point_terms"]
-    achieved = terms * head["terms"]["flops_per_term"] / (kernel_ms / 1e3) / 1e12
+    nt, pt = head["terms"]["node_terms"], head["terms"]["point_terms"]
+    terms = nt + pt
+    alg_bytes = nt * NODE_RECORD_BYTES + pt * POINT_RECORD_BYTES
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    tflops = terms * head["terms"]["flops_per_term"] / (kernel_ms / 1e3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -252,34 +305,45 @@ def main():
             traffic = json.load(open(tpath)).get("k_stochastic_plummer_1m_spl512")
         except Exception:
             traffic = None
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                "kernel": "k_stochastic<3,GRAVITY,fast>", "compute": "FP32 VALU (no matrix cores)",
-                "terms_per_launch": terms, "node_terms": head["terms"]["node_terms"],
-                "point_terms": head["terms"]["point_terms"],
-                "flops_per_term": head["terms"]["flops_per_term"], "kernel_ms": kernel_ms}
+    roofline = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "kernel": "k_stochastic<3,GRAVITY,fast>", "kernel_ms": kernel_ms,
+        "algorithmic_bytes_per_launch": alg_bytes,
+        "bytes_per_node_term": NODE_RECORD_BYTES, "bytes_per_point_term": POINT_RECORD_BYTES,
+        "node_terms": nt, "point_terms": pt,
+        "note": "algorithmic bytes count every record read per query; a wavefront shares "
+                "its sample paths, so records are served from LDS and L2 and frac exceeds 1. "
+                "traffic is the measured HBM bytes per launch",
+        "measured_hbm_frac": (traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS
+                              if traffic else None),
+        "compute": {"unit": "TFLOP/s", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                    "frac": tflops / FP32_PEAK_TFLOPS, "flops_per_term": head["terms"]["flops_per_term"],
+                    "kind": "FP32 VALU"}}
 
     variants = None
     if not args.no_variants:
         variants = []
         specs = [("plummer3d", 1 << 20, 1 << 20, 10.0, (0.5, 1.0), (1, 4, 8, 16)),
                  ("gmm2d", 1 << 20, 1 << 20, None, (0.5, 1.0), (1, 4, 16, 64)),
-                 ("icosphere_wn", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16)),
+                 ("icosphere_wn", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16, 64)),
                  ("icosphere_smoothdist", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16))]
         vargs = argparse.Namespace(**vars(args))
         vargs.sweep_steps, vargs.brute_steps = 2, 1
         for name, n, q, rmax, thetas, spls in specs:
             log(f"variant {name}")
             v = measure(make_workload(name, n, q, rmax=rmax), vargs, thetas, spls)
-            variants.append({k: v[k] for k in (
+            variants.append({k: v.get(k) for k in (
                 "workload", "n_sources", "n_queries", "subdomains", "build_ms", "brute_force",
-                "stochastic_at_target", "barnes_hut_at_target", "stochastic_over_brute_force",
-                "barnes_hut_over_brute_force", "barnes_hut", "stochastic")})
+                "brute_force_mfma", "stochastic_at_target", "barnes_hut_at_target",
+                "stochastic_over_brute_force", "barnes_hut_over_brute_force", "barnes_hut",
+                "stochastic")})
 
+    precision = None if args.no_variants else precision_variant(args)
     serving = None if args.no_serve else serving_bench(args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(wl, hs["samples_per_subdomain"], args.cpu_seconds, value)
+        cpu = cpu_baseline(wl, hs["samples_per_subdomain"], hs["tree_k"], args.cpu_seconds, value)
 
     line = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1,
@@ -291,10 +355,11 @@ def main():
             "n_sources": head["n_sources"], "n_queries": head["n_queries"],
             "plummer_rmax": args.rmax, "method": "stochastic",
             "samples_per_subdomain": hs["samples_per_subdomain"],
-            "subdomains": head["subdomains"], "samples_per_query": hs["samples_per_query"],
+            "subdomains": hs["samples_per_query"] // hs["samples_per_subdomain"],
+            "samples_per_query": hs["samples_per_query"],
             "sorted_queries": hs["sorted"], "mean_rel_err": head["headline"]["mean_rel_err"],
             "target_err_met": hs["target_met"], "err_queries": head["err_queries"],
-            "tree": "octree k=2, leaf size 1",
+            "tree": f"octree k={hs['tree_k']}, leaf size 1",
         },
         "latency_ms": ({"p50": serving["poisson"]["p50_ms"], "p99": serving["poisson"]["p99_ms"],
                         "kind": "serving request latency, submit to results in host memory, "
@@ -307,14 +372,67 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_brute_force": value / head["brute_force"]["queries_per_s"],
         "brute_force": head["brute_force"],
+        "brute_force_mfma": head.get("brute_force_mfma"),
         "barnes_hut_at_target": head["barnes_hut_at_target"],
         "barnes_hut_over_brute_force": head["barnes_hut_over_brute_force"],
         "methods": {"barnes_hut": head["barnes_hut"], "stochastic": head["stochastic"]},
         "build_ms": head["build_ms"], "trees": head["trees"],
         "variants": variants,
+        "precision_variant": precision,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
+
+
+def precision_variant(args):
+    """Brute force on the 2^18 x 2^18 uniform cube (gravity, smooth distance):
+    the direct fp32 kernel and the matrix-core kernel, each with its float64
+    error per result over all queries."""
+    import torch
+
+    import sbh_capi as C
+
+    dev = torch.device("cuda:0")
+    rows = []
+    for name in ("cube3d_gravity", "cube3d_smoothdist"):
+        wl = make_workload(name, 1 << 18, 1 << 18)
+        kern = wl["kernel"]
+        t = C.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"], alpha=wl["alpha"], split_log=1)
+        q32 = wl["queries"].astype(np.float32)
+        nq = len(q32)
+        qd = torch.from_numpy(np.ascontiguousarray(q32.T)).to(dev)
+        out = torch.empty(nq, dtype=torch.float32, device=dev)
+        ref = torch.empty(nq, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ref_ms = t.eval_f64_device(kern, qd.data_ptr(), nq, ref.data_ptr())
+        ref_np = ref.cpu().numpy()
+        F = FLOPS_PER_PAIR[(3, kern)]
+        pairs = float(len(wl["pts"])) * nq
+        row = {"workload": wl["desc"], "fp64_ms": ref_ms}
+        for label, flags in (("direct", 0), ("mfma", C.EVAL_MFMA)):
+            t.eval_device(C.BRUTE_FORCE, kern, qd.data_ptr(), nq, out.data_ptr(), flags=flags)
+            torch.cuda.synchronize()
+            kms = [t.eval_device(C.BRUTE_FORCE, kern, qd.data_ptr(), nq, out.data_ptr(),
+                                 flags=flags) for _ in range(max(1, args.brute_steps))]
+            ms = float(np.mean(kms))
+            rel = np.abs(out.double().cpu().numpy() - ref_np) / np.abs(ref_np)
+            row[label] = {"kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
+                          "tflops": pairs * F / (ms / 1e3) / 1e12,
+                          "frac_fp32_peak": pairs * F / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS,
+                          "mean_rel_err": float(rel.mean()), "max_rel_err": float(rel.max())}
+            if label == "mfma":
+                mf = pairs * MFMA_FLOPS_PER_PAIR / (ms / 1e3) / 1e12
+                row[label]["matrix_tflops"] = mf
+                row[label]["matrix_frac"] = mf / FP32_PEAK_TFLOPS
+            log(f"[{name}] brute force {label}: {ms:.2f} ms, "
+                f"{row[label]['tflops']:.1f} TFLOP/s, mean rel err {rel.mean():.2e}, "
+                f"max {rel.max():.2e}")
+        row["mfma_speedup"] = row["direct"]["kernel_ms"] / row["mfma"]["kernel_ms"]
+        rows.append(row)
+        t.close()
+        del qd, out, ref
+        torch.cuda.empty_cache()
+    return rows
 
 
 def profile(wl, args):
@@ -333,7 +451,7 @@ def profile(wl, args):
         log(f"read+write {args.copy_mib} MiB x3 (calibration)")
         del a, b
     t = C.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"],
-               split_log=2 if args.profile_kernel == "stochastic" else 1)
+               split_log=args.tree_k if args.profile_kernel == "stochastic" else 1)
     q32 = wl["queries"].astype(np.float32)
     nq = len(q32)
     qd = torch.from_numpy(np.ascontiguousarray(q32.T)).to(dev)
@@ -342,10 +460,14 @@ def profile(wl, args):
     for _ in range(args.steps):
         if args.profile_kernel == "stochastic":
             ms = t.eval_device(C.STOCHASTIC, "gravity", qd.data_ptr(), nq, out.data_ptr(),
-                               samples_per_subdomain=args.spl[0], seed=1)
+                               samples_per_subdomain=args.spl[0], seed=1,
+                               flags=0 if args.no_sort else C.EVAL_SORT_QUERIES)
         elif args.profile_kernel == "bh":
             ms = t.eval_device(C.BARNES_HUT, "gravity", qd.data_ptr(), nq, out.data_ptr(),
                                beta=1.0 / args.theta, vote_group=1, flags=C.EVAL_SORT_QUERIES)
+        elif args.profile_kernel == "mfma":
+            ms = t.eval_device(C.BRUTE_FORCE, "gravity", qd.data_ptr(), nq, out.data_ptr(),
+                               flags=C.EVAL_MFMA)
         else:
             ms = t.eval_device(C.BRUTE_FORCE, "gravity", qd.data_ptr(), nq, out.data_ptr())
         log(f"{args.profile_kernel}: {ms:.3f} ms")
@@ -449,24 +571,25 @@ def serving_bench(args):
     }
 
 
-def cpu_baseline(wl, spl, budget_s, gpu_qps):
+def cpu_baseline(wl, spl, k, budget_s, gpu_qps):
     """CPU baseline leg: the oracle (a C++ restatement of the reference's CPU
-    estimator with the same threading model) timed on this machine's host
-    cores on a bounded sample of the headline workload."""
+    estimator with the same threading model and its per-query sample paths)
+    timed on this machine's host cores on a bounded sample of the headline
+    workload."""
     import oracle
 
     threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
-    ot = oracle.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"], split_log=2)
+    ot = oracle.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"], split_log=k)
     build_s = time.perf_counter() - t0
     q32 = wl["queries"].astype(np.float32)
     probe = q32[:2048]
     t0 = time.perf_counter()
-    ot.stochastic("gravity", probe, spl, seed=1, nthreads=threads)
+    ot.stochastic("gravity", probe, spl, seed=1, group=1, nthreads=threads)
     per_q = (time.perf_counter() - t0) / len(probe)
     nsample = int(min(len(q32), max(2048, budget_s / max(per_q, 1e-9))))
     t0 = time.perf_counter()
-    ot.stochastic("gravity", q32[:nsample], spl, seed=1, nthreads=threads)
+    ot.stochastic("gravity", q32[:nsample], spl, seed=1, group=1, nthreads=threads)
     el = time.perf_counter() - t0
     qps = nsample / el
     log(f"cpu baseline: {qps:.0f} q/s on {threads} threads ({nsample} queries, {el:.1f} s); "
@@ -474,7 +597,8 @@ def cpu_baseline(wl, spl, budget_s, gpu_qps):
     return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"stochastic estimator, samples_per_subdomain={spl}, first {nsample} of the "
                       f"{len(q32)} Plummer queries against the {len(wl['pts'])}-source octree "
-                      f"(k=2); tree build excluded ({build_s:.1f} s single-threaded)",
+                      f"(k={k}), one sample path per query; tree build excluded "
+                      f"({build_s:.1f} s single-threaded)",
             "gpu_over_cpu": gpu_qps / qps}
 
 

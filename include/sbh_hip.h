@@ -71,6 +71,13 @@ typedef struct sbh_tree sbh_tree;
                                       cores (V_MFMA_F32_16X16X4_F32) in coordinates
                                       centred per 64-query wavefront, near-field pairs
                                       recomputed directly */
+#define SBH_EVAL_PER_QUERY_PATHS 32 /* stochastic: one sample path per query keyed by the
+                                      query's index, as the reference CPU sampler draws
+                                      them (include/tree_sampler.h:48-104); default: the
+                                      64 queries of a wavefront share their paths, as the
+                                      reference GPU sampler shares them per block
+                                      (include/tree_sampler_gpu.h:41-71).  Per-query
+                                      results do not depend on the query order */
 
 typedef struct sbh_eval_params {
   float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */

@@ -65,7 +65,7 @@ def build_module(name: str, source: str, link_lib: bool) -> str:
 
 def build_all() -> list[str]:
     outs = [build_library()]
-    outs.append(build_module("sbh", "py_sbh.cpp", link_lib=False))
+    outs.append(build_module("sbh", "py_sbh.cpp", link_lib=True))
     outs.append(build_module("sbh_gpu", "py_sbh_gpu.cpp", link_lib=True))
     return outs
 

@@ -284,6 +284,10 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     if (q == 0) return;
     SBH_REQUIRE(queries && results, "NULL queries or results");
     const sbh_eval_params& p = *params;
+    SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
+                             SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
+                             SBH_EVAL_PER_QUERY_PATHS)) == 0,
+                "unknown eval flags");
     if (method == SBH_BARNES_HUT) {
       SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
       SBH_REQUIRE(p.vote_group == 1 || p.vote_group == 32 || p.vote_group == 64,
@@ -355,7 +359,8 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       if (method == SBH_BARNES_HUT)
         sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, exact, s);
       else
-        sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed, dq, nq, ord, dout, dcnt,
+        sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed,
+                            (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout, dcnt,
                             split, exact, s);
       SBH_HIP(hipEventRecord(tree->ev1, s));
     }

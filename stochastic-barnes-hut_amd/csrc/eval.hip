@@ -207,6 +207,31 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
   }
 }
 
+// One sample path per query: thread = query, path key = the query's index in
+// the caller's array.
+template <int D, int K, bool EXACT>
+__global__ __launch_bounds__(kThreads) void k_stochastic_query(
+    TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
+    const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
+    const uint32_t* __restrict__ order, float* __restrict__ out, int32_t* __restrict__ counts,
+    int split_counts) {
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  if (slot >= nq) return;
+  const uint32_t qi = order ? order[slot] : slot;
+  float x, y, z;
+  load_query(qx, qy, qz, qi, D, x, y, z);
+  const LaneResult r = stochastic_estimate_query<D, K, EXACT>(t, spl, key0, key1, qi, x, y, z);
+  out[qi] = r.value;
+  if (counts) {
+    if (split_counts) {
+      counts[qi] = r.nodes;
+      counts[nq + qi] = r.points;
+    } else {
+      counts[qi] = r.points + r.nodes;
+    }
+  }
+}
+
 // Morton keys of the queries inside the tree's root box: 21 bits per axis in
 // 3D, 31 in 2D (63- and 62-bit keys), so that even a root box thousands of
 // times wider than the dense core still orders the core's queries.
@@ -350,12 +375,17 @@ void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, cons
 }
 
 template <int D, int K, bool EXACT>
-void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, const float* qx, const float* qy,
-                  const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
-                  int split, hipStream_t s) {
-  k_stochastic<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
-      view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out, counts,
-      split);
+void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, bool per_query, const float* qx,
+                  const float* qy, const float* qz, uint32_t nq, const uint32_t* order,
+                  float* out, int32_t* counts, int split, hipStream_t s) {
+  if (per_query)
+    k_stochastic_query<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+        view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
+        counts, split);
+  else
+    k_stochastic<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+        view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
+        counts, split);
   SBH_HIP(hipGetLastError());
 }
 
@@ -431,17 +461,19 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
   });
 }
 
-void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, const float* q,
-                    uint32_t nq, const uint32_t* order, float* out, int32_t* counts, int split,
-                    bool exact, hipStream_t s) {
+void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,
+                    const float* q, uint32_t nq, const uint32_t* order, float* out,
+                    int32_t* counts, int split, bool exact, hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
     if (exact)
-      launch_stoch<D, K, true>(t, spl, seed, qx, qy, qz, nq, order, out, counts, split, s);
+      launch_stoch<D, K, true>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
+                               split, s);
     else
-      launch_stoch<D, K, false>(t, spl, seed, qx, qy, qz, nq, order, out, counts, split, s);
+      launch_stoch<D, K, false>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
+                                split, s);
   });
 }
 

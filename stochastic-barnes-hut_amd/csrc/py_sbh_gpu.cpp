@@ -8,6 +8,7 @@
 // Each evaluation returns (float32[q], kernel milliseconds), as the reference.
 // Keyword extensions keep the reference defaults: vote_group (32, the
 // reference's warp vote), seed (0), sort_queries (False), return_counts,
+// per_query_paths (False: the 64 queries of a wavefront share sample paths),
 // exact_terms (False: hardware sqrt / reciprocal / exp2 as the reference's
 // fast-math GPU build; True: correctly rounded, bit-reproducible terms).
 // The GIL is released around device work.
@@ -195,7 +196,7 @@ void bind_tree(py::module_& m, const char* name) {
 
 py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, float beta,
                     int spl, uint64_t seed, int vote_group, bool sort, bool counts,
-                    bool exact = false, bool mfma = false) {
+                    bool exact = false, int extra_flags = 0) {
   std::vector<float> q = t.soa(queries);
   const int64_t nq = queries.shape(0);
   py::array_t<float> out(nq);
@@ -206,7 +207,7 @@ py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, fl
   p.seed = seed;
   p.vote_group = vote_group;
   p.flags = (sort ? SBH_EVAL_SORT_QUERIES : 0) | (exact ? SBH_EVAL_EXACT_TERMS : 0) |
-            (mfma ? SBH_EVAL_MFMA : 0);
+            extra_flags;
   float ms = 0.0f;
   float* op = out.mutable_data();
   int32_t* cp = counts ? cnt.mutable_data() : nullptr;
@@ -224,7 +225,8 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
   std::string s = suffix;
   m.def(("eval_brute_force_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, bool mfma) {
-          return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false, false, mfma);
+          return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false, false,
+                          mfma ? SBH_EVAL_MFMA : 0);
         },
         "tree"_a, "queries"_a, "mfma"_a = false);
   m.def(("eval_barnes_hut_" + s).c_str(),
@@ -237,12 +239,13 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
         "return_counts"_a = false, "exact_terms"_a = false);
   m.def(("multi_level_prefix_control_variate_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, int spl, uint64_t seed, bool sort_queries,
-                 bool return_counts, bool exact_terms) {
+                 bool return_counts, bool exact_terms, bool per_query_paths) {
           return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.f, spl, seed, 1, sort_queries,
-                          return_counts, exact_terms);
+                          return_counts, exact_terms,
+                          per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0);
         },
         "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
-        "return_counts"_a = false, "exact_terms"_a = false);
+        "return_counts"_a = false, "exact_terms"_a = false, "per_query_paths"_a = false);
   m.def(("eval_brute_force_f64_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array queries) {
           std::vector<float> q = t.soa(queries);

@@ -35,9 +35,11 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
             const uint32_t* order, float* out, int32_t* counts, int split, bool exact,
             hipStream_t s);
-void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, const float* q,
-                    uint32_t nq, const uint32_t* order, float* out, int32_t* counts, int split,
-                    bool exact, hipStream_t s);
+// per_query: one sample path per query keyed by its index (otherwise the 64
+// queries of a wavefront share their paths, keyed by the wavefront's index).
+void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,
+                    const float* q, uint32_t nq, const uint32_t* order, float* out,
+                    int32_t* counts, int split, bool exact, hipStream_t s);
 
 // Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)
 // uint32 values.

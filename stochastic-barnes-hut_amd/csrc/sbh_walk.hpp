@@ -142,25 +142,26 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMulti = 1u;  // leaf with more than one point
 
+// Adds node v's terms to acc: its points one by one if v is a leaf (in point
+// order, as the reference sums them), else its moment.
 template <int D, int K, bool EXACT>
-__device__ __forceinline__ float node_term(const TreeView& t, uint32_t v, float x, float y,
-                                           float z, int32_t& cnt, int32_t& cnt_nodes) {
+__device__ __forceinline__ void node_accumulate(const TreeView& t, uint32_t v, float x, float y,
+                                                float z, float& acc, int32_t& cnt,
+                                                int32_t& cnt_nodes) {
   const uint4 l = t.nodeL[v];
   const bool leaf = (l.w & kLeafBit) != 0;
-  float s = 0.0f;
   uint32_t np = 0;
   if (leaf) {
     const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
     for (uint32_t j = p0; j < p1; ++j)
-      s += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+      acc += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
     np = p1 - p0;
   } else {
-    s = term<D, K, EXACT>(t.nodeC[v], t.nodeN[v], t.alpha, t.alpha_k, x, y, z);
+    acc += term<D, K, EXACT>(t.nodeC[v], t.nodeN[v], t.alpha, t.alpha_k, x, y, z);
   }
   // value selects, not a selected reference: both counters stay in registers
   cnt += (int32_t)np;
   cnt_nodes += leaf ? 0 : 1;
-  return s;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -193,13 +194,14 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
   const uint4 rl = nodeL[0];
   float result;
   if (rl.w & kLeafBit) {
-    result = node_term<D, K, EXACT>(t, 0, x, y, z, cnt, cnt_nodes);
+    result = 0.0f;
+    node_accumulate<D, K, EXACT>(t, 0, x, y, z, result, cnt, cnt_nodes);
   } else {
     float bh = 0.0f, mlcv = 0.0f;
     const uint32_t nsub = rl.w, first = rl.y;
     for (uint32_t si = 0; si < nsub; ++si) {
       const uint32_t S = uniform_u32(first + si);
-      bh += node_term<D, K, EXACT>(t, S, x, y, z, cnt, cnt_nodes);
+      node_accumulate<D, K, EXACT>(t, S, x, y, z, bh, cnt, cnt_nodes);
       const uint4 Sl = nodeL[S];
       const uint32_t sstart = n_start[S];
       const uint32_t nS = n_end[S] - sstart;
@@ -334,6 +336,104 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
     result = bh + mlcv;
   }
   return {result, cnt, cnt_nodes};
+}
+
+// ---------------------------------------------------------------------------
+// The same estimator with one sample path per query, as the reference's CPU
+// sampler draws them (include/tree_sampler.h:21-113: every query picks its
+// own point and runs its own roulette).  The path of sample `ss` in subdomain
+// `si` of query `qid` comes from Philox block {ss, si, qid, b}: the stream a
+// 1-query group gets in the wavefront-shared estimator, so a query's result
+// does not depend on which queries share its launch or in what order.  Lanes
+// walk independently, reading node records and children straight from
+// memory (neighbouring queries in Morton order touch the same records).
+// ---------------------------------------------------------------------------
+template <int D, int K, bool EXACT>
+__device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& t, int spl,
+                                                                uint32_t key0, uint32_t key1,
+                                                                uint32_t qid, float x, float y,
+                                                                float z) {
+#pragma clang fp contract(off)
+  const float4* __restrict__ nodeC = t.nodeC;
+  const uint4* __restrict__ nodeL = t.nodeL;
+  const float4* __restrict__ nodeN = t.nodeN;
+  const uint32_t* __restrict__ n_start = t.n_start;
+  const uint32_t* __restrict__ n_end = t.n_end;
+  const uint32_t* __restrict__ n_skip = t.n_skip;
+  const float alpha = t.alpha, alpha_k = t.alpha_k;
+  int32_t cnt = 0, cnt_nodes = 0;
+  const uint4 rl = nodeL[0];
+  if (rl.w & kLeafBit) {
+    float v = 0.0f;
+    node_accumulate<D, K, EXACT>(t, 0, x, y, z, v, cnt, cnt_nodes);
+    return {v, cnt, cnt_nodes};
+  }
+  float bh = 0.0f, mlcv = 0.0f;
+  const uint32_t nsub = rl.w, first = rl.y;
+  for (uint32_t si = 0; si < nsub; ++si) {
+    const uint32_t S = first + si;
+    node_accumulate<D, K, EXACT>(t, S, x, y, z, bh, cnt, cnt_nodes);
+    const uint4 Sl = nodeL[S];
+    if (Sl.w & kLeafBit) continue;  // the subdomain's points were summed exactly
+    const uint32_t sstart = n_start[S];
+    const uint32_t nS = n_end[S] - sstart;
+    const float4 SC = nodeC[S];
+    const float cS = ratio_exact<D>(SC, __uint_as_float(Sl.x), x, y, z);
+    float sub_total = 0.0f;
+    for (int ss = 0; ss < spl; ++ss) {
+      const Philox4 r0 = philox4x32_10((uint32_t)ss, si, qid, 0u, key0, key1);
+      const uint32_t idx = sstart + scale_draw(r0.x, r0.y, nS);
+      Philox4 blk = r0;
+      uint32_t blk_id = 0;
+      float event_prob = 1.0f;
+      float wd_all = 0.0f;
+      float cur_beta = fmaxf(1.0f, cS);
+      uint32_t level = 0;
+      uint32_t cur = S;
+      uint4 curL = Sl;
+      float4 curC = SC;
+      while (!(curL.w & kLeafBit)) {
+        const uint32_t wi = level + 2;
+        if ((wi >> 2) != blk_id) {
+          blk_id = wi >> 2;
+          blk = philox4x32_10((uint32_t)ss, si, qid, blk_id, key0, key1);
+        }
+        const uint32_t wsel = wi & 3;
+        const uint32_t word = wsel == 0 ? blk.x : wsel == 1 ? blk.y : wsel == 2 ? blk.z : blk.w;
+        const float p = rr_uniform(word);
+        ++level;
+        const float ffr = ratio_exact<D>(curC, __uint_as_float(curL.x), x, y, z);
+        const float tp = fminf(cur_beta / ffr, 1.0f);
+        if (p > tp) {
+          event_prob *= 1.0f - tp;
+          break;
+        }
+        event_prob *= tp;
+        const uint32_t prev = cur;
+        const uint32_t f = curL.y, nch = curL.w;
+        float cur_result = 0.0f;
+        uint32_t pick = 0;
+        for (uint32_t j = 0; j < nch; ++j) {
+          const uint32_t c = f + j;
+          node_accumulate<D, K, EXACT>(t, c, x, y, z, cur_result, cnt, cnt_nodes);
+          pick = n_start[c] <= idx ? j : pick;  // children are in point order
+        }
+        cur = n_skip[f + pick];
+        curL = nodeL[cur];
+        curC = nodeC[cur];
+        cur_beta = fmaxf(cur_beta, ffr);
+        const float prev_result =
+            term<D, K, EXACT>(nodeC[prev], nodeN[prev], alpha, alpha_k, x, y, z);
+        const float diff = cur_result - prev_result;
+        const float wd =
+            (float)nS / (event_prob * (float)(n_end[prev] - n_start[prev])) * diff;
+        wd_all += wd;
+      }
+      sub_total += wd_all;
+    }
+    mlcv += sub_total / (float)spl;
+  }
+  return {bh + mlcv, cnt, cnt_nodes};
 }
 
 }  // namespace sbh

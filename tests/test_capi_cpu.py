@@ -82,6 +82,8 @@ def test_plugin_modules_keep_reference_names():
     for kern in ("gravity", "wn", "smoothdist"):
         for fn in ("eval_brute_force_", "eval_barnes_hut_", "multi_level_prefix_control_variate_"):
             assert hasattr(sbh_gpu, fn + kern)
+        for fn in ("eval_barnes_hut_", "multi_level_prefix_control_variate_"):
+            assert hasattr(sbh, fn + kern)
     assert sbh.fp == 32 and sbh_gpu.fp == 32
 
 
@@ -96,8 +98,28 @@ def test_host_tree_holds_float32_inputs():
     assert t.alpha == 2.5
     with pytest.raises(ValueError):
         sbh.Octree1(pts[:, :2], np.zeros((50, 2)), np.ones(50), [0, 0], [1, 1])
+    # queries of the wrong dimension are rejected before any device work
+    with pytest.raises(ValueError):
+        sbh.eval_barnes_hut_gravity(t, pts[:, :2], 2.0)
+    # a tree of another module is not a host tree
     with pytest.raises(TypeError):
+        sbh.eval_barnes_hut_gravity(object(), pts, 2.0)
+
+
+def test_cpu_plugin_api_needs_a_gpu():
+    """The sbh evaluation functions run on the GPU only: without one they
+    raise instead of computing on the host."""
+    import sbh
+    import sbh_gpu
+
+    if sbh_gpu.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    pts = np.random.default_rng(2).random((40, 3))
+    t = sbh.Octree1(pts, np.zeros_like(pts), np.ones(40), pts.min(0), pts.max(0))
+    with pytest.raises(RuntimeError):
         sbh.eval_barnes_hut_gravity(t, pts, 2.0)
+    with pytest.raises(RuntimeError):
+        sbh.multi_level_prefix_control_variate_wn(t, pts, 1)
 
 
 def test_gpu_tree_validation_errors_on_host():

@@ -164,17 +164,23 @@ def test_brute_force(sbh_gpu, dim, kernel):
 @pytest.mark.parametrize("workload", ["cube", "plummer"])
 def test_brute_force_mfma(sbh_gpu, kernel, workload):
     """Matrix-core squared distances with the near field recomputed: within
-    1e-5 of the oracle and of float64, for every query (ragged count)."""
+    1e-5 of the oracle and of float64, for every query (ragged count).  The
+    untruncated Plummer sphere (coordinates up to ~1e3) covers large offsets
+    for gravity; for smooth distance it is truncated at r <= 10, since
+    exp(-10 d) underflows to zero even in float64 for its outer queries."""
     if workload == "cube":
         pts, nrm, m, q = W.uniform_cube(9000, 3001, seed=27)
     else:
-        pts, nrm, m, q = W.plummer(9000, 3001)
+        pts, nrm, m, q = W.plummer(9000, 3001, rmax=None if kernel == "gravity" else 10.0)
     lo, hi = W.bounds(pts)
     alpha = 10.0 if kernel == "smoothdist" else 1.0
     g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1, alpha=alpha)
     got, ms = getattr(sbh_gpu, "eval_brute_force_" + kernel)(g, q, mfma=True)
     ref = o.brute_force(kernel, q)
     ref64 = o.brute_force(kernel, q, fp64=True)
+    assert np.isfinite(got).all()
+    if kernel == "smoothdist":
+        assert (ref64 > 0).all()
     err = rel_err(got, ref64, kernel)
     print(f"[mfma] {kernel} {workload}: max {err.max():.2e} mean {err.mean():.2e}")
     assert rel_err(got, ref, kernel).max() <= TOL
@@ -244,6 +250,72 @@ def test_stochastic_same_seed(sbh_gpu, kernel, dim, k):
         ferr = rel_err(fast, ref, kernel)
         print(f"[fast] {kernel} d{dim} k{k} spl{spl}: mean {ferr.mean():.2e} max {ferr.max():.2e}")
         assert ferr.mean() <= TOL and np.percentile(ferr, 99) <= 10 * TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("dim,k,leaf", [(3, 2, 1), (3, 1, 4), (2, 2, 1), (2, 3, 3)])
+def test_stochastic_per_query_paths(sbh_gpu, kernel, dim, k, leaf):
+    """One sample path per query (the reference CPU sampler's draws): values
+    and term counts equal the oracle's 1-query groups, and do not depend on
+    the evaluation order.  Leaf sizes above 1 cover multi-point leaves."""
+    pts, nrm, m, q, lo, hi = _cube(5000, 257, dim, 91 + k, kernel)
+    alpha = 10.0 if kernel == "smoothdist" else 1.0
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, k, leaf_size=leaf, alpha=alpha)
+    fn = getattr(sbh_gpu, "multi_level_prefix_control_variate_" + kernel)
+    for spl, seed in ((1, 3), (5, 2**33 + 1)):
+        ref, rcnt = o.stochastic(kernel, q, spl, seed=seed, group=1, counts=True)
+        got, _, cnt = fn(g, q, spl, seed=seed, return_counts=True, exact_terms=True,
+                         per_query_paths=True)
+        np.testing.assert_array_equal(cnt, rcnt)
+        assert rel_err(got, ref, kernel).max() <= TOL
+        if kernel != "smoothdist":
+            np.testing.assert_array_equal(got, ref)
+        srt, _, scnt = fn(g, q, spl, seed=seed, return_counts=True, exact_terms=True,
+                          per_query_paths=True, sort_queries=True)
+        np.testing.assert_array_equal(srt, got)
+        np.testing.assert_array_equal(scnt, cnt)
+        # a query's result does not depend on the other queries of the call
+        one, _ = fn(g, q[37:38], spl, seed=seed, exact_terms=True, per_query_paths=True)
+        assert rel_err(one, o.stochastic(kernel, q[37:38], spl, seed=seed, group=1),
+                       kernel).max() <= TOL
+        fast, _ = fn(g, q, spl, seed=seed, per_query_paths=True)
+        ferr = rel_err(fast, ref, kernel)
+        assert ferr.mean() <= TOL and np.percentile(ferr, 99) <= 10 * TOL
+    # the shared-path estimator on multi-point leaves
+    ref = o.stochastic(kernel, q, 2, seed=9, group=64)
+    got, _ = fn(g, q, 2, seed=9, exact_terms=True)
+    assert rel_err(got, ref, kernel).max() <= TOL
+
+
+@pytest.mark.parametrize("cls,dim,k", [("Octree1", 3, 1), ("Octree2", 3, 2), ("Octree3", 3, 3),
+                                       ("Quadtree1", 2, 1), ("Quadtree2", 2, 2),
+                                       ("Quadtree3", 2, 3)])
+def test_cpu_plugin_api(sbh_gpu, cls, dim, k):
+    """The sbh module's evaluation functions (src/sbh.cpp:216-270): the
+    reference CPU plugin's per-query Barnes-Hut and per-query-path stochastic
+    estimator, bit-identical to the oracle with its 1-query groups."""
+    import sbh
+
+    for kernel in KERNELS:
+        pts, nrm, m, q, lo, hi = _cube(3000, 200, dim, 97 + k, kernel)
+        alpha = 10.0 if kernel == "smoothdist" else 1.0
+        t = getattr(sbh, cls)(pts, nrm, m, lo, hi, alpha)
+        o = oracle.Tree(pts, nrm, m, lo, hi, alpha=alpha, split_log=k)
+        bh = getattr(sbh, "eval_barnes_hut_" + kernel)(t, q, 2.0)
+        assert isinstance(bh, np.ndarray) and bh.dtype == np.float32 and bh.shape == (200,)
+        ref_bh = o.barnes_hut(kernel, q, 2.0, vote_group=1)
+        vals, cnt = getattr(sbh, "multi_level_prefix_control_variate_" + kernel)(t, q, 2)
+        assert vals.dtype == np.float32 and cnt.dtype == np.int32
+        ref_s, ref_c = o.stochastic(kernel, q, 2, seed=0, group=1, counts=True)
+        np.testing.assert_array_equal(cnt, ref_c)
+        if kernel == "smoothdist":
+            assert rel_err(bh, ref_bh, kernel).max() <= TOL
+            assert rel_err(vals, ref_s, kernel).max() <= TOL
+        else:
+            np.testing.assert_array_equal(bh, ref_bh)
+            np.testing.assert_array_equal(vals, ref_s)
+        exact = getattr(sbh, "eval_barnes_hut_" + kernel)(t, q, np.inf)
+        assert rel_err(exact, o.brute_force(kernel, q, fp64=True), kernel).max() <= TOL
 
 
 def test_stochastic_unbiased_and_variance(sbh_gpu):
