@@ -222,6 +222,17 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
         wall, kms = run(th, C.STOCHASTIC, headline_steps, warmup, **hkw)
         res["headline"] = {"wall_ms": wall, "kernel_ms": kms, "mean_rel_err": err_of()}
     if want_counts:
+        # the CPU plugin's semantics at the same budget: one path per query
+        _, kms = run(th, C.STOCHASTIC, args.sweep_steps, 1,
+                     samples_per_subdomain=hkw["samples_per_subdomain"], seed=1,
+                     flags=C.EVAL_SORT_QUERIES | C.EVAL_PER_QUERY_PATHS)
+        ms = float(np.mean(kms))
+        res["stochastic_per_query_paths"] = {
+            "tree_k": head["tree_k"], "samples_per_subdomain": hkw["samples_per_subdomain"],
+            "sorted": True, "kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
+            "mean_rel_err": err_of()}
+        log(f"  SBH per-query paths: {ms:.2f} ms err "
+            f"{res['stochastic_per_query_paths']['mean_rel_err']:.2e}")
         th.eval_device(C.STOCHASTIC, kern, qd.data_ptr(), nq, out.data_ptr(),
                        counts_ptr=cnt.data_ptr(), samples_per_subdomain=hkw["samples_per_subdomain"],
                        seed=1, flags=hflags | C.EVAL_SPLIT_COUNTS)
@@ -245,7 +256,7 @@ def main():
     ap.add_argument("--err-queries", type=int, default=16384)
     ap.add_argument("--sweep-steps", type=int, default=3)
     ap.add_argument("--brute-steps", type=int, default=2)
-    ap.add_argument("--spl", type=int, nargs="+", default=[1, 4, 16, 64, 256, 512],
+    ap.add_argument("--spl", type=int, nargs="+", default=[1, 4, 16, 64, 256, 512, 768, 1024],
                     help="samples per subdomain swept for the stochastic estimator")
     ap.add_argument("--rmax", type=float, default=None,
                     help="truncate the headline Plummer sphere at this radius")
@@ -373,6 +384,7 @@ def main():
         "speedup_vs_brute_force": value / head["brute_force"]["queries_per_s"],
         "brute_force": head["brute_force"],
         "brute_force_mfma": head.get("brute_force_mfma"),
+        "stochastic_per_query_paths": head.get("stochastic_per_query_paths"),
         "barnes_hut_at_target": head["barnes_hut_at_target"],
         "barnes_hut_over_brute_force": head["barnes_hut_over_brute_force"],
         "methods": {"barnes_hut": head["barnes_hut"], "stochastic": head["stochastic"]},

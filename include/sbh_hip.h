@@ -163,6 +163,26 @@ int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* o
 int sbh_tree_stream(sbh_tree* tree, void** stream);
 
 /*
+ * Tree images.  No counterpart in the reference (its trees exist only inside
+ * the Python objects that built them, include/py_tree_gpu.h:23-81): a built
+ * tree is written to a byte image (format version 1: header, every node and
+ * point array, FNV-1a checksum) and loaded back resident on the current GPU
+ * without rebuilding, e.g. for restarts of a serving process.  A loaded tree
+ * evaluates exactly as the tree that was saved.  Loading checks the checksum,
+ * the sizes and every index the walks follow before any device memory is
+ * written, and fails with SBH_ERR_INVALID on a damaged image.
+ *   sbh_tree_serialize: with buffer == NULL (or capacity too small) only
+ *   *size is set (and SBH_OK returned for NULL, SBH_ERR_INVALID otherwise).
+ */
+int sbh_tree_serialize(const sbh_tree* tree, void* buffer, int64_t capacity, int64_t* size);
+/* Shape of a tree (e.g. one just loaded): dim, split_log, leaf_size, alpha. */
+int sbh_tree_layout(const sbh_tree* tree, int32_t* dim, int32_t* split_log, int32_t* leaf_size,
+                    float* alpha);
+int sbh_tree_deserialize(const void* buffer, int64_t size, sbh_tree** out);
+int sbh_tree_save(const sbh_tree* tree, const char* path);
+int sbh_tree_load(const char* path, sbh_tree** out);
+
+/*
  * Serving.  No counterpart in the reference (it re-copies queries per call
  * and runs synchronously, src/sbh_gpu.cu:212-248): resident trees, requests
  * queued and admitted in arrival order while their working sets (queries +

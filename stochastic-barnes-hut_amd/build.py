@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["tree_build.hip", "eval.hip", "brute_mfma.hip", "serve.hip", "capi.hip"]
+HIP_SOURCES = ["tree_build.hip", "eval.hip", "brute_mfma.hip", "serve.hip", "tree_io.hip", "capi.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared",
     # no packed-fp32 instructions in generated code

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -171,6 +172,85 @@ int sbh_tree_stream(sbh_tree* tree, void** stream) {
   return guarded([&] {
     SBH_REQUIRE(tree && stream, "NULL argument");
     *stream = (void*)tree->t.stream;
+  });
+}
+
+int sbh_tree_serialize(const sbh_tree* tree, void* buffer, int64_t capacity, int64_t* size) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && size, "NULL argument");
+    std::vector<uint8_t> img;
+    sbh::serialize_tree(tree->t, img);
+    *size = (int64_t)img.size();
+    if (!buffer) return;
+    SBH_REQUIRE(capacity >= (int64_t)img.size(), "buffer smaller than the tree image");
+    std::memcpy(buffer, img.data(), img.size());
+  });
+}
+
+namespace {
+sbh_tree* tree_from_image(const uint8_t* data, size_t size) {
+  sbh_tree* h = new sbh_tree();
+  try {
+    sbh::deserialize_tree(h->t, data, size);  // validates, then creates the stream
+    SBH_HIP(hipEventCreate(&h->ev0));
+    SBH_HIP(hipEventCreate(&h->ev1));
+  } catch (...) {
+    sbh_tree_destroy(h);
+    throw;
+  }
+  return h;
+}
+}  // namespace
+
+int sbh_tree_layout(const sbh_tree* tree, int32_t* dim, int32_t* split_log, int32_t* leaf_size,
+                    float* alpha) {
+  return guarded([&] {
+    SBH_REQUIRE(tree, "tree is NULL");
+    if (dim) *dim = tree->t.dim;
+    if (split_log) *split_log = tree->t.split_log;
+    if (leaf_size) *leaf_size = tree->t.leaf_size;
+    if (alpha) *alpha = tree->t.alpha;
+  });
+}
+
+int sbh_tree_deserialize(const void* buffer, int64_t size, sbh_tree** out) {
+  return guarded([&] {
+    SBH_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    SBH_REQUIRE(buffer && size > 0, "empty tree image");
+    *out = tree_from_image(static_cast<const uint8_t*>(buffer), (size_t)size);
+  });
+}
+
+int sbh_tree_save(const sbh_tree* tree, const char* path) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && path, "NULL argument");
+    std::vector<uint8_t> img;
+    sbh::serialize_tree(tree->t, img);
+    FILE* f = std::fopen(path, "wb");
+    SBH_REQUIRE(f, std::string("cannot open ") + path + " for writing");
+    const size_t w = std::fwrite(img.data(), 1, img.size(), f);
+    const bool ok = (w == img.size()) && std::fflush(f) == 0;
+    std::fclose(f);
+    SBH_REQUIRE(ok, std::string("short write to ") + path);
+  });
+}
+
+int sbh_tree_load(const char* path, sbh_tree** out) {
+  return guarded([&] {
+    SBH_REQUIRE(out && path, "NULL argument");
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    SBH_REQUIRE(f, std::string("cannot open ") + path);
+    std::vector<uint8_t> img;
+    uint8_t buf[1 << 16];
+    size_t r;
+    while ((r = std::fread(buf, 1, sizeof(buf), f)) > 0) img.insert(img.end(), buf, buf + r);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    SBH_REQUIRE(!err, std::string("read error on ") + path);
+    SBH_REQUIRE(!img.empty(), std::string(path) + " is empty");
+    *out = tree_from_image(img.data(), img.size());
   });
 }
 

@@ -76,8 +76,35 @@ struct GPUTreeBase {
     }
     check(status, "tree build");
   }
+  // Adopts a tree handle (a loaded image).
+  GPUTreeBase(sbh_tree* handle, int d, int k) : h(handle), dim(d), split_log(k) {}
   ~GPUTreeBase() {
     if (h) sbh_tree_destroy(h);
+  }
+
+  void save(const std::string& path) const {
+    int status;
+    {
+      py::gil_scoped_release nogil;
+      status = sbh_tree_save(h, path.c_str());
+    }
+    check(status, "tree save");
+  }
+
+  py::bytes to_bytes() const {
+    std::string img;
+    int status;
+    {
+      py::gil_scoped_release nogil;
+      int64_t size = 0;
+      status = sbh_tree_serialize(h, nullptr, 0, &size);
+      if (status == SBH_OK) {
+        img.resize((size_t)size);
+        status = sbh_tree_serialize(h, img.data(), size, &size);
+      }
+    }
+    check(status, "tree serialize");
+    return py::bytes(img);
   }
   GPUTreeBase(const GPUTreeBase&) = delete;
   GPUTreeBase& operator=(const GPUTreeBase&) = delete;
@@ -176,6 +203,27 @@ std::unique_ptr<GPUTree<D, K>> from_host(const py::object& cpu, bool moments64) 
       D, K, attr_array(cpu, "points"), attr_array(cpu, "normals"), attr_array(cpu, "masses"),
       attr_array(cpu, "min_corner"), attr_array(cpu, "max_corner"),
       cpu.attr("alpha").cast<float>(), leaf, moments64, 100);
+}
+
+// Wraps a loaded handle in the class of its shape (OctreeGPUk / QuadtreeGPUk).
+py::object adopt(sbh_tree* h) {
+  int32_t d = 0, k = 0;
+  const int status = sbh_tree_layout(h, &d, &k, nullptr, nullptr);
+  if (status != SBH_OK) {
+    sbh_tree_destroy(h);
+    check(status, "tree layout");
+  }
+  switch (d * 10 + k) {
+    case 21: return py::cast(std::make_unique<GPUTree<2, 1>>(h, 2, 1));
+    case 22: return py::cast(std::make_unique<GPUTree<2, 2>>(h, 2, 2));
+    case 23: return py::cast(std::make_unique<GPUTree<2, 3>>(h, 2, 3));
+    case 31: return py::cast(std::make_unique<GPUTree<3, 1>>(h, 3, 1));
+    case 32: return py::cast(std::make_unique<GPUTree<3, 2>>(h, 3, 2));
+    case 33: return py::cast(std::make_unique<GPUTree<3, 3>>(h, 3, 3));
+    default:
+      sbh_tree_destroy(h);
+      throw std::runtime_error("loaded tree has an unknown shape");
+  }
 }
 
 template <int D, int K>
@@ -397,7 +445,10 @@ PYBIND11_MODULE(sbh_gpu, m) {
       .def("nodes", &GPUTreeBase::nodes)
       .def_readonly("dim", &GPUTreeBase::dim)
       .def_readonly("split_log", &GPUTreeBase::split_log)
-      .def_readonly("build_ms", &GPUTreeBase::build_ms);
+      .def_readonly("build_ms", &GPUTreeBase::build_ms)
+      .def("save", &GPUTreeBase::save, "path"_a,
+           "write the tree's image (format of sbh_tree_save) to a file")
+      .def("to_bytes", &GPUTreeBase::to_bytes);
   bind_tree<2, 1>(m, "QuadtreeGPU1");
   bind_tree<2, 2>(m, "QuadtreeGPU2");
   bind_tree<2, 3>(m, "QuadtreeGPU3");
@@ -425,6 +476,33 @@ PYBIND11_MODULE(sbh_gpu, m) {
   m.attr("REQ_DONE") = SBH_REQ_DONE;
   m.attr("REQ_FAILED") = SBH_REQ_FAILED;
   m.attr("REQ_CANCELLED") = SBH_REQ_CANCELLED;
+  m.def(
+      "load_tree",
+      [](const std::string& path) {
+        sbh_tree* h = nullptr;
+        int status;
+        {
+          py::gil_scoped_release nogil;
+          status = sbh_tree_load(path.c_str(), &h);
+        }
+        check(status, "tree load");
+        return adopt(h);
+      },
+      "path"_a, "a tree saved with tree.save(path), resident on the current GPU");
+  m.def(
+      "tree_from_bytes",
+      [](const py::bytes& b) {
+        std::string img = b;
+        sbh_tree* h = nullptr;
+        int status;
+        {
+          py::gil_scoped_release nogil;
+          status = sbh_tree_deserialize(img.data(), (int64_t)img.size(), &h);
+        }
+        check(status, "tree load");
+        return adopt(h);
+      },
+      "image"_a);
   m.def("device_count", [] {
     int32_t n = 0;
     check(sbh_device_count(&n), "device count");

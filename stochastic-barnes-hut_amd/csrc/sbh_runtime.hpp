@@ -45,6 +45,7 @@ constexpr int kErrInternal = 6;
 template <typename T>
 class DevBuf {
  public:
+  using value_type = T;
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   ~DevBuf() { release(); }
@@ -150,5 +151,11 @@ struct DeviceTree {
 // Host pointers; blocks until the build is complete.
 void build_device_tree(DeviceTree& t, const float* points, const float* normals,
                        const float* masses, float* build_ms);
+
+// Byte image of a built tree (tree_io.hip), and the tree back from one: the
+// image is checked (checksum, sizes, and every index the walks follow) before
+// any device memory is written; the loader creates t.stream if it is unset.
+void serialize_tree(const DeviceTree& t, std::vector<uint8_t>& out);
+void deserialize_tree(DeviceTree& t, const uint8_t* data, size_t size);
 
 }  // namespace sbh

@@ -62,6 +62,12 @@ def lib():
         L.sbh_eval_f64.argtypes = [vp, i32, vp, i64, vp, i32, ctypes.POINTER(f32)]
         L.sbh_query_order.argtypes = [vp, vp, i64, vp, i32]
         L.sbh_tree_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.sbh_tree_serialize.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
+        L.sbh_tree_deserialize.argtypes = [vp, i64, ctypes.POINTER(vp)]
+        L.sbh_tree_save.argtypes = [vp, ctypes.c_char_p]
+        L.sbh_tree_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.sbh_tree_layout.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                      ctypes.POINTER(i32), ctypes.POINTER(f32)]
         _lib = L
     return _lib
 
@@ -97,6 +103,39 @@ class Tree:
                                 ctypes.byref(ms)))
         self.h = h
         self.build_ms = ms.value
+
+    @classmethod
+    def _adopt(cls, h):
+        self = cls.__new__(cls)
+        self.h = h
+        d, k, leaf = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().sbh_tree_layout(h, ctypes.byref(d), ctypes.byref(k), ctypes.byref(leaf), None))
+        self.dim, self.split_log, self.build_ms = d.value, k.value, 0.0
+        self.n = self.info()["num_points"]
+        return self
+
+    @classmethod
+    def load(cls, path) -> "Tree":
+        """A tree saved with save(path), resident on the current GPU."""
+        h = ctypes.c_void_p()
+        check(lib().sbh_tree_load(os.fsencode(path), ctypes.byref(h)))
+        return cls._adopt(h)
+
+    @classmethod
+    def from_bytes(cls, image: bytes) -> "Tree":
+        h = ctypes.c_void_p()
+        check(lib().sbh_tree_deserialize(image, len(image), ctypes.byref(h)))
+        return cls._adopt(h)
+
+    def save(self, path):
+        check(lib().sbh_tree_save(self.h, os.fsencode(path)))
+
+    def to_bytes(self) -> bytes:
+        size = ctypes.c_int64()
+        check(lib().sbh_tree_serialize(self.h, None, 0, ctypes.byref(size)))
+        buf = ctypes.create_string_buffer(size.value)
+        check(lib().sbh_tree_serialize(self.h, buf, size.value, ctypes.byref(size)))
+        return buf.raw[:size.value]
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -130,3 +130,20 @@ def test_gpu_tree_validation_errors_on_host():
         sbh_gpu.OctreeGPU1(pts, np.zeros_like(pts), -np.ones(20), [0, 0, 0], [1, 1, 1])
     with pytest.raises(ValueError):
         sbh_gpu.OctreeGPU1(pts[:, :2], np.zeros((20, 2)), np.ones(20), [0, 0, 0], [1, 1, 1])
+
+
+def test_tree_image_loader_refuses_damage_before_the_device():
+    """Damaged images fail with SBH_ERR_INVALID whether or not a GPU exists:
+    the loader checks them on the host first."""
+    import ctypes
+
+    import sbh_capi as C
+
+    L = C.lib()
+    h = ctypes.c_void_p()
+    assert L.sbh_tree_deserialize(b"garbage!" * 4, 32, ctypes.byref(h)) == C.ERR_INVALID
+    assert "magic" in L.sbh_last_error().decode()
+    assert L.sbh_tree_deserialize(b"SBHTREE\0" + b"\0" * 8, 16, ctypes.byref(h)) == C.ERR_INVALID
+    assert L.sbh_tree_deserialize(None, 0, ctypes.byref(h)) == C.ERR_INVALID
+    assert L.sbh_tree_load(b"/nonexistent/tree.sbht", ctypes.byref(h)) == C.ERR_INVALID
+    assert not h.value
