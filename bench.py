@@ -40,6 +40,9 @@ HBM_PEAK_GBS = 8000.0
 NODE_RECORD_BYTES, POINT_RECORD_BYTES = 32, 16
 # V_MFMA_F32_16X16X4_F32: 16*16*4 multiply-adds; one per 16 x 16 query-source block
 MFMA_FLOPS_PER_PAIR = 8
+# V_MFMA_F32_16X16X16_BF16: 16*16*16 multiply-adds per block; dense bf16 peak
+MFMA_BF16_FLOPS_PER_PAIR = 32
+BF16_PEAK_TFLOPS = 2500.0
 # algorithmic flops per pair (add/sub/mul = 1, sqrt/rcp/exp = 1, divide = 2)
 FLOPS_PER_PAIR = {(3, "gravity"): 13, (3, "wn"): 22, (3, "smoothdist"): 12,
                   (2, "gravity"): 10, (2, "wn"): 14, (2, "smoothdist"): 9}
@@ -421,7 +424,8 @@ def precision_variant(args):
         F = FLOPS_PER_PAIR[(3, kern)]
         pairs = float(len(wl["pts"])) * nq
         row = {"workload": wl["desc"], "fp64_ms": ref_ms}
-        for label, flags in (("direct", 0), ("mfma", C.EVAL_MFMA)):
+        for label, flags in (("direct", 0), ("mfma", C.EVAL_MFMA),
+                             ("mfma_bf16", C.EVAL_MFMA_BF16)):
             t.eval_device(C.BRUTE_FORCE, kern, qd.data_ptr(), nq, out.data_ptr(), flags=flags)
             torch.cuda.synchronize()
             kms = [t.eval_device(C.BRUTE_FORCE, kern, qd.data_ptr(), nq, out.data_ptr(),
@@ -436,10 +440,15 @@ def precision_variant(args):
                 mf = pairs * MFMA_FLOPS_PER_PAIR / (ms / 1e3) / 1e12
                 row[label]["matrix_tflops"] = mf
                 row[label]["matrix_frac"] = mf / FP32_PEAK_TFLOPS
+            elif label == "mfma_bf16":
+                mf = pairs * MFMA_BF16_FLOPS_PER_PAIR / (ms / 1e3) / 1e12
+                row[label]["matrix_tflops"] = mf
+                row[label]["matrix_frac"] = mf / BF16_PEAK_TFLOPS
             log(f"[{name}] brute force {label}: {ms:.2f} ms, "
                 f"{row[label]['tflops']:.1f} TFLOP/s, mean rel err {rel.mean():.2e}, "
                 f"max {rel.max():.2e}")
         row["mfma_speedup"] = row["direct"]["kernel_ms"] / row["mfma"]["kernel_ms"]
+        row["mfma_bf16_speedup"] = row["direct"]["kernel_ms"] / row["mfma_bf16"]["kernel_ms"]
         rows.append(row)
         t.close()
         del qd, out, ref

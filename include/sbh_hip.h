@@ -71,6 +71,12 @@ typedef struct sbh_tree sbh_tree;
                                       cores (V_MFMA_F32_16X16X4_F32) in coordinates
                                       centred per 64-query wavefront, near-field pairs
                                       recomputed directly */
+#define SBH_EVAL_MFMA_BF16 64      /* brute force, 3D gravity / smooth distance: the same
+                                      expansion with bf16 operands, every coordinate
+                                      split into bf16 hi + lo parts and the products
+                                      h.h + h.l + l.h + l.l accumulated in fp32
+                                      (V_MFMA_F32_16X16X16_BF16); pairs closer than 4x
+                                      the wavefront radius recomputed directly */
 #define SBH_EVAL_PER_QUERY_PATHS 32 /* stochastic: one sample path per query keyed by the
                                       query's index, as the reference CPU sampler draws
                                       them (include/tree_sampler.h:48-104); default: the

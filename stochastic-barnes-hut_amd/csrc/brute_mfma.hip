@@ -2,17 +2,23 @@
 //
 // For a wavefront of 64 spatially ordered queries with centre c (and
 // radius r = max |q - c|), and a tile of 16 sources:
-//     d2[i][j] = |q'_i|^2 + sum_k A[i][k] B[k][j]
-//     A[i] = (q'_x, q'_y, q'_z, 1)          q' = q - c
-//     B[j] = (-2 p'_x, -2 p'_y, -2 p'_z, |p'|^2)   p' = p - c
-// one V_MFMA_F32_16X16X4_F32 per 16x16 block, |q'|^2 entering through the
-// accumulator input.  Centring on the wavefront keeps the expansion's
-// cancellation small: the fp32 error of d2 is a few ulp of (|q'| + |p'|)^2,
-// within a few ulp of d2 itself once d >= 2r.  Pairs closer than that (the
-// near field, d2 < 4.5 r^2) are recomputed directly in fp32.  The remaining
-// per-pair work (sqrt, add, reciprocal or exp2, fma) runs on the VALU.
-// Partial sums per lane are kept for 4 x 4 query rows and reduced over the 16
-// lanes that share them at the end.
+//     d2[i][j] = |q'_i|^2 + sum_k A[i][k] B[k][j]           q' = q - c, p' = p - c
+// with |q'|^2 entering through the accumulator input, in one of two operand
+// formats:
+//   fp32   A[i] = (q'_x, q'_y, q'_z, 1), B[j] = (-2p'_x, -2p'_y, -2p'_z, |p'|^2):
+//          one V_MFMA_F32_16X16X4_F32 per 16 x 16 block;
+//   bf16   every coordinate split into bf16 hi + lo parts (x = h + l up to
+//          2^-18 |x|) and |p'|^2 into three bf16 parts; the 16 K slots carry
+//          the products h.h, h.l, l.h, l.l of each axis and the three parts of
+//          |p'|^2, so the fp32 accumulation returns -2 q'.p' + |p'|^2 to
+//          ~2^-17 |q'||p'|: one V_MFMA_F32_16X16X16_BF16 per 16 x 16 block.
+// Centring on the wavefront keeps the expansion's cancellation small: the
+// error of d2 stays a few ulp of (|q'| + |p'|)^2 (fp32) or ~1e-5 |q'||p'|
+// (bf16), small against d2 once d >= 2r (fp32) or d >= 4r (bf16).  Pairs
+// closer than that (the near field) are recomputed directly in fp32.  The
+// remaining per-pair work (sqrt, add, reciprocal or exp2, fma) runs on the
+// VALU.  Partial sums per lane are kept for 4 x 4 query rows and reduced over
+// the 16 lanes that share them at the end.
 //
 // Results match the direct formulation to ~1e-6 relative (checked against
 // the oracle and float64 in the GPU tests).
@@ -28,6 +34,40 @@ namespace sbh {
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
+
+// bf16 bits of four values (each already a bf16 value, so the conversion is exact)
+__device__ __forceinline__ s16x4 bf16x4(float a, float b, float c, float d) {
+  s16x4 r;
+  r[0] = __builtin_bit_cast(short, (__bf16)a);
+  r[1] = __builtin_bit_cast(short, (__bf16)b);
+  r[2] = __builtin_bit_cast(short, (__bf16)c);
+  r[3] = __builtin_bit_cast(short, (__bf16)d);
+  return r;
+}
+
+// Item t of K-slot group g (lane / 16) of the bf16 operands; slot 4g + t:
+//   g0: x.hh x.hl x.lh y.hh   g1: y.hl y.lh z.hh z.hl
+//   g2: z.lh s.h  s.m  s.l    g3: x.ll y.ll z.ll 0
+// A slots hold the query factor, B slots the source factor (-2 p' parts, or
+// the parts of |p'|^2 against A = 1).
+__device__ __forceinline__ s16x4 a_slots(uint32_t g, float xh, float xl, float yh, float yl,
+                                         float zh, float zl) {
+  if (g == 0) return bf16x4(xh, xh, xl, yh);
+  if (g == 1) return bf16x4(yh, yl, zh, zh);
+  if (g == 2) return bf16x4(zl, 1.0f, 1.0f, 1.0f);
+  return bf16x4(xl, yl, zl, 0.0f);
+}
+__device__ __forceinline__ s16x4 b_slots(uint32_t g, float xh, float xl, float yh, float yl,
+                                         float zh, float zl, float sh, float sm, float sl) {
+  const float v0 = g == 0 ? xh : g == 1 ? yl : g == 2 ? zh : xl;
+  const float v1 = g == 0 ? xl : g == 1 ? yh : g == 2 ? sh : yl;
+  const float v2 = g == 0 ? xh : g == 1 ? zh : g == 2 ? sm : zl;
+  const float v3 = g == 0 ? yh : g == 1 ? zl : g == 2 ? sl : 0.0f;
+  return bf16x4(v0, v1, v2, v3);
+}
 
 constexpr int kThreads = 256;   // 4 wavefronts
 constexpr int kStage = 256;     // sources staged in LDS per step
@@ -52,7 +92,7 @@ __device__ __forceinline__ float far_term(float d2, float m, float alpha_k) {
     return __builtin_amdgcn_exp2f(-alpha_k * dist);
 }
 
-template <int K>
+template <int K, bool BF16>
 __global__ __launch_bounds__(kThreads) void k_brute_mfma(
     const float4* __restrict__ pts, uint32_t n, float alpha_k, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
@@ -74,7 +114,7 @@ __global__ __launch_bounds__(kThreads) void k_brute_mfma(
   const float cz = 0.5f * (wave_min(z) + wave_max(z));
   const float ux = x - cx, uy = y - cy, uz = z - cz;
   const float r2 = wave_max(ux * ux + uy * uy + uz * uz);
-  const float near2 = 4.5f * r2;
+  const float near2 = (BF16 ? 16.0f : 4.5f) * r2;
   wq[threadIdx.x] = make_float4(x, y, z, ux * ux + uy * uy + uz * uz);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -83,12 +123,19 @@ __global__ __launch_bounds__(kThreads) void k_brute_mfma(
   // A operands of the four 16-query blocks: lane holds A[i = lane % 16][k = lane / 16]
   const uint32_t i16 = lane & 15, k4 = lane >> 4;
   float a[4];
+  s16x4 ab[4];
   f32x4 cinit[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const float4 qq = wq[wbase + 16 * t + i16];
-    const float comp = k4 == 0 ? qq.x - cx : k4 == 1 ? qq.y - cy : k4 == 2 ? qq.z - cz : 1.0f;
-    a[t] = comp;
+    const float qx = qq.x - cx, qy = qq.y - cy, qz = qq.z - cz;
+    if constexpr (BF16) {
+      const float xh = bf16_round(qx), yh = bf16_round(qy), zh = bf16_round(qz);
+      ab[t] = a_slots(k4, xh, bf16_round(qx - xh), yh, bf16_round(qy - yh), zh,
+                      bf16_round(qz - zh));
+    } else {
+      a[t] = k4 == 0 ? qx : k4 == 1 ? qy : k4 == 2 ? qz : 1.0f;
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) cinit[t][v] = wq[wbase + 16 * t + 4 * k4 + v].w;
   }
@@ -123,13 +170,28 @@ __global__ __launch_bounds__(kThreads) void k_brute_mfma(
       const float4 p = stage[j];
       const bool valid = (sb + j) < s1;
       const float px = p.x - cx, py = p.y - cy, pz = p.z - cz;
-      const float b = k4 == 0 ? -2.0f * px
-                    : k4 == 1 ? -2.0f * py
-                    : k4 == 2 ? -2.0f * pz
-                              : px * px + py * py + pz * pz;
+      float b = 0.0f;
+      s16x4 bb;
+      if constexpr (BF16) {
+        const float xh = bf16_round(px), yh = bf16_round(py), zh = bf16_round(pz);
+        const float s2 = px * px + py * py + pz * pz;
+        const float sh = bf16_round(s2), sm = bf16_round(s2 - sh);
+        bb = b_slots(k4, -2.0f * xh, -2.0f * bf16_round(px - xh), -2.0f * yh,
+                     -2.0f * bf16_round(py - yh), -2.0f * zh, -2.0f * bf16_round(pz - zh), sh,
+                     sm, bf16_round(s2 - sh - sm));
+      } else {
+        b = k4 == 0 ? -2.0f * px
+          : k4 == 1 ? -2.0f * py
+          : k4 == 2 ? -2.0f * pz
+                    : px * px + py * py + pz * pz;
+      }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b, cinit[t], 0, 0, 0);
+        f32x4 d;
+        if constexpr (BF16)
+          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab[t], bb, cinit[t], 0, 0, 0);
+        else
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b, cinit[t], 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           float d2 = d[v];
@@ -198,7 +260,7 @@ uint32_t brute_mfma_split_count(uint32_t n, uint32_t nq) {
   return std::min(ns, std::max(1u, n / (8 * kStage)));
 }
 
-void run_brute_mfma(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
+void run_brute_mfma(const DeviceTree& t, int kernel, bool bf16, const float* q, uint32_t nq,
                     const uint32_t* order, float* out, double* partial, uint32_t nsplit,
                     hipStream_t s) {
   SBH_REQUIRE(t.dim == 3 && (kernel == GRAVITY || kernel == SMOOTH_DIST),
@@ -210,12 +272,14 @@ void run_brute_mfma(const DeviceTree& t, int kernel, const float* q, uint32_t nq
   dim3 grid(nblk(nq, kThreads), nsplit);
   const float alpha_k = t.alpha * 1.4426950408889634f;
   double* part = nsplit > 1 ? partial : nullptr;
+  auto launch = [&](auto kern) {
+    kern<<<grid, kThreads, 0, s>>>(t.pts.get(), t.n, alpha_k, qx, qy, qz, nq, order, per_split,
+                                   part, out);
+  };
   if (kernel == GRAVITY)
-    k_brute_mfma<GRAVITY><<<grid, kThreads, 0, s>>>(t.pts.get(), t.n, alpha_k, qx, qy, qz, nq,
-                                                    order, per_split, part, out);
+    bf16 ? launch(k_brute_mfma<GRAVITY, true>) : launch(k_brute_mfma<GRAVITY, false>);
   else
-    k_brute_mfma<SMOOTH_DIST><<<grid, kThreads, 0, s>>>(t.pts.get(), t.n, alpha_k, qx, qy, qz,
-                                                        nq, order, per_split, part, out);
+    bf16 ? launch(k_brute_mfma<SMOOTH_DIST, true>) : launch(k_brute_mfma<SMOOTH_DIST, false>);
   SBH_HIP(hipGetLastError());
   if (nsplit > 1) {
     k_reduce_splits_mfma<<<nblk(nq, 256), 256, 0, s>>>(partial, nsplit, nq, out);

@@ -366,7 +366,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     const sbh_eval_params& p = *params;
     SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
                              SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
-                             SBH_EVAL_PER_QUERY_PATHS)) == 0,
+                             SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16)) == 0,
                 "unknown eval flags");
     if (method == SBH_BARNES_HUT) {
       SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
@@ -400,7 +400,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     }
     // Counting only exists for the tree walks.
     if (method == SBH_BRUTE_FORCE) {
-      const bool mfma = (p.flags & SBH_EVAL_MFMA) != 0;
+      const bool mfma = (p.flags & (SBH_EVAL_MFMA | SBH_EVAL_MFMA_BF16)) != 0;
       if (mfma)
         SBH_REQUIRE(t.dim == 3 && (kernel == SBH_GRAVITY || kernel == SBH_SMOOTH_DIST),
                     "SBH_EVAL_MFMA covers 3D gravity and smooth distance");
@@ -413,7 +413,8 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       SBH_HIP(hipEventRecord(tree->ev0, s));
       if (mfma) {
         sbh::query_order(t, dq, nq, tree->order.get(), tree->sort_scratch.get(), s);
-        sbh::run_brute_mfma(t, kernel, dq, nq, tree->order.get(), dout, tree->partial.get(), ns,
+        sbh::run_brute_mfma(t, kernel, (p.flags & SBH_EVAL_MFMA_BF16) != 0, dq, nq,
+                            tree->order.get(), dout, tree->partial.get(), ns,
                             s);
       } else {
         sbh::run_brute(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);

@@ -272,9 +272,22 @@ py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, fl
 void bind_kernel(py::module_& m, const char* suffix, int kernel) {
   std::string s = suffix;
   m.def(("eval_brute_force_" + s).c_str(),
-        [kernel](GPUTreeBase& t, F32Array q, bool mfma) {
+        [kernel](GPUTreeBase& t, F32Array q, py::object mfma) {
+          // mfma: False, True / "f32" (fp32 operands) or "bf16" (split bf16 operands)
+          int flags = 0;
+          if (py::isinstance<py::str>(mfma)) {
+            const std::string m = mfma.cast<std::string>();
+            if (m == "bf16")
+              flags = SBH_EVAL_MFMA_BF16;
+            else if (m == "f32")
+              flags = SBH_EVAL_MFMA;
+            else
+              throw py::value_error("mfma must be False, True, 'f32' or 'bf16'");
+          } else if (mfma.cast<bool>()) {
+            flags = SBH_EVAL_MFMA;
+          }
           return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false, false,
-                          mfma ? SBH_EVAL_MFMA : 0);
+                          flags);
         },
         "tree"_a, "queries"_a, "mfma"_a = false);
   m.def(("eval_barnes_hut_" + s).c_str(),

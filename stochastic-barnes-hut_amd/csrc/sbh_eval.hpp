@@ -20,10 +20,10 @@ inline TreeView view_of(const DeviceTree& t) {
 uint32_t brute_split_count(uint32_t n, uint32_t nq);
 void run_brute(const DeviceTree& t, int kernel, const float* q, uint32_t nq, float* out,
                double* partial, uint32_t nsplit, hipStream_t s);
-// Matrix-core squared distances (3D gravity / smooth distance); `order` is a
-// spatial order of the queries (query_order).
+// Matrix-core squared distances (3D gravity / smooth distance) with fp32 or
+// split bf16 operands; `order` is a spatial order of the queries (query_order).
 uint32_t brute_mfma_split_count(uint32_t n, uint32_t nq);
-void run_brute_mfma(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
+void run_brute_mfma(const DeviceTree& t, int kernel, bool bf16, const float* q, uint32_t nq,
                     const uint32_t* order, float* out, double* partial, uint32_t nsplit,
                     hipStream_t s);
 void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq, double* out,

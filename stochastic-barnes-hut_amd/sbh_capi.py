@@ -21,7 +21,7 @@ KERNELS = {"gravity": GRAVITY, "wn": WINDING_NUMBER, "smoothdist": SMOOTH_DIST}
 BRUTE_FORCE, BARNES_HUT, STOCHASTIC = 0, 1, 2
 TREE_MOMENTS_F64 = 1
 EVAL_DEVICE_POINTERS, EVAL_SORT_QUERIES, EVAL_SPLIT_COUNTS = 1, 2, 4
-EVAL_EXACT_TERMS, EVAL_MFMA, EVAL_PER_QUERY_PATHS = 8, 16, 32
+EVAL_EXACT_TERMS, EVAL_MFMA, EVAL_PER_QUERY_PATHS, EVAL_MFMA_BF16 = 8, 16, 32, 64
 
 
 class EvalParams(ctypes.Structure):

@@ -160,9 +160,10 @@ def test_brute_force(sbh_gpu, dim, kernel):
     assert rel_err(g64, ref64, kernel).max() <= 1e-12
 
 
+@pytest.mark.parametrize("operands", ["f32", "bf16"])
 @pytest.mark.parametrize("kernel", ["gravity", "smoothdist"])
 @pytest.mark.parametrize("workload", ["cube", "plummer"])
-def test_brute_force_mfma(sbh_gpu, kernel, workload):
+def test_brute_force_mfma(sbh_gpu, kernel, workload, operands):
     """Matrix-core squared distances with the near field recomputed: within
     1e-5 of the oracle and of float64, for every query (ragged count).  The
     untruncated Plummer sphere (coordinates up to ~1e3) covers large offsets
@@ -175,14 +176,14 @@ def test_brute_force_mfma(sbh_gpu, kernel, workload):
     lo, hi = W.bounds(pts)
     alpha = 10.0 if kernel == "smoothdist" else 1.0
     g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1, alpha=alpha)
-    got, ms = getattr(sbh_gpu, "eval_brute_force_" + kernel)(g, q, mfma=True)
+    got, ms = getattr(sbh_gpu, "eval_brute_force_" + kernel)(g, q, mfma=operands)
     ref = o.brute_force(kernel, q)
     ref64 = o.brute_force(kernel, q, fp64=True)
     assert np.isfinite(got).all()
     if kernel == "smoothdist":
         assert (ref64 > 0).all()
     err = rel_err(got, ref64, kernel)
-    print(f"[mfma] {kernel} {workload}: max {err.max():.2e} mean {err.mean():.2e}")
+    print(f"[mfma] {operands} {kernel} {workload}: max {err.max():.2e} mean {err.mean():.2e}")
     assert rel_err(got, ref, kernel).max() <= TOL
     assert err.max() <= TOL
 
