@@ -316,7 +316,12 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("k_stochastic_plummer_1m_spl512")
+            # measured for this exact configuration by the two counter runs
+            # (FETCH_SIZE, WRITE_SIZE) of the profiling mode; absent otherwise
+            key = "k_stochastic_plummer_1m_k%d_spl%d_%s" % (
+                hs["tree_k"], hs["samples_per_subdomain"],
+                "sorted" if hs["sorted"] else "unsorted")
+            traffic = json.load(open(tpath)).get(key)
         except Exception:
             traffic = None
     roofline = {

@@ -33,11 +33,12 @@ constexpr uint32_t kVersion = 1;
 constexpr uint32_t kEndHost = 0xFFFFFFFFu;      // rope terminator
 constexpr uint32_t kLeafBitHost = 0x80000000u;  // leaf flag in a walk record
 
+// FNV-1a, 64-bit: offset basis 0xcbf29ce484222325, prime 0x100000001b3.
 uint64_t fnv1a(const uint8_t* p, size_t n) {
-  uint64_t h = 1469598103934665603ull;
+  uint64_t h = 0xcbf29ce484222325ull;
   for (size_t i = 0; i < n; ++i) {
     h ^= p[i];
-    h *= 1099511628211ull;
+    h *= 0x100000001b3ull;
   }
   return h;
 }
