@@ -183,7 +183,8 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
                                                           int split_counts) {
   __shared__ float4 sSrc[kThreads];
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
-  __shared__ uint4 sInfo[kThreads];
+  __shared__ uint4 sRec[kThreads];
+  __shared__ uint2 sAux[kThreads];
   const uint32_t wbase = threadIdx.x & ~63u;
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
   const bool live = slot < nq;
@@ -191,9 +192,10 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const LaneResult r = stochastic_estimate<D, K, EXACT>(
-      t, spl, key0, key1, g, live, x, y, z, sSrc + wbase,
-      sNrm + (K == WINDING_NUMBER ? wbase : 0), sInfo + wbase, counts != nullptr);
+  const SamplerLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
+                       sAux + wbase};
+  const LaneResult r =
+      stochastic_estimate<D, K, EXACT>(t, spl, key0, key1, g, live, x, y, z, lds, counts != nullptr);
   if (live) {
     out[qi] = r.value;
     if (counts) {

@@ -140,8 +140,6 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
 // LDS with one coalesced load per lane (children are contiguous in
 // breadth-first order) and every lane then reads them as broadcasts.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kMulti = 1u;  // leaf with more than one point
-
 // Adds node v's terms to acc: its points one by one if v is a leaf (in point
 // order, as the reference sums them), else its moment.
 template <int D, int K, bool EXACT>
@@ -170,14 +168,34 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// sSrc / sNrm / sInfo: this wavefront's 64 LDS slots.  g: wavefront index
-// within the request (the path key).
+// This wavefront's 64 LDS slots for staging the children of a path node.
+struct SamplerLds {
+  float4* src;  // term source: node moment, or the point of a one-point leaf
+  float4* nrm;  // its normal (winding number only)
+  uint4* rec;   // the child's walk record (nodeL)
+  uint2* aux;   // {end of the child's single-child chain, the child's point count}
+};
+
+// Weight of one level: nS / (P_RR * n_prev) * diff, with the reference's
+// operation order; the fast mode uses the hardware reciprocal for the divide.
+template <bool EXACT>
+__device__ __forceinline__ float level_weight(float nS, float event_prob, float n_prev,
+                                              float diff) {
+#pragma clang fp contract(off)
+  const float den = event_prob * n_prev;
+  if constexpr (EXACT)
+    return nS / den * diff;
+  else
+    return nS * __builtin_amdgcn_rcpf(den) * diff;
+}
+
+// g: wavefront index within the request (the path key).
 template <int D, int K, bool EXACT>
 __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int spl,
                                                           uint32_t key0, uint32_t key1,
                                                           uint32_t g, bool live, float x, float y,
-                                                          float z, float4* sSrc, float4* sNrm,
-                                                          uint4* sInfo, bool count = true) {
+                                                          float z, SamplerLds lds,
+                                                          bool count = true) {
 #pragma clang fp contract(off)
   constexpr bool kWN = (K == WINDING_NUMBER);
   const uint32_t lane = threadIdx.x & 63;
@@ -207,7 +225,9 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
       const uint32_t nS = n_end[S] - sstart;
       float sub_total = 0.0f;
       if (!(Sl.w & kLeafBit)) {
-        const float cS = ratio_exact<D>(nodeC[S], __uint_as_float(Sl.x), x, y, z);
+        const float4 SC = nodeC[S];
+        const float4 SN = kWN ? nodeN[S] : SC;
+        const float cS = ratio_exact<D>(SC, __uint_as_float(Sl.x), x, y, z);
         for (int ss = 0; ss < spl; ++ss) {
           Philox4 r0 = philox4x32_10((uint32_t)ss, si, g, 0u, key0, key1);
           const uint32_t idx = uniform_u32(sstart + scale_draw(r0.x, r0.y, nS));
@@ -217,10 +237,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
           float wd_all = 0.0f;
           float cur_beta = fmaxf(1.0f, cS);
           bool alive = live;
-          uint32_t cur = S;
           uint32_t level = 0;
+          // the path node: its walk record, moment, normal and point count
           uint4 curL = Sl;
-          float4 curC = nodeC[S];
+          float4 curC = SC, curN = SN;
+          uint32_t cur_cnt = nS;
           while (!(curL.w & kLeafBit)) {
             if (__ballot(alive) == 0) break;
             const uint32_t wi = level + 2;
@@ -243,37 +264,42 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
               }
             }
             if (__ballot(alive) == 0) break;
-            const uint32_t prev = cur;
-            const uint32_t f = curL.y, nch = curL.w;
+            const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
             float cur_result = 0.0f;
-            uint32_t pick = 0;
+            // record of the child holding the sample (captured from LDS)
+            uint4 nextL = curL;
+            float4 nextC = curC, nextN = curN;
+            uint2 nextAux = make_uint2(0u, 0u);
+            uint32_t chosen = 0;
             for (uint32_t b = 0; b < nch; b += 64) {
-              // stage children b .. b+63 of prev
+              // stage children b .. b+63 of the path node, one per lane
               const uint32_t cj = b + lane;
               const bool valid = cj < nch;
               uint32_t cstart = 0xFFFFFFFFu;
+              uint32_t np_lane = 0;
               if (valid) {
                 const uint32_t c = f + cj;
                 const uint4 l = nodeL[c];
                 cstart = n_start[c];
+                const uint32_t cend = n_end[c];
+                const uint32_t sk = n_skip[c];
                 float4 src, nv = make_float4(0.f, 0.f, 0.f, 0.f);
-                uint4 info = make_uint4(0u, 0u, 0u, 0u);
                 if (l.w & kLeafBit) {
-                  const uint32_t np = l.w & ~kLeafBit;
+                  np_lane = l.w & ~kLeafBit;
                   src = pts[l.y];
                   if (kWN) nv = nrm[l.y];
-                  info = make_uint4(np > 1 ? kMulti : 0u, l.y, np, 0u);  // z: points
                 } else {
                   src = nodeC[c];
                   if (kWN) nv = nodeN[c];
-                  info = make_uint4(0u, 0u, 0u, 0u);
                 }
-                sSrc[lane] = src;
-                if (kWN) sNrm[lane] = nv;
-                sInfo[lane] = info;
+                lds.src[lane] = src;
+                if (kWN) lds.nrm[lane] = nv;
+                lds.rec[lane] = l;
+                lds.aux[lane] = make_uint2(sk, cend - cstart);
               }
-              pick += (uint32_t)__popcll(__ballot(valid && cstart <= idx));
-              const uint32_t np_lane = valid ? sInfo[lane].z : 0u;
+              // children are in point order: those starting at or before the
+              // sample form a prefix, the last of which holds it
+              const uint32_t below = (uint32_t)__popcll(__ballot(valid && cstart <= idx));
               const uint64_t multi = __ballot(np_lane > 1);
               const uint32_t m = min(64u, nch - b);
               wave_lds_sync();
@@ -282,23 +308,31 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
                   // every child is one source: a node moment or a one-point leaf
 #pragma unroll 4
                   for (uint32_t j = 0; j < m; ++j) {
-                    const float4 src = sSrc[j];
-                    const float4 nv = kWN ? sNrm[j] : src;
+                    const float4 src = lds.src[j];
+                    const float4 nv = kWN ? lds.nrm[j] : src;
                     cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
                   }
                 } else {
                   for (uint32_t j = 0; j < m; ++j) {
-                    const uint4 info = sInfo[j];
-                    if (info.x & kMulti) {
-                      for (uint32_t pp = info.y; pp < info.y + info.z; ++pp)
+                    const uint4 l = lds.rec[j];
+                    const uint32_t np = l.w & ~kLeafBit;
+                    if ((l.w & kLeafBit) && np > 1) {
+                      for (uint32_t pp = l.y; pp < l.y + np; ++pp)
                         cur_result += term<D, K, EXACT>(pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
                     } else {
-                      const float4 src = sSrc[j];
-                      const float4 nv = kWN ? sNrm[j] : src;
+                      const float4 src = lds.src[j];
+                      const float4 nv = kWN ? lds.nrm[j] : src;
                       cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
                     }
                   }
                 }
+              }
+              if (below > 0) {
+                nextL = lds.rec[below - 1];
+                nextC = lds.src[below - 1];
+                if (kWN) nextN = lds.nrm[below - 1];
+                nextAux = lds.aux[below - 1];
+                chosen = f + b + below - 1;
               }
               // term counts of this batch, the same for every lane that evaluated it
               if (count) {
@@ -315,17 +349,27 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
               }
               wave_lds_sync();
             }
-            // child holding the sample, then down any single-child chain
-            cur = uniform_u32(n_skip[f + pick - 1]);
-            curL = nodeL[cur];
-            curC = nodeC[cur];
+            // descend to the child holding the sample, then down any
+            // single-child chain (the same points, so the same count)
+            const float4 prevC = curC, prevN = curN;
+            const uint32_t prev_cnt = cur_cnt;
+            const uint32_t sk = uniform_u32(nextAux.x);
+            cur_cnt = uniform_u32(nextAux.y);
+            if (sk == uniform_u32(chosen)) {
+              curL = make_uint4(uniform_u32(nextL.x), uniform_u32(nextL.y), nextL.z,
+                                uniform_u32(nextL.w));
+              curC = nextC;
+              curN = nextN;
+            } else {
+              curL = nodeL[sk];
+              curC = nodeC[sk];
+              curN = kWN ? nodeN[sk] : curC;
+            }
             cur_beta = fmaxf(cur_beta, ffr);
             if (alive) {
-              const float prev_result =
-                  term<D, K, EXACT>(nodeC[prev], nodeN[prev], alpha, alpha_k, x, y, z);
+              const float prev_result = term<D, K, EXACT>(prevC, prevN, alpha, alpha_k, x, y, z);
               const float diff = cur_result - prev_result;
-              const float wd = (float)nS / (event_prob * (float)(n_end[prev] - n_start[prev])) * diff;
-              wd_all += wd;
+              wd_all += level_weight<EXACT>((float)nS, event_prob, (float)prev_cnt, diff);
             }
           }
           sub_total += wd_all;

@@ -60,11 +60,11 @@ struct Task {
 
 template <int D, int K>
 __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, uint32_t block,
-                                           bool live, float x, float y, float z, float4* sSrc,
-                                           float4* sNrm, uint4* sInfo) {
+                                           bool live, float x, float y, float z,
+                                           SamplerLds lds) {
   if (stochastic)
     return stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
-                                            sSrc, sNrm, sInfo, false)
+                                            lds, false)
         .value;
   return bh_walk<D, K, 1, false>(r.tree, r.beta, live, x, y, z).value;
 }
@@ -75,7 +75,8 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
                                                           uint32_t* __restrict__ next_task) {
   __shared__ float4 sSrc[kServeThreads];
   __shared__ float4 sNrm[kServeThreads];
-  __shared__ uint4 sInfo[kServeThreads];
+  __shared__ uint4 sRec[kServeThreads];
+  __shared__ uint2 sAux[kServeThreads];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   while (true) {
@@ -92,28 +93,26 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
     float x, y, z;
     load_query(r.qx, r.qy, r.qz, live ? slot : 0, dim3 ? 3 : 2, x, y, z);
     const bool stoch = variant & 1;
-    float4* a = sSrc + wbase;
-    float4* b = sNrm + wbase;
-    uint4* c = sInfo + wbase;
+    const SamplerLds c{sSrc + wbase, sNrm + wbase, sRec + wbase, sAux + wbase};
     float v;
     switch (variant >> 1) {
       case 0:
-        v = serve_one<2, GRAVITY>(r, stoch, tk.block, live, x, y, z, a, b, c);
+        v = serve_one<2, GRAVITY>(r, stoch, tk.block, live, x, y, z, c);
         break;
       case 1:
-        v = serve_one<2, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, a, b, c);
+        v = serve_one<2, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c);
         break;
       case 2:
-        v = serve_one<2, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, a, b, c);
+        v = serve_one<2, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c);
         break;
       case 3:
-        v = serve_one<3, GRAVITY>(r, stoch, tk.block, live, x, y, z, a, b, c);
+        v = serve_one<3, GRAVITY>(r, stoch, tk.block, live, x, y, z, c);
         break;
       case 4:
-        v = serve_one<3, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, a, b, c);
+        v = serve_one<3, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c);
         break;
       default:
-        v = serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, a, b, c);
+        v = serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c);
         break;
     }
     if (live) r.out[slot] = v;
