@@ -5,6 +5,7 @@ import and keep the reference's names."""
 import ctypes
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -147,3 +148,65 @@ def test_tree_image_loader_refuses_damage_before_the_device():
     assert L.sbh_tree_deserialize(None, 0, ctypes.byref(h)) == C.ERR_INVALID
     assert L.sbh_tree_load(b"/nonexistent/tree.sbht", ctypes.byref(h)) == C.ERR_INVALID
     assert not h.value
+
+
+GOLDEN_IMAGE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                            "tree_cube300_k2.sbht")
+
+
+def test_golden_tree_image_matches_oracle():
+    """A tree image written by the GPU build (tests/golden/make_tree_image.py)
+    holds the oracle's tree: leaf order, node ranges and fp32 masses, with a
+    valid FNV-1a checksum."""
+    import oracle
+    import tree_image as TI
+    sys.path.insert(0, os.path.dirname(GOLDEN_IMAGE))
+    import make_tree_image as M
+
+    img = open(GOLDEN_IMAGE, "rb").read()
+    im = TI.parse(img)
+    assert im["checksum"] == TI.fnv1a(img[:-8])
+    pts, nrm, m, lo, hi = M.inputs()
+    assert (im["dim"], im["split_log"], im["n"]) == (3, M.K, M.N)
+    o = oracle.Tree(pts, nrm, m, lo, hi, split_log=M.K)
+    assert im["num_nodes"] == o.num_nodes
+    np.testing.assert_array_equal(im["perm"].astype(np.int64), o.leaf_order())
+    on = o.nodes()
+    np.testing.assert_array_equal(im["n_start"].astype(np.int64), on["start"])
+    np.testing.assert_array_equal(im["n_end"].astype(np.int64), on["end"])
+    np.testing.assert_array_equal(im["nodeC"][:, 3].view(np.uint32), on["mass"].view(np.uint32))
+    np.testing.assert_array_equal(im["n_diam"].view(np.uint32), on["diam"].view(np.uint32))
+
+
+def test_golden_tree_image_validation_on_host():
+    """The loader validates on the host: the intact image passes validation
+    (and then needs a device), damaged but correctly sealed copies fail."""
+    import ctypes
+
+    import sbh_capi as C
+    import sbh_gpu
+    import tree_image as TI
+
+    if sbh_gpu.device_count() > 0:
+        pytest.skip("a GPU is visible (the GPU tests load images)")
+    L = C.lib()
+    img = open(GOLDEN_IMAGE, "rb").read()
+    im = TI.parse(img)
+    h = ctypes.c_void_p()
+    assert L.sbh_tree_deserialize(img, len(img), ctypes.byref(h)) == C.ERR_NODEVICE
+    cases = [("nodeL", 4, im["num_nodes"], "child"),       # root's first child out of range
+             ("dfsL", 16 * 3 + 8, 1, "rope"),              # rope pointing backwards
+             ("n_skip", 0, im["num_nodes"], "chain"),      # chain target out of range
+             ("perm", 0, 10**6, "leaf order"),
+             ("n_end", 0, 5, "")]                           # root no longer holds every point
+    for name, off, val, msg in cases:
+        bad = bytearray(img)
+        at = im["offsets"][name] + off
+        bad[at:at + 4] = np.uint32(val).tobytes()
+        bad = bytes(TI.reseal(bad))
+        assert L.sbh_tree_deserialize(bad, len(bad), ctypes.byref(h)) == C.ERR_INVALID, name
+        assert msg in L.sbh_last_error().decode(), (name, L.sbh_last_error())
+    bad = bytearray(img)
+    bad[100] ^= 1
+    assert L.sbh_tree_deserialize(bytes(bad), len(bad), ctypes.byref(h)) == C.ERR_INVALID
+    assert "checksum" in L.sbh_last_error().decode()
