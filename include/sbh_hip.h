@@ -59,13 +59,14 @@ typedef struct sbh_tree sbh_tree;
                                       vote_group 1 gives identical values either way */
 #define SBH_EVAL_SPLIT_COUNTS 4    /* counts holds 2q values: node terms in [0, q),
                                       point terms in [q, 2q) */
-#define SBH_EVAL_EXACT_TERMS 8     /* Barnes-Hut / stochastic: every term with correctly
-                                      rounded fp32 operations in the reference CPU
-                                      plugin's order (bit-reproducible); default is the
+#define SBH_EVAL_EXACT_TERMS 8     /* Barnes-Hut / stochastic: every term, and the
+                                      sampler's opening ratios, roulette probabilities
+                                      and weights, with correctly rounded fp32
+                                      operations in the reference CPU plugin's order
+                                      (bit-reproducible against it); default is the
                                       hardware sqrt / reciprocal / exp2 the reference's
                                       fast-math GPU build uses (CMakeLists.txt:27).
-                                      Opening tests, roulette and weights are exact in
-                                      both modes */
+                                      Barnes-Hut opening tests are exact in both modes */
 #define SBH_EVAL_MFMA 16           /* brute force, 3D gravity / smooth distance: squared
                                       distances |q'|^2 + |p'|^2 - 2 q'.p' on the matrix
                                       cores (V_MFMA_F32_16X16X4_F32) in coordinates

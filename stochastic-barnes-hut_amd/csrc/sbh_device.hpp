@@ -127,6 +127,48 @@ __device__ __forceinline__ float kernel_fast(float4 src, float4 nrm, float alpha
   }
 }
 
+// acc + kernel_fast(...), with the term's final multiply fused into the
+// accumulation.
+template <int D, int K>
+__device__ __forceinline__ float kernel_fast_acc(float acc, float4 src, float4 nrm, float alpha,
+                                                 float qx, float qy, float qz) {
+  float dx = src.x - qx;
+  float dy = src.y - qy;
+  float dz = (D == 3) ? (src.z - qz) : 0.0f;
+  float dsq = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
+                       : __builtin_fmaf(dx, dx, dy * dy);
+  if constexpr (K == GRAVITY) {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    return __builtin_fmaf(-src.w, __builtin_amdgcn_rcpf(dist + 1e-1f), acc);
+  } else if constexpr (K == WINDING_NUMBER) {
+    if constexpr (D == 3) {
+      float dist = __builtin_amdgcn_sqrtf(dsq);
+      float qdn = __builtin_fmaf(nrm.x, dx, __builtin_fmaf(nrm.y, dy, nrm.z * dz));
+      float den = kFourPi * __builtin_fmaf(dsq, dist, 1e-5f);
+      return __builtin_fmaf(src.w * qdn, __builtin_amdgcn_rcpf(den), acc);
+    } else {
+      float qdn = __builtin_fmaf(nrm.x, dx, nrm.y * dy);
+      return __builtin_fmaf(src.w * qdn, __builtin_amdgcn_rcpf(kTwoPi * (dsq + 1e-5f)), acc);
+    }
+  } else {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    return acc + __builtin_amdgcn_exp2f(-alpha * dist);
+  }
+}
+
+// Opening ratio ||q - c|| / diam with hardware sqrt and the caller's
+// hardware reciprocal of diam (a few ulp from ratio_exact).
+template <int D>
+__device__ __forceinline__ float ratio_fast(float4 c, float inv_diam, float qx, float qy,
+                                            float qz) {
+  float dx = qx - c.x;
+  float dy = qy - c.y;
+  float dz = (D == 3) ? (qz - c.z) : 0.0f;
+  float d2 = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
+                      : __builtin_fmaf(dx, dx, dy * dy);
+  return __builtin_amdgcn_sqrtf(d2) * inv_diam;
+}
+
 // float64 formulation of the same kernels (error reporting only).
 template <int D, int K>
 __device__ __forceinline__ double kernel_f64(float4 src, float4 nrm, float alpha, float qx,

@@ -51,6 +51,38 @@ __device__ __forceinline__ float term(float4 src, float4 nrm, float alpha, float
     return kernel_fast<D, K>(src, nrm, alpha_k, x, y, z);
 }
 
+// acc + term: exact (reference order, no fusion) or fast (the term's final
+// multiply fused into the accumulation).
+template <int D, int K, bool EXACT>
+__device__ __forceinline__ float accum(float acc, float4 src, float4 nrm, float alpha,
+                                       float alpha_k, float x, float y, float z) {
+#pragma clang fp contract(off)
+  if constexpr (EXACT)
+    return acc + kernel_exact<D, K>(src, nrm, alpha, x, y, z);
+  else
+    return kernel_fast_acc<D, K>(acc, src, nrm, alpha_k, x, y, z);
+}
+
+// Opening ratio of a sampler path node and the roulette probability
+// min(cur_beta / ratio, 1) (include/tree_sampler_gpu.h:83-92): correctly
+// rounded in the exact mode; with hardware sqrt / reciprocal in the fast mode,
+// as the reference's fast-math GPU build computes them.  Decisions and weights
+// use the same value either way, so the estimator stays unbiased.
+template <int D, bool EXACT>
+__device__ __forceinline__ float path_ratio(float4 c, float diam, float x, float y, float z) {
+  if constexpr (EXACT)
+    return ratio_exact<D>(c, diam, x, y, z);
+  else
+    return ratio_fast<D>(c, __builtin_amdgcn_rcpf(diam), x, y, z);
+}
+template <bool EXACT>
+__device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
+  if constexpr (EXACT)
+    return fminf(cur_beta / ratio, 1.0f);
+  else
+    return fminf(cur_beta * __builtin_amdgcn_rcpf(ratio), 1.0f);
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic Barnes-Hut (include/barnes_hut.h:23-78 per query,
 // include/barnes_hut_gpu.h:19-72 vote).
@@ -106,10 +138,10 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       if (leaf) {
         const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
         for (uint32_t j = p0; j < p1; ++j)
-          acc += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+          acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
         cnt += (int32_t)(p1 - p0);
       } else {
-        acc += term<D, K, EXACT>(c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
+        acc = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
         cnt_nodes += 1;
       }
     }
@@ -152,10 +184,10 @@ __device__ __forceinline__ void node_accumulate(const TreeView& t, uint32_t v, f
   if (leaf) {
     const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
     for (uint32_t j = p0; j < p1; ++j)
-      acc += term<D, K, EXACT>(t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+      acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
     np = p1 - p0;
   } else {
-    acc += term<D, K, EXACT>(t.nodeC[v], t.nodeN[v], t.alpha, t.alpha_k, x, y, z);
+    acc = accum<D, K, EXACT>(acc, t.nodeC[v], t.nodeN[v], t.alpha, t.alpha_k, x, y, z);
   }
   // value selects, not a selected reference: both counters stay in registers
   cnt += (int32_t)np;
@@ -227,7 +259,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
       if (!(Sl.w & kLeafBit)) {
         const float4 SC = nodeC[S];
         const float4 SN = kWN ? nodeN[S] : SC;
-        const float cS = ratio_exact<D>(SC, __uint_as_float(Sl.x), x, y, z);
+        const float cS = path_ratio<D, EXACT>(SC, __uint_as_float(Sl.x), x, y, z);
         for (int ss = 0; ss < spl; ++ss) {
           Philox4 r0 = philox4x32_10((uint32_t)ss, si, g, 0u, key0, key1);
           const uint32_t idx = uniform_u32(sstart + scale_draw(r0.x, r0.y, nS));
@@ -253,8 +285,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
             const uint32_t word = wsel == 0 ? blk.x : wsel == 1 ? blk.y : wsel == 2 ? blk.z : blk.w;
             const float p = rr_uniform(word);
             ++level;
-            const float ffr = ratio_exact<D>(curC, __uint_as_float(curL.x), x, y, z);
-            const float tp = fminf(cur_beta / ffr, 1.0f);
+            const float ffr = path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
+            const float tp = traverse_prob<EXACT>(cur_beta, ffr);
             if (alive) {
               if (p > tp) {
                 event_prob *= 1.0f - tp;
@@ -310,7 +342,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
                   for (uint32_t j = 0; j < m; ++j) {
                     const float4 src = lds.src[j];
                     const float4 nv = kWN ? lds.nrm[j] : src;
-                    cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
+                    cur_result = accum<D, K, EXACT>(cur_result, src, nv, alpha, alpha_k, x, y, z);
                   }
                 } else {
                   for (uint32_t j = 0; j < m; ++j) {
@@ -318,11 +350,12 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
                     const uint32_t np = l.w & ~kLeafBit;
                     if ((l.w & kLeafBit) && np > 1) {
                       for (uint32_t pp = l.y; pp < l.y + np; ++pp)
-                        cur_result += term<D, K, EXACT>(pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
+                        cur_result =
+                            accum<D, K, EXACT>(cur_result, pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
                     } else {
                       const float4 src = lds.src[j];
                       const float4 nv = kWN ? lds.nrm[j] : src;
-                      cur_result += term<D, K, EXACT>(src, nv, alpha, alpha_k, x, y, z);
+                      cur_result = accum<D, K, EXACT>(cur_result, src, nv, alpha, alpha_k, x, y, z);
                     }
                   }
                 }
@@ -422,7 +455,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& 
     const uint32_t sstart = n_start[S];
     const uint32_t nS = n_end[S] - sstart;
     const float4 SC = nodeC[S];
-    const float cS = ratio_exact<D>(SC, __uint_as_float(Sl.x), x, y, z);
+    const float cS = path_ratio<D, EXACT>(SC, __uint_as_float(Sl.x), x, y, z);
     float sub_total = 0.0f;
     for (int ss = 0; ss < spl; ++ss) {
       const Philox4 r0 = philox4x32_10((uint32_t)ss, si, qid, 0u, key0, key1);
@@ -446,8 +479,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& 
         const uint32_t word = wsel == 0 ? blk.x : wsel == 1 ? blk.y : wsel == 2 ? blk.z : blk.w;
         const float p = rr_uniform(word);
         ++level;
-        const float ffr = ratio_exact<D>(curC, __uint_as_float(curL.x), x, y, z);
-        const float tp = fminf(cur_beta / ffr, 1.0f);
+        const float ffr = path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
+        const float tp = traverse_prob<EXACT>(cur_beta, ffr);
         if (p > tp) {
           event_prob *= 1.0f - tp;
           break;
@@ -469,9 +502,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& 
         const float prev_result =
             term<D, K, EXACT>(nodeC[prev], nodeN[prev], alpha, alpha_k, x, y, z);
         const float diff = cur_result - prev_result;
-        const float wd =
-            (float)nS / (event_prob * (float)(n_end[prev] - n_start[prev])) * diff;
-        wd_all += wd;
+        wd_all += level_weight<EXACT>((float)nS, event_prob,
+                                      (float)(n_end[prev] - n_start[prev]), diff);
       }
       sub_total += wd_all;
     }

@@ -244,13 +244,22 @@ def test_stochastic_same_seed(sbh_gpu, kernel, dim, k):
         assert err.max() <= TOL, (spl, seed, err.max())
         if kernel != "smoothdist":
             np.testing.assert_array_equal(got, ref)
-        # hardware sqrt / reciprocal terms: same paths and counts, values within
-        # the fp32 rounding of the terms amplified by the sample weights
-        fast, _, fcnt = fn(g, q, spl, seed=seed, return_counts=True)
-        np.testing.assert_array_equal(fcnt, rcnt)
-        ferr = rel_err(fast, ref, kernel)
-        print(f"[fast] {kernel} d{dim} k{k} spl{spl}: mean {ferr.mean():.2e} max {ferr.max():.2e}")
-        assert ferr.mean() <= TOL and np.percentile(ferr, 99) <= 10 * TOL
+        _check_fast(fn(g, q, spl, seed=seed, return_counts=True), ref, rcnt, kernel,
+                    f"{kernel} d{dim} k{k} spl{spl}")
+
+
+def _check_fast(res, ref, rcnt, kernel, label):
+    """The fast mode (hardware sqrt / reciprocal / exp2 for terms, opening
+    ratios and roulette probabilities) follows the same paths as the exact
+    estimator except where a roulette uniform falls within a few ulp of the
+    probability; on every other query the terms agree to fp32 rounding."""
+    fast, _, fcnt = res
+    same = fcnt == rcnt
+    print(f"[fast] {label}: {np.count_nonzero(~same)} of {len(same)} paths differ", flush=True)
+    assert np.count_nonzero(~same) <= max(2, len(same) // 100)
+    ferr = rel_err(fast[same], ref[same], kernel)
+    print(f"[fast] {label}: mean {ferr.mean():.2e} max {ferr.max():.2e}")
+    assert ferr.mean() <= TOL and np.percentile(ferr, 99) <= 10 * TOL
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -279,9 +288,8 @@ def test_stochastic_per_query_paths(sbh_gpu, kernel, dim, k, leaf):
         one, _ = fn(g, q[37:38], spl, seed=seed, exact_terms=True, per_query_paths=True)
         assert rel_err(one, o.stochastic(kernel, q[37:38], spl, seed=seed, group=1),
                        kernel).max() <= TOL
-        fast, _ = fn(g, q, spl, seed=seed, per_query_paths=True)
-        ferr = rel_err(fast, ref, kernel)
-        assert ferr.mean() <= TOL and np.percentile(ferr, 99) <= 10 * TOL
+        _check_fast(fn(g, q, spl, seed=seed, per_query_paths=True, return_counts=True), ref,
+                    rcnt, kernel, f"per-query {kernel} d{dim} k{k} leaf{leaf} spl{spl}")
     # the shared-path estimator on multi-point leaves
     ref = o.stochastic(kernel, q, 2, seed=9, group=64)
     got, _ = fn(g, q, 2, seed=9, exact_terms=True)
@@ -348,7 +356,7 @@ def test_ragged_and_empty_queries(sbh_gpu):
         qq = q[:nq]
         a, _ = sbh_gpu.eval_brute_force_gravity(g, qq)
         b, _ = sbh_gpu.eval_barnes_hut_gravity(g, qq, 2.0)
-        c, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, qq, 2)
+        c, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, qq, 2, exact_terms=True)
         assert a.shape == b.shape == c.shape == (nq,)
         if nq:
             assert rel_err(a, o.brute_force("gravity", qq), "gravity").max() <= TOL
