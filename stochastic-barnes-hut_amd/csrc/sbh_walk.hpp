@@ -79,8 +79,8 @@ template <bool EXACT>
 __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
   if constexpr (EXACT)
     return fminf(cur_beta / ratio, 1.0f);
-  else
-    return fminf(cur_beta * __builtin_amdgcn_rcpf(ratio), 1.0f);
+  else  // exactly 1 whenever the correctly rounded quotient is >= 1
+    return ratio <= cur_beta ? 1.0f : fminf(cur_beta * __builtin_amdgcn_rcpf(ratio), 1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,6 +260,69 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
         const float4 SC = nodeC[S];
         const float4 SN = kWN ? nodeN[S] : SC;
         const float cS = path_ratio<D, EXACT>(SC, __uint_as_float(Sl.x), x, y, z);
+        // Level 0 is the same for every sample: at the subdomain the ratio
+        // equals cur_beta (or is below 1), so the roulette keeps every path
+        // (tp = 1, P_RR stays 1) and the level adds nS / (1 * nS) * (sum of
+        // S's children - f(S)) whatever point was drawn.  It is evaluated
+        // once here; each sample then starts from the child holding its point
+        // with that weight as the first term of its sum, as it would have
+        // computed it.  (Subdomains with more than 64 children take the
+        // general loop.)
+        const uint32_t f0 = uniform_u32(Sl.y), nch0 = uniform_u32(Sl.w);
+        const bool cached0 = nch0 <= 64;
+        float wd0 = 0.0f;
+        int32_t cnt0 = 0, cnt0_nodes = 0;
+        uint32_t c0_start = 0xFFFFFFFFu;  // lane j: start of S's child j
+        if (cached0) {
+          uint32_t np_lane = 0;
+          if (lane < nch0) {
+            const uint32_t c = f0 + lane;
+            const uint4 l = nodeL[c];
+            c0_start = n_start[c];
+            float4 src, nv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (l.w & kLeafBit) {
+              np_lane = l.w & ~kLeafBit;
+              src = pts[l.y];
+              if (kWN) nv = nrm[l.y];
+            } else {
+              src = nodeC[c];
+              if (kWN) nv = nodeN[c];
+            }
+            lds.src[lane] = src;
+            if (kWN) lds.nrm[lane] = nv;
+            lds.rec[lane] = l;
+          }
+          const uint64_t multi = __ballot(np_lane > 1);
+          wave_lds_sync();
+          float r0sum = 0.0f;
+          if (live) {
+            for (uint32_t j = 0; j < nch0; ++j) {
+              const uint4 l = lds.rec[j];
+              const uint32_t np = l.w & ~kLeafBit;
+              if (multi != 0 && (l.w & kLeafBit) && np > 1) {
+                for (uint32_t pp = l.y; pp < l.y + np; ++pp)
+                  r0sum = accum<D, K, EXACT>(r0sum, pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
+              } else {
+                const float4 src = lds.src[j];
+                const float4 nv = kWN ? lds.nrm[j] : src;
+                r0sum = accum<D, K, EXACT>(r0sum, src, nv, alpha, alpha_k, x, y, z);
+              }
+            }
+          }
+          if (count) {
+            uint32_t pt = np_lane, nd = (lane < nch0 && np_lane == 0) ? 1u : 0u;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+              pt += __shfl_xor(pt, off, 64);
+              nd += __shfl_xor(nd, off, 64);
+            }
+            cnt0 = (int32_t)pt;
+            cnt0_nodes = (int32_t)nd;
+          }
+          wave_lds_sync();
+          const float prev0 = term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z);
+          wd0 = level_weight<EXACT>((float)nS, 1.0f, (float)nS, r0sum - prev0);
+        }
         for (int ss = 0; ss < spl; ++ss) {
           Philox4 r0 = philox4x32_10((uint32_t)ss, si, g, 0u, key0, key1);
           const uint32_t idx = uniform_u32(sstart + scale_draw(r0.x, r0.y, nS));
@@ -274,6 +337,24 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
           uint4 curL = Sl;
           float4 curC = SC, curN = SN;
           uint32_t cur_cnt = nS;
+          if (cached0) {
+            // level 0 from the cache: descend to the child holding the point
+            const uint32_t below =
+                (uint32_t)__popcll(__ballot(lane < nch0 && c0_start <= idx));
+            const uint32_t chosen = uniform_u32(f0 + below - 1);
+            const uint32_t sk = n_skip[chosen];
+            cur_cnt = n_end[chosen] - n_start[chosen];
+            curL = nodeL[sk];
+            curC = nodeC[sk];
+            curN = kWN ? nodeN[sk] : curC;
+            cur_beta = fmaxf(cur_beta, cS);
+            if (live) {
+              wd_all += wd0;
+              cnt += cnt0;
+              cnt_nodes += cnt0_nodes;
+            }
+            level = 1;
+          }
           while (!(curL.w & kLeafBit)) {
             if (__ballot(alive) == 0) break;
             const uint32_t wi = level + 2;
@@ -456,6 +537,14 @@ __device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& 
     const uint32_t nS = n_end[S] - sstart;
     const float4 SC = nodeC[S];
     const float cS = path_ratio<D, EXACT>(SC, __uint_as_float(Sl.x), x, y, z);
+    // level 0, the same for every sample (see stochastic_estimate)
+    const uint32_t f0 = Sl.y, nch0 = Sl.w;
+    float r0sum = 0.0f;
+    int32_t cnt0 = 0, cnt0_nodes = 0;
+    for (uint32_t j = 0; j < nch0; ++j)
+      node_accumulate<D, K, EXACT>(t, f0 + j, x, y, z, r0sum, cnt0, cnt0_nodes);
+    const float wd0 = level_weight<EXACT>(
+        (float)nS, 1.0f, (float)nS, r0sum - term<D, K, EXACT>(SC, nodeN[S], alpha, alpha_k, x, y, z));
     float sub_total = 0.0f;
     for (int ss = 0; ss < spl; ++ss) {
       const Philox4 r0 = philox4x32_10((uint32_t)ss, si, qid, 0u, key0, key1);
@@ -464,11 +553,16 @@ __device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& 
       uint32_t blk_id = 0;
       float event_prob = 1.0f;
       float wd_all = 0.0f;
-      float cur_beta = fmaxf(1.0f, cS);
-      uint32_t level = 0;
-      uint32_t cur = S;
-      uint4 curL = Sl;
-      float4 curC = SC;
+      float cur_beta = fmaxf(cS, fmaxf(1.0f, cS));
+      uint32_t pick = 0;
+      for (uint32_t j = 1; j < nch0; ++j) pick = n_start[f0 + j] <= idx ? j : pick;
+      uint32_t cur = n_skip[f0 + pick];
+      uint4 curL = nodeL[cur];
+      float4 curC = nodeC[cur];
+      wd_all += wd0;
+      cnt += cnt0;
+      cnt_nodes += cnt0_nodes;
+      uint32_t level = 1;
       while (!(curL.w & kLeafBit)) {
         const uint32_t wi = level + 2;
         if ((wi >> 2) != blk_id) {
