@@ -209,6 +209,50 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
   }
 }
 
+// Fast-mode estimator with sorted samples and the per-level node cache, for
+// trees whose nodes have at most 64 children.
+template <int D, int K>
+__global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
+    TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
+    const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
+    const uint32_t* __restrict__ order, float* __restrict__ out, int32_t* __restrict__ counts,
+    int split_counts) {
+  constexpr uint32_t kWaves = kThreads / 64;
+  __shared__ float4 sSrc[kThreads];
+  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
+  __shared__ uint4 sRec[kThreads];
+  __shared__ uint4 sKeys[kWaves * kChunk];
+  __shared__ uint4 sInfo[kWaves * kCacheLevels];
+  __shared__ uint32_t sStart[kWaves * kCacheLevels * 64];
+  __shared__ float sDiff[kWaves * kCacheLevels * 64];
+  __shared__ float sRatio[kWaves * kCacheLevels * 64];
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t wbase = threadIdx.x & ~63u;
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = slot < nq;
+  const uint32_t g = uniform_u32(slot >> 6);
+  const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
+  float x, y, z;
+  load_query(qx, qy, qz, qi, D, x, y, z);
+  const uint32_t cl = wave * kCacheLevels;
+  const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
+                      sKeys + wave * kChunk, sInfo + cl, sStart + cl * 64, sDiff + cl * 64,
+                      sRatio + cl * 64};
+  const LaneResult r =
+      stochastic_estimate_sorted<D, K>(t, spl, key0, key1, g, live, x, y, z, lds, counts != nullptr);
+  if (live) {
+    out[qi] = r.value;
+    if (counts) {
+      if (split_counts) {
+        counts[qi] = r.nodes;
+        counts[nq + qi] = r.points;
+      } else {
+        counts[qi] = r.points + r.nodes;
+      }
+    }
+  }
+}
+
 // One sample path per query: thread = query, path key = the query's index in
 // the caller's array.
 template <int D, int K, bool EXACT>
@@ -380,8 +424,14 @@ template <int D, int K, bool EXACT>
 void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, bool per_query, const float* qx,
                   const float* qy, const float* qz, uint32_t nq, const uint32_t* order,
                   float* out, int32_t* counts, int split, hipStream_t s) {
+  // nodes have at most 2^(dim * split_log) children
+  const bool small_fanout = t.dim * t.split_log <= 6;
   if (per_query)
     k_stochastic_query<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+        view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
+        counts, split);
+  else if (!EXACT && small_fanout)
+    k_stochastic_sorted<D, K><<<nblk(nq, kThreads), kThreads, 0, s>>>(
         view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
         counts, split);
   else

@@ -497,6 +497,288 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 }
 
 // ---------------------------------------------------------------------------
+// Fast-mode estimator for trees whose nodes have at most 64 children
+// (octrees k <= 2, quadtrees k <= 3): the same samples, the same per-sample
+// paths, roulette draws and weights as stochastic_estimate<D, K, false>,
+// evaluated with reuse across samples.
+//
+// A path node v contributes diff(v) = (sum of v's children's terms) - f(v)
+// weighted by the path's roulette probability; diff(v) and v's opening ratio
+// depend only on v and the query, not on the path that reached v.  The
+// samples of a subdomain are therefore drawn a chunk at a time, sorted by
+// their sampled point (so consecutive samples share the upper part of their
+// paths), and diff, ratio and children starts of the most recent node at
+// each of the top kCacheLevels levels are kept in LDS, keyed by node.  A
+// sample whose path passes a cached node reads them instead of evaluating
+// the node's children.  Only the order in which the samples' contributions
+// are summed differs from the per-sample loop.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kChunk = 128;        // samples drawn and sorted together
+constexpr uint32_t kCacheLevels = 4;    // path levels 1..4 cached
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+
+// This wavefront's LDS for the fast estimator.
+struct SortedLds {
+  float4* src;        // 64 staging slots: term sources
+  float4* nrm;        // 64 staging slots: normals (winding number only)
+  uint4* rec;         // 64 staging slots: children's walk records
+  uint4* keys;        // kChunk samples: {sampled point, sample, level-1 roulette word, 0}
+  uint4* cinfo;       // kCacheLevels: {node, point terms, node terms, 0}
+  uint32_t* cstart;   // kCacheLevels x 64: children starts of the cached node
+  float* cdiff;       // kCacheLevels x 64: diff of the cached node, per lane
+  float* cratio;      // kCacheLevels x 64: its opening ratio, per lane
+};
+
+// Sorts keys[0, kChunk) by .x (bitonic network; every lane takes kChunk / 64
+// compare-exchanges per stage).
+__device__ __forceinline__ void wave_sort_keys(uint4* keys) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t k = 2; k <= kChunk; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (uint32_t m = 0; m < kChunk / 64; ++m) {
+        const uint32_t i = lane + 64 * m;
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint4 a = keys[i], b = keys[ixj];
+          const bool asc = (i & k) == 0;
+          if ((a.x > b.x) == asc) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
+template <int D, int K>
+__device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView& t, int spl,
+                                                                 uint32_t key0, uint32_t key1,
+                                                                 uint32_t g, bool live, float x,
+                                                                 float y, float z, SortedLds lds,
+                                                                 bool count = true) {
+#pragma clang fp contract(off)
+  constexpr bool EXACT = false;
+  constexpr bool kWN = (K == WINDING_NUMBER);
+  const uint32_t lane = threadIdx.x & 63;
+  const float4* __restrict__ nodeC = t.nodeC;
+  const uint4* __restrict__ nodeL = t.nodeL;
+  const float4* __restrict__ nodeN = t.nodeN;
+  const uint32_t* __restrict__ n_start = t.n_start;
+  const uint32_t* __restrict__ n_end = t.n_end;
+  const uint32_t* __restrict__ n_skip = t.n_skip;
+  const float4* __restrict__ pts = t.pts;
+  const float4* __restrict__ nrm = t.nrm;
+  const float alpha = t.alpha, alpha_k = t.alpha_k;
+  int32_t cnt = 0, cnt_nodes = 0;
+  const uint4 rl = nodeL[0];
+  if (rl.w & kLeafBit) {
+    float v = 0.0f;
+    node_accumulate<D, K, EXACT>(t, 0, x, y, z, v, cnt, cnt_nodes);
+    return {v, cnt, cnt_nodes};
+  }
+  if (lane < kCacheLevels) lds.cinfo[lane] = make_uint4(kNoNode, 0u, 0u, 0u);
+  wave_lds_sync();
+
+  // Evaluates the children of v (at most 64) for every live lane: returns
+  // their sum; *start = this lane's child's start (~0 past the last child);
+  // *pt, *nd = the batch's point and node term counts (if counted).
+  auto children_sum = [&](uint32_t f, uint32_t nch, uint32_t* start, uint32_t* pt,
+                          uint32_t* nd) -> float {
+    uint32_t cstart = 0xFFFFFFFFu, np_lane = 0;
+    if (lane < nch) {
+      const uint32_t c = f + lane;
+      const uint4 l = nodeL[c];
+      cstart = n_start[c];
+      float4 src, nv = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (l.w & kLeafBit) {
+        np_lane = l.w & ~kLeafBit;
+        src = pts[l.y];
+        if (kWN) nv = nrm[l.y];
+      } else {
+        src = nodeC[c];
+        if (kWN) nv = nodeN[c];
+      }
+      lds.src[lane] = src;
+      if (kWN) lds.nrm[lane] = nv;
+      lds.rec[lane] = l;
+    }
+    const uint64_t multi = __ballot(np_lane > 1);
+    wave_lds_sync();
+    float r = 0.0f;
+    if (live) {
+      if (multi == 0) {
+#pragma unroll 4
+        for (uint32_t j = 0; j < nch; ++j) {
+          const float4 src = lds.src[j];
+          const float4 nv = kWN ? lds.nrm[j] : src;
+          r = accum<D, K, EXACT>(r, src, nv, alpha, alpha_k, x, y, z);
+        }
+      } else {
+        for (uint32_t j = 0; j < nch; ++j) {
+          const uint4 l = lds.rec[j];
+          const uint32_t np = l.w & ~kLeafBit;
+          if ((l.w & kLeafBit) && np > 1) {
+            for (uint32_t pp = l.y; pp < l.y + np; ++pp)
+              r = accum<D, K, EXACT>(r, pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
+          } else {
+            const float4 src = lds.src[j];
+            const float4 nv = kWN ? lds.nrm[j] : src;
+            r = accum<D, K, EXACT>(r, src, nv, alpha, alpha_k, x, y, z);
+          }
+        }
+      }
+    }
+    *pt = 0;
+    *nd = 0;
+    if (count) {
+      uint32_t a = np_lane, b = (lane < nch && np_lane == 0) ? 1u : 0u;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+      }
+      *pt = a;
+      *nd = b;
+    }
+    wave_lds_sync();
+    *start = cstart;
+    return r;
+  };
+
+  float bh = 0.0f, mlcv = 0.0f;
+  const uint32_t nsub = rl.w, first = rl.y;
+  for (uint32_t si = 0; si < nsub; ++si) {
+    const uint32_t S = uniform_u32(first + si);
+    node_accumulate<D, K, EXACT>(t, S, x, y, z, bh, cnt, cnt_nodes);
+    const uint4 Sl = nodeL[S];
+    if (Sl.w & kLeafBit) continue;  // the subdomain's points were summed exactly
+    const uint32_t sstart = n_start[S];
+    const uint32_t nS = n_end[S] - sstart;
+    const float4 SC = nodeC[S];
+    const float4 SN = kWN ? nodeN[S] : SC;
+    const float cS = path_ratio<D, EXACT>(SC, __uint_as_float(Sl.x), x, y, z);
+    // level 0, the same for every sample (see stochastic_estimate)
+    const uint32_t f0 = uniform_u32(Sl.y), nch0 = uniform_u32(Sl.w);
+    uint32_t c0_start, pt0, nd0;
+    const float r0sum = children_sum(f0, nch0, &c0_start, &pt0, &nd0);
+    const float wd0 = level_weight<EXACT>((float)nS, 1.0f, (float)nS,
+                                          r0sum - term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z));
+    float sub_total = 0.0f;
+    for (int s0 = 0; s0 < spl; s0 += (int)kChunk) {
+      const uint32_t ncur = (uint32_t)min((int)kChunk, spl - s0);
+      // draw the chunk's sampled points (Philox block 0 of each sample)
+#pragma unroll
+      for (uint32_t m = 0; m < kChunk / 64; ++m) {
+        const uint32_t i = lane + 64 * m;
+        uint4 key = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+        if (i < ncur) {
+          const uint32_t ss = (uint32_t)s0 + i;
+          const Philox4 r = philox4x32_10(ss, si, g, 0u, key0, key1);
+          key = make_uint4(sstart + scale_draw(r.x, r.y, nS), ss, r.w, 0u);
+        }
+        lds.keys[i] = key;
+      }
+      wave_lds_sync();
+      wave_sort_keys(lds.keys);
+      for (uint32_t i = 0; i < ncur; ++i) {
+        const uint4 key = lds.keys[i];
+        const uint32_t idx = uniform_u32(key.x), ss = uniform_u32(key.y);
+        const uint32_t w1 = uniform_u32(key.z);
+        // level 0 from the subdomain's cache
+        const uint32_t below0 = (uint32_t)__popcll(__ballot(lane < nch0 && c0_start <= idx));
+        const uint32_t chosen0 = uniform_u32(f0 + below0 - 1);
+        uint32_t cur = n_skip[chosen0];
+        uint32_t cur_cnt = n_end[chosen0] - n_start[chosen0];
+        uint4 curL = nodeL[cur];
+        float4 curC = nodeC[cur];
+        float4 curN = kWN ? nodeN[cur] : curC;
+        float event_prob = 1.0f;
+        float wd_all = 0.0f;
+        float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
+        bool alive = live;
+        if (live) {
+          wd_all += wd0;
+          cnt += (int32_t)pt0;
+          cnt_nodes += (int32_t)nd0;
+        }
+        Philox4 blk{0u, 0u, 0u, w1};  // block 0: only its level-1 word is used here
+        uint32_t blk_id = 0;
+        uint32_t level = 1;
+        while (!(curL.w & kLeafBit)) {
+          const uint32_t wi = level + 2;
+          if ((wi >> 2) != blk_id) {
+            blk_id = wi >> 2;
+            blk = philox4x32_10(ss, si, g, blk_id, key0, key1);
+          }
+          const uint32_t wsel = wi & 3;
+          const uint32_t word = wsel == 0 ? blk.x : wsel == 1 ? blk.y : wsel == 2 ? blk.z : blk.w;
+          const float p = rr_uniform(word);
+          const uint32_t slot = level - 1;
+          const uint32_t cslot = min(slot, kCacheLevels - 1) * 64 + lane;  // in-bounds address
+          const bool hit = slot < kCacheLevels &&
+                           uniform_u32(lds.cinfo[min(slot, kCacheLevels - 1)].x) == cur;
+          const float ffr = hit ? lds.cratio[cslot]
+                                : path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
+          ++level;
+          const float tp = traverse_prob<EXACT>(cur_beta, ffr);
+          if (alive) {
+            if (p > tp) {
+              event_prob *= 1.0f - tp;
+              alive = false;
+            } else {
+              event_prob *= tp;
+            }
+          }
+          if (__ballot(alive) == 0) break;
+          const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
+          float diff;
+          uint32_t below, pt, nd;
+          if (hit) {
+            diff = lds.cdiff[cslot];
+            const uint32_t cs = lane < nch ? lds.cstart[cslot] : 0xFFFFFFFFu;
+            below = (uint32_t)__popcll(__ballot(cs <= idx));
+            const uint4 ci = lds.cinfo[slot];
+            pt = uniform_u32(ci.y);
+            nd = uniform_u32(ci.z);
+          } else {
+            uint32_t cs;
+            const float r = children_sum(f, nch, &cs, &pt, &nd);
+            diff = r - term<D, K, EXACT>(curC, curN, alpha, alpha_k, x, y, z);
+            below = (uint32_t)__popcll(__ballot(cs <= idx));
+            if (slot < kCacheLevels) {
+              lds.cdiff[cslot] = diff;
+              lds.cratio[cslot] = ffr;
+              lds.cstart[cslot] = cs;
+              if (lane == 0) lds.cinfo[slot] = make_uint4(cur, pt, nd, 0u);
+              wave_lds_sync();
+            }
+          }
+          if (alive) {
+            wd_all += level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
+            cnt += (int32_t)pt;
+            cnt_nodes += (int32_t)nd;
+          }
+          cur_beta = fmaxf(cur_beta, ffr);
+          const uint32_t chosen = uniform_u32(f + below - 1);
+          cur = n_skip[chosen];
+          cur_cnt = n_end[chosen] - n_start[chosen];
+          curL = nodeL[cur];
+          curC = nodeC[cur];
+          curN = kWN ? nodeN[cur] : curC;
+        }
+        sub_total += wd_all;
+      }
+      wave_lds_sync();
+    }
+    mlcv += sub_total / (float)spl;
+  }
+  return {bh + mlcv, cnt, cnt_nodes};
+}
+
+// ---------------------------------------------------------------------------
 // The same estimator with one sample path per query, as the reference's CPU
 // sampler draws them (include/tree_sampler.h:21-113: every query picks its
 // own point and runs its own roulette).  The path of sample `ss` in subdomain
