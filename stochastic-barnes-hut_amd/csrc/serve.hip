@@ -51,6 +51,7 @@ struct ReqDesc {
   float beta;
   int32_t spl;
   uint32_t key0, key1;
+  int32_t sorted;  // nodes have at most 64 children: the sorted-sample estimator
 };
 
 struct Task {
@@ -61,7 +62,13 @@ struct Task {
 template <int D, int K>
 __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, uint32_t block,
                                            bool live, float x, float y, float z,
-                                           SamplerLds lds) {
+                                           SamplerLds lds, SortedLds sorted_lds) {
+  // the same estimator as a one-shot evaluation of the request picks, so a
+  // served result equals the one-shot result
+  if (stochastic && uniform_u32((uint32_t)r.sorted))
+    return stochastic_estimate_sorted<D, K>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
+                                            sorted_lds, false)
+        .value;
   if (stochastic)
     return stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
                                             lds, false)
@@ -77,6 +84,12 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
   __shared__ float4 sNrm[kServeThreads];
   __shared__ uint4 sRec[kServeThreads];
   __shared__ uint2 sAux[kServeThreads];
+  constexpr uint32_t kWaves = kServeThreads / 64;
+  __shared__ uint4 sKeys[kWaves * kChunk];
+  __shared__ uint4 sInfo[kWaves * kCacheLevels];
+  __shared__ uint32_t sStart[kWaves * kCacheLevels * 64];
+  __shared__ float sDiff[kWaves * kCacheLevels * 64];
+  __shared__ float sRatio[kWaves * kCacheLevels * 64];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   while (true) {
@@ -94,25 +107,28 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
     load_query(r.qx, r.qy, r.qz, live ? slot : 0, dim3 ? 3 : 2, x, y, z);
     const bool stoch = variant & 1;
     const SamplerLds c{sSrc + wbase, sNrm + wbase, sRec + wbase, sAux + wbase};
+    const uint32_t cl = (threadIdx.x >> 6) * kCacheLevels;
+    const SortedLds sc{sSrc + wbase, sNrm + wbase, sRec + wbase, sKeys + (threadIdx.x >> 6) * kChunk,
+                       sInfo + cl, sStart + cl * 64, sDiff + cl * 64, sRatio + cl * 64};
     float v;
     switch (variant >> 1) {
       case 0:
-        v = serve_one<2, GRAVITY>(r, stoch, tk.block, live, x, y, z, c);
+        v = serve_one<2, GRAVITY>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
       case 1:
-        v = serve_one<2, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c);
+        v = serve_one<2, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
       case 2:
-        v = serve_one<2, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c);
+        v = serve_one<2, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
       case 3:
-        v = serve_one<3, GRAVITY>(r, stoch, tk.block, live, x, y, z, c);
+        v = serve_one<3, GRAVITY>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
       case 4:
-        v = serve_one<3, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c);
+        v = serve_one<3, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
       default:
-        v = serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c);
+        v = serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
     }
     if (live) r.out[slot] = v;
@@ -463,6 +479,7 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     d.spl = r->params.samples_per_subdomain;
     d.key0 = (uint32_t)r->params.seed;
     d.key1 = (uint32_t)(r->params.seed >> 32);
+    d.sorted = t.dim * t.split_log <= 6 ? 1 : 0;  // at most 2^6 children per node
     L.h_descs.push_back(d);
     const uint32_t nb = (r->nq + 63) / 64;
     for (uint32_t b = 0; b < nb; ++b) L.h_tasks.push_back(Task{(uint32_t)i, b});
