@@ -529,17 +529,17 @@ struct SortedLds {
   float* cratio;      // kCacheLevels x 64: its opening ratio, per lane
 };
 
-// Sorts keys[0, kChunk) by .x (bitonic network; every lane takes kChunk / 64
-// compare-exchanges per stage).
-__device__ __forceinline__ void wave_sort_keys(uint4* keys) {
+// Sorts keys[0, n) by .x, n a power of two <= kChunk (bitonic network;
+// every lane takes up to kChunk / 64 compare-exchanges per stage).
+__device__ __forceinline__ void wave_sort_keys(uint4* keys, uint32_t n) {
   const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t k = 2; k <= kChunk; k <<= 1) {
+  for (uint32_t k = 2; k <= n; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
 #pragma unroll
       for (uint32_t m = 0; m < kChunk / 64; ++m) {
         const uint32_t i = lane + 64 * m;
         const uint32_t ixj = i ^ j;
-        if (ixj > i) {
+        if (i < n && ixj > i) {
           const uint4 a = keys[i], b = keys[ixj];
           const bool asc = (i & k) == 0;
           if ((a.x > b.x) == asc) {
@@ -682,7 +682,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         lds.keys[i] = key;
       }
       wave_lds_sync();
-      wave_sort_keys(lds.keys);
+      // sort the drawn samples (and sentinels up to the next power of two)
+      uint32_t nsort = 1;
+      while (nsort < ncur) nsort <<= 1;
+      wave_sort_keys(lds.keys, nsort);
       for (uint32_t i = 0; i < ncur; ++i) {
         const uint4 key = lds.keys[i];
         const uint32_t idx = uniform_u32(key.x), ss = uniform_u32(key.y);
