@@ -325,11 +325,16 @@ __global__ void k_morton(const float* qx, const float* qy, const float* qz, uint
 }
 
 // LSD radix sort of (key, index) pairs, 8 bits per pass: per-block digit
-// histograms, a digit-major scan of them, then a stable scatter in which each
-// thread owns one digit and walks its block's tile in order.
+// histograms, a digit-major scan of them, then a stable scatter.  In the
+// scatter each wavefront ranks its 64-element chunks with eight ballots (a
+// lane's rank among the lanes holding the same digit), the block adds the
+// chunks' digit counts in chunk order, and every element writes to its
+// digit's block offset + the counts of earlier chunks + its rank.
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 4;  // per thread
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSortChunks = kSortTile / 64;
+static_assert(kSortThreads == 256, "the scatter's prefix step takes one thread per digit");
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint64_t* keys, uint32_t n,
                                                             int shift, uint32_t nblocks,
@@ -352,23 +357,51 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint64_t* k
                                                                uint32_t* vals_out, uint32_t n,
                                                                int shift, uint32_t nblocks,
                                                                const uint32_t* offsets) {
-  __shared__ uint8_t td[kSortTile];
-  __shared__ uint32_t tv[kSortTile];
-  uint32_t base = blockIdx.x * kSortTile;
-  uint32_t cnt = min((uint32_t)kSortTile, n - base);
-  for (uint32_t j = threadIdx.x; j < cnt; j += kSortThreads) {
-    td[j] = (uint8_t)((keys_in[base + j] >> shift) & 0xFFu);
-    tv[j] = j;
+  __shared__ uint32_t cnt[kSortChunks * 256];   // [chunk][digit] counts
+  __shared__ uint32_t start[kSortChunks * 256];  // [chunk][digit] output starts
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < kSortChunks * 256; i += kSortThreads) cnt[i] = 0;
+  __syncthreads();
+  const uint32_t tile = blockIdx.x * kSortTile;
+  uint64_t key[kSortItems];
+  uint32_t val[kSortItems], dig[kSortItems], rank[kSortItems];
+  bool ok[kSortItems];
+#pragma unroll
+  for (int m = 0; m < kSortItems; ++m) {
+    const uint32_t c = wave + (kSortThreads / 64) * m;  // chunks in tile order
+    const uint32_t i = tile + c * 64 + lane;
+    ok[m] = i < n;
+    key[m] = ok[m] ? keys_in[i] : 0ull;
+    val[m] = ok[m] ? vals_in[i] : 0u;
+    const uint32_t d = (uint32_t)(key[m] >> shift) & 0xFFu;
+    dig[m] = d;
+    uint64_t peers = __ballot(ok[m]);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t set = __ballot(bit);
+      peers &= bit ? set : ~set;
+    }
+    rank[m] = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    if (ok[m] && rank[m] == 0) cnt[c * 256 + d] = (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  const uint32_t d = threadIdx.x;
-  uint32_t pos = offsets[d * nblocks + blockIdx.x];
-  for (uint32_t j = 0; j < cnt; ++j) {
-    if (td[j] == d) {
-      keys_out[pos] = keys_in[base + j];
-      vals_out[pos] = vals_in[base + j];
-      ++pos;
+  {
+    const uint32_t d = threadIdx.x;  // one thread per digit
+    uint32_t run = offsets[d * nblocks + blockIdx.x];
+    for (uint32_t c = 0; c < (uint32_t)kSortChunks; ++c) {
+      start[c * 256 + d] = run;
+      run += cnt[c * 256 + d];
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < kSortItems; ++m) {
+    if (!ok[m]) continue;
+    const uint32_t c = wave + (kSortThreads / 64) * m;
+    const uint32_t pos = start[c * 256 + dig[m]] + rank[m];
+    keys_out[pos] = key[m];
+    vals_out[pos] = val[m];
   }
 }
 
