@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
   __shared__ float4 sSrc[kThreads];
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
   __shared__ uint4 sRec[kThreads];
-  __shared__ uint4 sKeys[kWaves * kChunk];
+  __shared__ uint2 sKeys[kWaves * kChunk];
   __shared__ uint4 sInfo[kWaves * kCacheLevels];
   __shared__ uint32_t sStart[kWaves * kCacheLevels * 64];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];

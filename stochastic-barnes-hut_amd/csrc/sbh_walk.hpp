@@ -513,7 +513,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 // the node's children.  Only the order in which the samples' contributions
 // are summed differs from the per-sample loop.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kChunk = 128;        // samples drawn and sorted together
+constexpr uint32_t kChunk = 256;        // samples drawn and sorted together
 constexpr uint32_t kCacheLevels = 4;    // path levels 1..4 cached
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
@@ -522,7 +522,8 @@ struct SortedLds {
   float4* src;        // 64 staging slots: term sources
   float4* nrm;        // 64 staging slots: normals (winding number only)
   uint4* rec;         // 64 staging slots: children's walk records
-  uint4* keys;        // kChunk samples: {sampled point, sample, level-1 roulette word, 0}
+  uint2* keys;        // kChunk samples: {sampled point, level-1 roulette word with the
+                      // sample's index in the chunk in its low 8 bits}
   uint4* cinfo;       // kCacheLevels: {node, point terms, node terms, 0}
   uint32_t* cstart;   // kCacheLevels x 64: children starts of the cached node
   float* cdiff;       // kCacheLevels x 64: diff of the cached node, per lane
@@ -531,7 +532,7 @@ struct SortedLds {
 
 // Sorts keys[0, n) by .x, n a power of two <= kChunk (bitonic network;
 // every lane takes up to kChunk / 64 compare-exchanges per stage).
-__device__ __forceinline__ void wave_sort_keys(uint4* keys, uint32_t n) {
+__device__ __forceinline__ void wave_sort_keys(uint2* keys, uint32_t n) {
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 2; k <= n; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -540,7 +541,7 @@ __device__ __forceinline__ void wave_sort_keys(uint4* keys, uint32_t n) {
         const uint32_t i = lane + 64 * m;
         const uint32_t ixj = i ^ j;
         if (i < n && ixj > i) {
-          const uint4 a = keys[i], b = keys[ixj];
+          const uint2 a = keys[i], b = keys[ixj];
           const bool asc = (i & k) == 0;
           if ((a.x > b.x) == asc) {
             keys[i] = b;
@@ -669,15 +670,18 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     float sub_total = 0.0f;
     for (int s0 = 0; s0 < spl; s0 += (int)kChunk) {
       const uint32_t ncur = (uint32_t)min((int)kChunk, spl - s0);
-      // draw the chunk's sampled points (Philox block 0 of each sample)
+      // draw the chunk's sampled points (Philox block 0 of each sample); the
+      // level-1 roulette uses only the word's top 24 bits (rr_uniform), so
+      // its low 8 bits carry the sample's index within the chunk
+      static_assert(kChunk <= 256, "the sample index must fit in 8 bits");
 #pragma unroll
       for (uint32_t m = 0; m < kChunk / 64; ++m) {
         const uint32_t i = lane + 64 * m;
-        uint4 key = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+        uint2 key = make_uint2(0xFFFFFFFFu, 0u);
         if (i < ncur) {
           const uint32_t ss = (uint32_t)s0 + i;
           const Philox4 r = philox4x32_10(ss, si, g, 0u, key0, key1);
-          key = make_uint4(sstart + scale_draw(r.x, r.y, nS), ss, r.w, 0u);
+          key = make_uint2(sstart + scale_draw(r.x, r.y, nS), (r.w & 0xFFFFFF00u) | i);
         }
         lds.keys[i] = key;
       }
@@ -687,9 +691,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       while (nsort < ncur) nsort <<= 1;
       wave_sort_keys(lds.keys, nsort);
       for (uint32_t i = 0; i < ncur; ++i) {
-        const uint4 key = lds.keys[i];
-        const uint32_t idx = uniform_u32(key.x), ss = uniform_u32(key.y);
-        const uint32_t w1 = uniform_u32(key.z);
+        const uint2 key = lds.keys[i];
+        const uint32_t idx = uniform_u32(key.x);
+        const uint32_t w1 = uniform_u32(key.y);
+        const uint32_t ss = (uint32_t)s0 + (w1 & 0xFFu);
         // level 0 from the subdomain's cache
         const uint32_t below0 = (uint32_t)__popcll(__ballot(lane < nch0 && c0_start <= idx));
         const uint32_t chosen0 = uniform_u32(f0 + below0 - 1);

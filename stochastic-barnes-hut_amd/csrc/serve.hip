@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
   __shared__ uint4 sRec[kServeThreads];
   __shared__ uint2 sAux[kServeThreads];
   constexpr uint32_t kWaves = kServeThreads / 64;
-  __shared__ uint4 sKeys[kWaves * kChunk];
+  __shared__ uint2 sKeys[kWaves * kChunk];
   __shared__ uint4 sInfo[kWaves * kCacheLevels];
   __shared__ uint32_t sStart[kWaves * kCacheLevels * 64];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];
