@@ -223,9 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
   __shared__ uint4 sRec[kThreads];
   __shared__ uint2 sKeys[kWaves * kChunk];
   __shared__ uint4 sInfo[kWaves * kCacheLevels];
-  __shared__ uint32_t sStart[kWaves * kCacheLevels * 64];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];
-  __shared__ float sRatio[kWaves * kCacheLevels * 64];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t wbase = threadIdx.x & ~63u;
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
@@ -236,8 +234,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
   load_query(qx, qy, qz, qi, D, x, y, z);
   const uint32_t cl = wave * kCacheLevels;
   const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
-                      sKeys + wave * kChunk, sInfo + cl, sStart + cl * 64, sDiff + cl * 64,
-                      sRatio + cl * 64};
+                      sKeys + wave * kChunk, sInfo + cl, sDiff + cl * 64};
   const LaneResult r =
       stochastic_estimate_sorted<D, K>(t, spl, key0, key1, g, live, x, y, z, lds, counts != nullptr);
   if (live) {

@@ -507,14 +507,15 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 // depend only on v and the query, not on the path that reached v.  The
 // samples of a subdomain are therefore drawn a chunk at a time, sorted by
 // their sampled point (so consecutive samples share the upper part of their
-// paths), and diff, ratio and children starts of the most recent node at
-// each of the top kCacheLevels levels are kept in LDS, keyed by node.  A
-// sample whose path passes a cached node reads them instead of evaluating
-// the node's children.  Only the order in which the samples' contributions
+// paths), and diff of the most recent node at each of the top kCacheLevels
+// levels is kept in LDS, keyed by node.  A sample whose path passes a cached
+// node reads it instead of evaluating the node's children (its opening
+// ratio is recomputed and its children's starts read from memory, which
+// keeps the cache small enough for six levels at full occupancy).  Only the order in which the samples' contributions
 // are summed differs from the per-sample loop.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kChunk = 256;        // samples drawn and sorted together
-constexpr uint32_t kCacheLevels = 4;    // path levels 1..4 cached
+constexpr uint32_t kCacheLevels = 6;    // path levels 1..6 cached
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 // This wavefront's LDS for the fast estimator.
@@ -525,9 +526,7 @@ struct SortedLds {
   uint2* keys;        // kChunk samples: {sampled point, level-1 roulette word with the
                       // sample's index in the chunk in its low 8 bits}
   uint4* cinfo;       // kCacheLevels: {node, point terms, node terms, 0}
-  uint32_t* cstart;   // kCacheLevels x 64: children starts of the cached node
   float* cdiff;       // kCacheLevels x 64: diff of the cached node, per lane
-  float* cratio;      // kCacheLevels x 64: its opening ratio, per lane
 };
 
 // Sorts keys[0, n) by .x, n a power of two <= kChunk (bitonic network;
@@ -728,8 +727,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           const uint32_t cslot = min(slot, kCacheLevels - 1) * 64 + lane;  // in-bounds address
           const bool hit = slot < kCacheLevels &&
                            uniform_u32(lds.cinfo[min(slot, kCacheLevels - 1)].x) == cur;
-          const float ffr = hit ? lds.cratio[cslot]
-                                : path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
+          const float ffr = path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
           ++level;
           const float tp = traverse_prob<EXACT>(cur_beta, ffr);
           if (alive) {
@@ -746,7 +744,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           uint32_t below, pt, nd;
           if (hit) {
             diff = lds.cdiff[cslot];
-            const uint32_t cs = lane < nch ? lds.cstart[cslot] : 0xFFFFFFFFu;
+            const uint32_t cs = lane < nch ? n_start[f + lane] : 0xFFFFFFFFu;
             below = (uint32_t)__popcll(__ballot(cs <= idx));
             const uint4 ci = lds.cinfo[slot];
             pt = uniform_u32(ci.y);
@@ -758,8 +756,6 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             below = (uint32_t)__popcll(__ballot(cs <= idx));
             if (slot < kCacheLevels) {
               lds.cdiff[cslot] = diff;
-              lds.cratio[cslot] = ffr;
-              lds.cstart[cslot] = cs;
               if (lane == 0) lds.cinfo[slot] = make_uint4(cur, pt, nd, 0u);
               wave_lds_sync();
             }

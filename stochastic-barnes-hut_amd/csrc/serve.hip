@@ -87,9 +87,7 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
   constexpr uint32_t kWaves = kServeThreads / 64;
   __shared__ uint2 sKeys[kWaves * kChunk];
   __shared__ uint4 sInfo[kWaves * kCacheLevels];
-  __shared__ uint32_t sStart[kWaves * kCacheLevels * 64];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];
-  __shared__ float sRatio[kWaves * kCacheLevels * 64];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   while (true) {
@@ -109,7 +107,7 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
     const SamplerLds c{sSrc + wbase, sNrm + wbase, sRec + wbase, sAux + wbase};
     const uint32_t cl = (threadIdx.x >> 6) * kCacheLevels;
     const SortedLds sc{sSrc + wbase, sNrm + wbase, sRec + wbase, sKeys + (threadIdx.x >> 6) * kChunk,
-                       sInfo + cl, sStart + cl * 64, sDiff + cl * 64, sRatio + cl * 64};
+                       sInfo + cl, sDiff + cl * 64};
     float v;
     switch (variant >> 1) {
       case 0:
