@@ -5,6 +5,7 @@ boundary as the reference's pybind11 layer does).
                     queries 0.5 + 1.2 (U - 0.5)  (the 16384-point CPU case)
   plummer           Plummer sphere, r = (u^(-2/3) - 1)^(-1/2), equal masses 1/N
   gmm2d             8-component 2D Gaussian mixture
+  ring2d            2D winding number: dipoles on a jittered circle
   noisy_icosphere   subdivided icosphere, vertices moved radially by 3-octave
                     noise; sources are area-weighted normal dipoles at triangle
                     centroids (winding number / smooth distance)
@@ -12,6 +13,10 @@ boundary as the reference's pybind11 layer does).
 The reference's own mesh generators (python/source.py:118-155) need igl and
 gpytoolbox, which are not available; these replace them with closed-form
 shapes of the same role.
+
+save_case / load_case store a workload and any expected results as one
+.npz file (plain arrays, no pickled objects): the on-disk fixture format of
+the golden vectors under tests/golden/.
 """
 from __future__ import annotations
 
@@ -120,6 +125,22 @@ def _value_noise(p: np.ndarray, freq: float, rng) -> np.ndarray:
     return out
 
 
+def ring2d(n: int, q: int, seed: int = 0, radius: float = 0.4, jitter: float = 0.05):
+    """2D winding-number sources: dipoles on a jittered circle (the closed
+    curve python/sampler.py:5-20 samples), outward normals, masses = segment
+    lengths; queries uniform in the square [-1.5 r, 1.5 r]^2."""
+    rng = np.random.default_rng(seed)
+    ang = np.sort(rng.uniform(0.0, 2.0 * np.pi, n))
+    r = radius * (1.0 + jitter * rng.normal(size=n))
+    pts = r[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+    e = np.roll(pts, -1, axis=0) - pts
+    nrm = np.stack([e[:, 1], -e[:, 0]], 1)
+    length = np.linalg.norm(nrm, axis=1)
+    nrm /= length[:, None]
+    queries = rng.uniform(-1.5 * radius, 1.5 * radius, (q, 2))
+    return pts, nrm, length, queries
+
+
 def noisy_icosphere(level: int, q: int, amplitude: float = 0.1, seed: int = 4,
                     expand: float = 0.1):
     """Dipole sources of a closed noisy sphere; returns (points, normals, masses,
@@ -140,3 +161,37 @@ def noisy_icosphere(level: int, q: int, amplitude: float = 0.1, seed: int = 4,
     qlo, qhi = lo - expand * 0.5 * ext, hi + expand * 0.5 * ext
     queries = qlo + (qhi - qlo) * rng.random((q, 3))
     return centroids, normals, masses, queries
+
+
+CASE_FORMAT = 1
+CASE_INPUTS = ("points", "normals", "masses", "min_corner", "max_corner", "queries")
+
+
+def save_case(path: str, points, normals, masses, min_corner, max_corner, queries,
+              alpha: float = 1.0, split_log: int = 1, leaf_size: int = 1, **results) -> None:
+    """Writes a workload (inputs as float32, the precision the plugins
+    compute in) and named result arrays to one compressed .npz file."""
+    arrays = dict(points=points, normals=normals, masses=masses, min_corner=min_corner,
+                  max_corner=max_corner, queries=queries)
+    out = {k: np.ascontiguousarray(np.asarray(v, np.float32)) for k, v in arrays.items()}
+    out["meta"] = np.array([CASE_FORMAT, split_log, leaf_size], np.int64)
+    out["alpha"] = np.array([alpha], np.float32)
+    for k, v in results.items():
+        if k in out:
+            raise ValueError(f"result name {k!r} collides with an input")
+        out["result_" + k] = np.asarray(v)
+    np.savez_compressed(path, **out)
+
+
+def load_case(path: str) -> dict:
+    """Reads a file written by save_case: the inputs, alpha, split_log,
+    leaf_size and a dict of results.  Loads without unpickling."""
+    with np.load(path, allow_pickle=False) as z:
+        meta = z["meta"]
+        if int(meta[0]) != CASE_FORMAT:
+            raise ValueError(f"{path}: unsupported case format {int(meta[0])}")
+        case = {k: z[k] for k in CASE_INPUTS}
+        case["alpha"] = float(z["alpha"][0])
+        case["split_log"], case["leaf_size"] = int(meta[1]), int(meta[2])
+        case["results"] = {k[len("result_"):]: z[k] for k in z.files if k.startswith("result_")}
+    return case
