@@ -42,6 +42,15 @@ typedef struct sbh_tree sbh_tree;
 #define SBH_GRAVITY 0
 #define SBH_WINDING_NUMBER 1
 #define SBH_SMOOTH_DIST 2
+/* Kernels the reference defines but binds to no plugin (include/kernel.h:61-83):
+ * helmholtzKernelX / Y, and gravityGrad as one scalar kernel per component
+ * (-m / (d + 0.1)^3 * (q - p)_axis; Z in 3D only).  Evaluated by every method;
+ * not by the serving engine or the matrix-core brute force. */
+#define SBH_HELMHOLTZ_X 3
+#define SBH_HELMHOLTZ_Y 4
+#define SBH_GRAVITY_GRAD_X 5
+#define SBH_GRAVITY_GRAD_Y 6
+#define SBH_GRAVITY_GRAD_Z 7
 
 /* Evaluation methods. */
 #define SBH_BRUTE_FORCE 0 /* src/sbh_gpu.cu:212-322  evalBruteForce_*            */
@@ -78,6 +87,11 @@ typedef struct sbh_tree sbh_tree;
                                       h.h + h.l + l.h + l.l accumulated in fp32
                                       (V_MFMA_F32_16X16X16_BF16); pairs closer than 4x
                                       the wavefront radius recomputed directly */
+#define SBH_EVAL_SMOOTH_DISTANCE 128 /* smooth distance: results are -log(w) / alpha,
+                                      the distance the reference's driver derives
+                                      from the sum w (python/example.py:79); a
+                                      non-positive stochastic estimate gives NaN or
+                                      +inf as the driver's numpy transform would */
 #define SBH_EVAL_PER_QUERY_PATHS 32 /* stochastic: one sample path per query keyed by the
                                       query's index, as the reference CPU sampler draws
                                       them (include/tree_sampler.h:48-104); default: the

@@ -23,7 +23,9 @@ _LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 _lib = None
 
 GRAVITY, WINDING_NUMBER, SMOOTH_DIST = 0, 1, 2  # include/kernel.h:24-28
-KERNELS = {"gravity": GRAVITY, "wn": WINDING_NUMBER, "smoothdist": SMOOTH_DIST}
+KERNELS = {"gravity": GRAVITY, "wn": WINDING_NUMBER, "smoothdist": SMOOTH_DIST,
+           "helmholtz_x": 3, "helmholtz_y": 4, "gravity_grad_x": 5, "gravity_grad_y": 6,
+           "gravity_grad_z": 7}
 
 
 def build() -> str:

@@ -150,6 +150,34 @@ float k_smooth(const Src<D>& s, const float* q) {  // kernel.h:55-59
   return std::exp(-s.alpha * dist);
 }
 
+// helmholtzKernelX / helmholtzKernelY (kernel.h:68-83): m cos(freq 2 pi d) /
+// (d + 0.1) with freq 2, and m sin(freq 2 pi d) / (d + 0.1) with freq 1; the
+// argument is ((freq * 2) * pi) * dist in fp32, cosf / sinf.
+template <int D, int Y>
+float k_helm(const Src<D>& s, const float* q) {
+  float d[D];
+  for (int a = 0; a < D; ++a) d[a] = s.pos[a] - q[a];
+  float dist = std::sqrt(sqnorm<D>(d));
+  const float freq = Y ? 1.0f : 2.0f;
+  float arg = freq * 2 * kPi * dist;
+  float c = Y ? std::sin(arg) : std::cos(arg);
+  return s.mass * c / (dist + 1e-1f);
+}
+
+// Component `A` of gravityGrad (kernel.h:61-66): -m / (d + 0.1)^3 * (q - p).
+// The reference's template is never instantiated (no plugin binds it, and as
+// written it would mix a double pow with a float vector); the fp32 form here
+// is t = d + 0.1, s = -m / ((t * t) * t), s * (q_a - p_a).
+template <int D, int A>
+float k_grad(const Src<D>& s, const float* q) {
+  float d[D];
+  for (int a = 0; a < D; ++a) d[a] = q[a] - s.pos[a];
+  float dist = std::sqrt(sqnorm<D>(d));
+  float t = dist + 1e-1f;
+  float sc = -s.mass / (t * t * t);
+  return sc * d[A];
+}
+
 // float64 restatement of the same formulas (constants are the fp32 literals
 // promoted, so this is the exact value of the fp32 formula's mathematics).
 template <int D>
@@ -169,6 +197,18 @@ double k64(int kernel, const Src<D>& s, const float* q) {
     case 1:
       if (D == 2) return (double)s.mass * dn / (2.0 * pi * (dd + (double)1e-5f));
       return (double)s.mass * dn / (4.0 * pi * (dist * dd + (double)1e-5f));
+    case 3:
+      return (double)s.mass * std::cos(4.0 * pi * dist) / (dist + (double)1e-1f);
+    case 4:
+      return (double)s.mass * std::sin(2.0 * pi * dist) / (dist + (double)1e-1f);
+    case 5:
+    case 6:
+    case 7: {
+      const int a = kernel - 5;
+      if (a >= D) return 0.0;
+      const double t = dist + (double)1e-1f;
+      return -(double)s.mass / (t * t * t) * -d[a];  // (q - p) = -(p - q)
+    }
     default:
       return std::exp(-(double)s.alpha * dist);
   }
@@ -183,6 +223,17 @@ KernelFn<D> pick_kernel(int kernel) {
       return k_wn<D>;
     case 2:
       return k_smooth<D>;
+    case 3:
+      return k_helm<D, 0>;
+    case 4:
+      return k_helm<D, 1>;
+    case 5:
+      return k_grad<D, 0>;
+    case 6:
+      return k_grad<D, 1>;
+    case 7:
+      if constexpr (D == 3) return k_grad<D, 2>;
+      return nullptr;
   }
   return nullptr;
 }

@@ -357,8 +357,9 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     SBH_REQUIRE(tree && params, "NULL argument");
     SBH_REQUIRE(method == SBH_BRUTE_FORCE || method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
                 "unknown method");
-    SBH_REQUIRE(kernel == SBH_GRAVITY || kernel == SBH_WINDING_NUMBER || kernel == SBH_SMOOTH_DIST,
-                "unknown kernel");
+    SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
+    SBH_REQUIRE(kernel != SBH_GRAVITY_GRAD_Z || tree->t.dim == 3,
+                "the gradient's z component needs a 3D tree");
     SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
     if (kernel_ms) *kernel_ms = 0.0f;
     if (q == 0) return;
@@ -366,8 +367,11 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     const sbh_eval_params& p = *params;
     SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
                              SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
-                             SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16)) == 0,
+                             SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16 |
+                             SBH_EVAL_SMOOTH_DISTANCE)) == 0,
                 "unknown eval flags");
+    SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
+                "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
     if (method == SBH_BARNES_HUT) {
       SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
       SBH_REQUIRE(p.vote_group == 1 || p.vote_group == 32 || p.vote_group == 64,
@@ -445,6 +449,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
                             split, exact, s);
       SBH_HIP(hipEventRecord(tree->ev1, s));
     }
+    if (p.flags & SBH_EVAL_SMOOTH_DISTANCE) sbh::run_smooth_distance(dout, nq, t.alpha, s);
     if (!dev) {
       SBH_HIP(hipMemcpyAsync(results, dout, nq * 4, hipMemcpyDeviceToHost, s));
       if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, ncnt * 4, hipMemcpyDeviceToHost, s));
@@ -459,8 +464,9 @@ int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q
                  double* results, int32_t flags, float* kernel_ms) {
   return guarded([&] {
     SBH_REQUIRE(tree, "tree is NULL");
-    SBH_REQUIRE(kernel == SBH_GRAVITY || kernel == SBH_WINDING_NUMBER || kernel == SBH_SMOOTH_DIST,
-                "unknown kernel");
+    SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
+    SBH_REQUIRE(kernel != SBH_GRAVITY_GRAD_Z || tree->t.dim == 3,
+                "the gradient's z component needs a 3D tree");
     SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
     if (kernel_ms) *kernel_ms = 0.0f;
     if (q == 0) return;

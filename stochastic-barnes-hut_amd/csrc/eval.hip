@@ -483,6 +483,24 @@ void dispatch_kernel(int kernel, auto&& fn) {
     case SMOOTH_DIST:
       fn.template operator()<D, SMOOTH_DIST>();
       break;
+    case HELMHOLTZ_X:
+      fn.template operator()<D, HELMHOLTZ_X>();
+      break;
+    case HELMHOLTZ_Y:
+      fn.template operator()<D, HELMHOLTZ_Y>();
+      break;
+    case GRAVITY_GRAD_X:
+      fn.template operator()<D, GRAVITY_GRAD_X>();
+      break;
+    case GRAVITY_GRAD_Y:
+      fn.template operator()<D, GRAVITY_GRAD_Y>();
+      break;
+    case GRAVITY_GRAD_Z:
+      if constexpr (D == 3) {
+        fn.template operator()<D, GRAVITY_GRAD_Z>();
+        break;
+      }
+      [[fallthrough]];
     default:
       throw Error(kErrInvalid, "unknown kernel id");
   }
@@ -557,6 +575,19 @@ void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, boo
       launch_stoch<D, K, false>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
                                 split, s);
   });
+}
+
+namespace {
+// -log(w) / alpha in place (python/example.py:79).
+__global__ void k_smooth_distance(float* w, uint32_t n, float alpha) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = -logf(w[i]) / alpha;
+}
+}  // namespace
+
+void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s) {
+  k_smooth_distance<<<nblk(n, 256), 256, 0, s>>>(w, n, alpha);
+  SBH_HIP(hipGetLastError());
 }
 
 size_t sort_scratch_u32(uint32_t nq) {

@@ -5,10 +5,14 @@
 //   eval_brute_force_{gravity,wn,smoothdist}           src/sbh_gpu.cu:584-604
 //   eval_barnes_hut_{gravity,wn,smoothdist}            src/sbh_gpu.cu:606-623
 //   multi_level_prefix_control_variate_{...}           src/sbh_gpu.cu:625-660
+// and the kernels the reference defines without binding them
+// (include/kernel.h:61-83): *_helmholtz_x / _y, *_gravity_grad_x / _y / _z,
+// and *_gravity_grad returning (q, dim) gradients.
 // Each evaluation returns (float32[q], kernel milliseconds), as the reference.
 // Keyword extensions keep the reference defaults: vote_group (32, the
 // reference's warp vote), seed (0), sort_queries (False), return_counts,
 // per_query_paths (False: the 64 queries of a wavefront share sample paths),
+// smooth_distance (False: return the sum w; True: -log(w) / alpha),
 // exact_terms (False: hardware sqrt / reciprocal / exp2 as the reference's
 // fast-math GPU build; True: correctly rounded, bit-reproducible terms).
 // The GIL is released around device work.
@@ -272,41 +276,45 @@ py::object run_eval(GPUTreeBase& t, int method, int kernel, F32Array queries, fl
 void bind_kernel(py::module_& m, const char* suffix, int kernel) {
   std::string s = suffix;
   m.def(("eval_brute_force_" + s).c_str(),
-        [kernel](GPUTreeBase& t, F32Array q, py::object mfma) {
+        [kernel](GPUTreeBase& t, F32Array q, py::object mfma, bool smooth_distance) {
           // mfma: False, True / "f32" (fp32 operands) or "bf16" (split bf16 operands)
-          int flags = 0;
+          int flags = smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0;
           if (py::isinstance<py::str>(mfma)) {
             const std::string m = mfma.cast<std::string>();
             if (m == "bf16")
-              flags = SBH_EVAL_MFMA_BF16;
+              flags |= SBH_EVAL_MFMA_BF16;
             else if (m == "f32")
-              flags = SBH_EVAL_MFMA;
+              flags |= SBH_EVAL_MFMA;
             else
               throw py::value_error("mfma must be False, True, 'f32' or 'bf16'");
           } else if (mfma.cast<bool>()) {
-            flags = SBH_EVAL_MFMA;
+            flags |= SBH_EVAL_MFMA;
           }
           return run_eval(t, SBH_BRUTE_FORCE, kernel, q, 0.f, 1, 0, 1, false, false, false,
                           flags);
         },
-        "tree"_a, "queries"_a, "mfma"_a = false);
+        "tree"_a, "queries"_a, "mfma"_a = false, "smooth_distance"_a = false);
   m.def(("eval_barnes_hut_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, float beta, int vote_group, bool sort_queries,
-                 bool return_counts, bool exact_terms) {
+                 bool return_counts, bool exact_terms, bool smooth_distance) {
           return run_eval(t, SBH_BARNES_HUT, kernel, q, beta, 1, 0, vote_group, sort_queries,
-                          return_counts, exact_terms);
+                          return_counts, exact_terms,
+                          smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0);
         },
         "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "sort_queries"_a = false,
-        "return_counts"_a = false, "exact_terms"_a = false);
+        "return_counts"_a = false, "exact_terms"_a = false, "smooth_distance"_a = false);
   m.def(("multi_level_prefix_control_variate_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, int spl, uint64_t seed, bool sort_queries,
-                 bool return_counts, bool exact_terms, bool per_query_paths) {
+                 bool return_counts, bool exact_terms, bool per_query_paths,
+                 bool smooth_distance) {
           return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.f, spl, seed, 1, sort_queries,
                           return_counts, exact_terms,
-                          per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0);
+                          (per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0) |
+                              (smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0));
         },
         "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
-        "return_counts"_a = false, "exact_terms"_a = false, "per_query_paths"_a = false);
+        "return_counts"_a = false, "exact_terms"_a = false, "per_query_paths"_a = false,
+        "smooth_distance"_a = false);
   m.def(("eval_brute_force_f64_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array queries) {
           std::vector<float> q = t.soa(queries);
@@ -323,6 +331,50 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
           return py::make_tuple(out, ms);
         },
         "tree"_a, "queries"_a);
+}
+
+// gravityGrad (include/kernel.h:61-66) as (q, dim) arrays: one evaluation
+// per component kernel with the same parameters, so the stochastic components
+// share their sample paths.
+py::tuple gradient(GPUTreeBase& t, int method, F32Array q, float beta, int spl, uint64_t seed,
+                   int vote_group, bool sort, bool exact, int extra) {
+  const int64_t nq = q.ndim() == 2 ? q.shape(0) : 0;
+  py::array_t<float> out({nq, (int64_t)t.dim});
+  float total_ms = 0.0f;
+  for (int a = 0; a < t.dim; ++a) {
+    py::tuple r = run_eval(t, method, SBH_GRAVITY_GRAD_X + a, q, beta, spl, seed, vote_group,
+                           sort, false, exact, extra);
+    auto col = r[0].cast<py::array_t<float>>();
+    const float* c = col.data();
+    float* o = out.mutable_data();
+    for (int64_t i = 0; i < nq; ++i) o[i * t.dim + a] = c[i];
+    total_ms += r[1].cast<float>();
+  }
+  return py::make_tuple(out, total_ms);
+}
+
+void bind_gradient(py::module_& m) {
+  m.def("eval_brute_force_gravity_grad",
+        [](GPUTreeBase& t, F32Array q) {
+          return gradient(t, SBH_BRUTE_FORCE, q, 0.f, 1, 0, 1, false, false, 0);
+        },
+        "tree"_a, "queries"_a);
+  m.def("eval_barnes_hut_gravity_grad",
+        [](GPUTreeBase& t, F32Array q, float beta, int vote_group, bool sort_queries,
+           bool exact_terms) {
+          return gradient(t, SBH_BARNES_HUT, q, beta, 1, 0, vote_group, sort_queries,
+                          exact_terms, 0);
+        },
+        "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "sort_queries"_a = false,
+        "exact_terms"_a = false);
+  m.def("multi_level_prefix_control_variate_gravity_grad",
+        [](GPUTreeBase& t, F32Array q, int spl, uint64_t seed, bool sort_queries,
+           bool exact_terms, bool per_query_paths) {
+          return gradient(t, SBH_STOCHASTIC, q, 0.f, spl, seed, 1, sort_queries, exact_terms,
+                          per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0);
+        },
+        "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
+        "exact_terms"_a = false, "per_query_paths"_a = false);
 }
 
 // Serving (sbh_server_* in include/sbh_hip.h).  Query and result buffers of a
@@ -471,6 +523,12 @@ PYBIND11_MODULE(sbh_gpu, m) {
   bind_kernel(m, "gravity", SBH_GRAVITY);
   bind_kernel(m, "wn", SBH_WINDING_NUMBER);
   bind_kernel(m, "smoothdist", SBH_SMOOTH_DIST);
+  bind_kernel(m, "helmholtz_x", SBH_HELMHOLTZ_X);
+  bind_kernel(m, "helmholtz_y", SBH_HELMHOLTZ_Y);
+  bind_kernel(m, "gravity_grad_x", SBH_GRAVITY_GRAD_X);
+  bind_kernel(m, "gravity_grad_y", SBH_GRAVITY_GRAD_Y);
+  bind_kernel(m, "gravity_grad_z", SBH_GRAVITY_GRAD_Z);
+  bind_gradient(m);
   py::class_<PyRequest>(m, "Request")
       .def_readonly("id", &PyRequest::id)
       .def("state", &PyRequest::state)

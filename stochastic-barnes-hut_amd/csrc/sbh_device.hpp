@@ -12,7 +12,20 @@ constexpr int kWave = 64;
 constexpr uint32_t kEnd = 0xFFFFFFFFu;   // rope terminator
 constexpr uint32_t kLeafBit = 0x80000000u;
 
-enum Kernel : int { GRAVITY = 0, WINDING_NUMBER = 1, SMOOTH_DIST = 2 };  // include/kernel.h:24-28
+// include/kernel.h:24-28 (bound kernels) and :61-83 (unbound: Helmholtz,
+// and the gravity gradient as one scalar kernel per axis)
+enum Kernel : int {
+  GRAVITY = 0,
+  WINDING_NUMBER = 1,
+  SMOOTH_DIST = 2,
+  HELMHOLTZ_X = 3,
+  HELMHOLTZ_Y = 4,
+  GRAVITY_GRAD_X = 5,
+  GRAVITY_GRAD_Y = 6,
+  GRAVITY_GRAD_Z = 7,
+};
+constexpr bool is_helmholtz(int k) { return k == HELMHOLTZ_X || k == HELMHOLTZ_Y; }
+constexpr bool is_gradient(int k) { return k >= GRAVITY_GRAD_X && k <= GRAVITY_GRAD_Z; }
 
 // Constants of include/kernel.h:31-59 in the reference's float evaluation:
 // 4 * (float)M_PI and 2 * (float)M_PI are exact power-of-two multiples.
@@ -57,6 +70,20 @@ __device__ __forceinline__ float kernel_exact(float4 src, float4 nrm, float alph
       float qdn = nrm.x * dx + nrm.y * dy;
       return src.w * qdn / (kTwoPi * (dsq + 1e-5f));
     }
+  } else if constexpr (is_helmholtz(K)) {
+    // m cos(freq 2 pi d) / (d + 0.1), freq 2 (X, cos) or 1 (Y, sin): kernel.h:68-83
+    float dist = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
+    const float freq = (K == HELMHOLTZ_X) ? 2.0f : 1.0f;
+    float arg = freq * 2.0f * kPiF * dist;
+    float c = (K == HELMHOLTZ_X) ? cosf(arg) : sinf(arg);
+    return src.w * c / (dist + 1e-1f);
+  } else if constexpr (is_gradient(K)) {
+    // -m / (d + 0.1)^3 * (q - p), component K - GRAVITY_GRAD_X: kernel.h:61-66
+    float dist = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
+    float t = dist + 1e-1f;
+    float s = -src.w / (t * t * t);
+    float comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
+    return s * comp;
   } else {
     float dist = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
     return expf(-alpha * dist);
@@ -120,6 +147,17 @@ __device__ __forceinline__ float kernel_fast(float4 src, float4 nrm, float alpha
       float qdn = __builtin_fmaf(nrm.x, dx, nrm.y * dy);
       return src.w * qdn * __builtin_amdgcn_rcpf(kTwoPi * (dsq + 1e-5f));
     }
+  } else if constexpr (is_helmholtz(K)) {
+    // the hardware sine / cosine take revolutions: cos(2 pi freq d) = cos_rev(freq d)
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    float rev = __builtin_amdgcn_fractf((K == HELMHOLTZ_X ? 2.0f : 1.0f) * dist);
+    float c = (K == HELMHOLTZ_X) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+    return src.w * c * __builtin_amdgcn_rcpf(dist + 1e-1f);
+  } else if constexpr (is_gradient(K)) {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    float r = __builtin_amdgcn_rcpf(dist + 1e-1f);
+    float comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
+    return -src.w * (r * r * r) * comp;
   } else {
     float dist = __builtin_amdgcn_sqrtf(dsq);
     // exp(-a d) = 2^(-a d log2 e); alpha is pre-scaled by log2(e) by the caller.
@@ -150,6 +188,16 @@ __device__ __forceinline__ float kernel_fast_acc(float acc, float4 src, float4 n
       float qdn = __builtin_fmaf(nrm.x, dx, nrm.y * dy);
       return __builtin_fmaf(src.w * qdn, __builtin_amdgcn_rcpf(kTwoPi * (dsq + 1e-5f)), acc);
     }
+  } else if constexpr (is_helmholtz(K)) {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    float rev = __builtin_amdgcn_fractf((K == HELMHOLTZ_X ? 2.0f : 1.0f) * dist);
+    float c = (K == HELMHOLTZ_X) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+    return __builtin_fmaf(src.w * c, __builtin_amdgcn_rcpf(dist + 1e-1f), acc);
+  } else if constexpr (is_gradient(K)) {
+    float dist = __builtin_amdgcn_sqrtf(dsq);
+    float r = __builtin_amdgcn_rcpf(dist + 1e-1f);
+    float comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
+    return __builtin_fmaf(-src.w * (r * r * r), comp, acc);
   } else {
     float dist = __builtin_amdgcn_sqrtf(dsq);
     return acc + __builtin_amdgcn_exp2f(-alpha * dist);
@@ -186,6 +234,14 @@ __device__ __forceinline__ double kernel_f64(float4 src, float4 nrm, float alpha
       return (double)src.w * qdn / ((double)kFourPi * (dist * dsq + (double)1e-5f));
     else
       return (double)src.w * qdn / ((double)kTwoPi * (dsq + (double)1e-5f));
+  } else if constexpr (is_helmholtz(K)) {
+    const double pi = (double)kPiF;
+    const double c = (K == HELMHOLTZ_X) ? cos(4.0 * pi * dist) : sin(2.0 * pi * dist);
+    return (double)src.w * c / (dist + (double)1e-1f);
+  } else if constexpr (is_gradient(K)) {
+    const double t = dist + (double)1e-1f;
+    const double comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
+    return -(double)src.w / (t * t * t) * comp;
   } else {
     return exp(-(double)alpha * dist);
   }

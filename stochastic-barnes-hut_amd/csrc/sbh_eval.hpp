@@ -41,6 +41,9 @@ void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, boo
                     const float* q, uint32_t nq, const uint32_t* order, float* out,
                     int32_t* counts, int split, bool exact, hipStream_t s);
 
+// Smooth distance from its sum, in place: w <- -log(w) / alpha.
+void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s);
+
 // Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)
 // uint32 values.
 size_t sort_scratch_u32(uint32_t nq);

@@ -264,7 +264,8 @@ int64_t Server::submit(int32_t tree_id, int32_t method, int32_t kernel, const sb
   SBH_REQUIRE(tree_id >= 0 && tree_id < (int32_t)trees_.size(), "unknown tree id");
   SBH_REQUIRE(method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
               "serving runs Barnes-Hut or stochastic requests");
-  SBH_REQUIRE(kernel >= 0 && kernel <= 2, "unknown kernel");
+  SBH_REQUIRE(kernel >= 0 && kernel <= 2,
+              "serving evaluates the gravity, winding-number and smooth-distance kernels");
   SBH_REQUIRE(q >= 1 && q < (int64_t)0x7FFFFFFF, "query count out of range");
   SBH_REQUIRE(queries && results, "NULL queries or results");
   if (method == SBH_BARNES_HUT) SBH_REQUIRE(p.beta == p.beta, "beta is NaN");

@@ -17,11 +17,16 @@ LIB_PATH = os.path.join(_HERE, "libsbh_hip.so")
 
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_NODEVICE, ERR_BUDGET, ERR_INTERNAL = range(7)
 GRAVITY, WINDING_NUMBER, SMOOTH_DIST = 0, 1, 2
-KERNELS = {"gravity": GRAVITY, "wn": WINDING_NUMBER, "smoothdist": SMOOTH_DIST}
+HELMHOLTZ_X, HELMHOLTZ_Y, GRAVITY_GRAD_X, GRAVITY_GRAD_Y, GRAVITY_GRAD_Z = 3, 4, 5, 6, 7
+KERNELS = {"gravity": GRAVITY, "wn": WINDING_NUMBER, "smoothdist": SMOOTH_DIST,
+           "helmholtz_x": HELMHOLTZ_X, "helmholtz_y": HELMHOLTZ_Y,
+           "gravity_grad_x": GRAVITY_GRAD_X, "gravity_grad_y": GRAVITY_GRAD_Y,
+           "gravity_grad_z": GRAVITY_GRAD_Z}
 BRUTE_FORCE, BARNES_HUT, STOCHASTIC = 0, 1, 2
 TREE_MOMENTS_F64 = 1
 EVAL_DEVICE_POINTERS, EVAL_SORT_QUERIES, EVAL_SPLIT_COUNTS = 1, 2, 4
 EVAL_EXACT_TERMS, EVAL_MFMA, EVAL_PER_QUERY_PATHS, EVAL_MFMA_BF16 = 8, 16, 32, 64
+EVAL_SMOOTH_DISTANCE = 128
 
 
 class EvalParams(ctypes.Structure):

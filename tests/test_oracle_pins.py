@@ -24,8 +24,16 @@ PI32 = np.float64(np.float32(np.pi))
 def closed_form(kernel, dim, p, n, m, alpha, q):
     d = p - q
     r = np.sqrt((d * d).sum(1))
+    tenth = np.float64(np.float32(0.1))
     if kernel == "gravity":
-        return -m / (r + np.float64(np.float32(0.1)))
+        return -m / (r + tenth)
+    if kernel == "helmholtz_x":  # include/kernel.h:68-74
+        return m * np.cos(4 * PI32 * r) / (r + tenth)
+    if kernel == "helmholtz_y":  # include/kernel.h:77-83
+        return m * np.sin(2 * PI32 * r) / (r + tenth)
+    if kernel.startswith("gravity_grad_"):  # include/kernel.h:61-66, (q - p) component
+        a = "xyz".index(kernel[-1])
+        return -m / (r + tenth) ** 3 * (q[:, a] - p[:, a])
     if kernel == "wn":
         dn = (n * d).sum(1)
         if dim == 2:
@@ -49,6 +57,38 @@ def test_kernel_closed_forms(kernel, dim):
     np.testing.assert_allclose(o64, ref, rtol=1e-13, atol=1e-300)
     den = np.maximum(np.abs(ref), 1e-3) if kernel == "wn" else np.abs(ref)
     assert (np.abs(o32 - ref) / den).max() < 2e-6
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kernel", ["helmholtz_x", "helmholtz_y", "gravity_grad_x",
+                                    "gravity_grad_y", "gravity_grad_z"])
+def test_unbound_kernel_closed_forms(kernel, dim):
+    """The kernels the reference defines but binds to no plugin
+    (include/kernel.h:61-83), against float64 closed forms; fp32 error
+    relative to the term's magnitude scale (cosine and the (q - p) factor
+    have zeros)."""
+    if kernel == "gravity_grad_z" and dim == 2:
+        pytest.skip("no z component in 2D")
+    rng = np.random.default_rng(2)
+    n = 500
+    p = rng.normal(size=(n, dim)).astype(np.float32).astype(np.float64)
+    nv = rng.normal(size=(n, dim)).astype(np.float32).astype(np.float64)
+    m = rng.uniform(0.5, 2, n).astype(np.float32).astype(np.float64)
+    q = rng.normal(size=(n, dim)).astype(np.float32).astype(np.float64)
+    o32, o64 = oracle.kernel_pairs(dim, kernel, p, nv, m, 1.0, q)
+    ref = closed_form(kernel, dim, p, nv, m, 1.0, q)
+    np.testing.assert_allclose(o64, ref, rtol=1e-12, atol=1e-14)
+    r = np.sqrt(((p - q) ** 2).sum(1))
+    scale = m / (r + 0.1) if kernel.startswith("helm") else m * r / (r + 0.1) ** 3
+    assert (np.abs(o32 - ref) / scale).max() < 1e-5
+    # the gradient components are the components of one vector: -m (q - p) / (d + 0.1)^3
+    if kernel == "gravity_grad_x":
+        comps = [oracle.kernel_pairs(dim, "gravity_grad_" + "xyz"[a], p, nv, m, 1.0, q)[1]
+                 for a in range(dim)]
+        g = np.stack(comps, 1)
+        np.testing.assert_allclose(np.linalg.norm(g, axis=1), m * r / (r + 0.1) ** 3,
+                                   rtol=1e-6)
+        assert (np.einsum("ij,ij->i", g, q - p) <= 0).all()  # points from q towards p
 
 
 def test_kernel_signs_and_special_values():
