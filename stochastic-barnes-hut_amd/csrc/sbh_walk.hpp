@@ -120,14 +120,14 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   bool active = live;
   uint32_t resume = kEnd;
   uint32_t v = 0;
+  float4 c = nodeC[0];
+  uint4 l = nodeL[0];
   while (v != kEnd) {
     v = uniform_u32(v);
     if (!active && live && resume == v) active = true;
-    const float4 c = nodeC[v];
-    const uint4 l = nodeL[v];
     const bool leaf = (l.w & kLeafBit) != 0;
-    // every field of the record is consumed here, so its scalar loads issue
-    // together at the top of the step
+    // the record (c, l) of v was loaded at the end of the previous step, as
+    // soon as v was known
     const float diam = __uint_as_float(l.x);
     const float inv_diam = __builtin_amdgcn_rcpf(diam);
     bool reject = false;
@@ -136,24 +136,36 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
     const bool accept = active && ((rej & my_group) == 0);
     if (accept) {
       if (leaf) {
-        const uint32_t p0 = l.y, p1 = l.y + (l.w & ~kLeafBit);
-        for (uint32_t j = p0; j < p1; ++j)
-          acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
-        cnt += (int32_t)(p1 - p0);
+        const uint32_t np = l.w & ~kLeafBit;
+        if (np == 1) {
+          // a one-point leaf's record holds the point
+          acc = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
+        } else {
+          for (uint32_t j = l.y; j < l.y + np; ++j)
+            acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+        }
+        cnt += (int32_t)np;
       } else {
         acc = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
         cnt_nodes += 1;
       }
     }
+    uint32_t next;
     if (rej == 0) {
-      v = l.z;
+      next = l.z;
     } else {
       if (accept) {
         active = false;
         resume = l.z;
       }
-      v = l.y;
+      next = l.y;
     }
+    next = uniform_u32(next);
+    if (next != kEnd) {
+      c = nodeC[next];
+      l = nodeL[next];
+    }
+    v = next;
   }
   return {acc, cnt, cnt_nodes};
 }

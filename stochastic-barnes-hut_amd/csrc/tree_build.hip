@@ -564,18 +564,23 @@ __global__ void k_dfs_positions(uint32_t lb, uint32_t count, const uint32_t* n_f
   }
 }
 
+// The walk never reads a leaf's moment (leaves are summed point by point), so
+// a one-point leaf's depth-first record holds the point itself and the walk
+// sums it without a further load.
 __global__ void k_dfs_scatter(uint32_t nn, const uint32_t* n_dfs, const uint32_t* n_size,
                               const float4* nodeC, const float4* nodeN, const uint4* nodeL,
-                              float4* dfsC, float4* dfsN, uint4* dfsL) {
+                              const float4* pts, const float4* nrm, float4* dfsC, float4* dfsN,
+                              uint4* dfsL) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   uint32_t d = n_dfs[v];
   uint4 l = nodeL[v];
+  const bool one_point = l.w == (kLeafBit | 1u);
   if (!(l.w & kLeafBit)) l.y = d + 1;
   uint32_t rope = d + n_size[v];
   l.z = rope < nn ? rope : kEnd;
-  dfsC[d] = nodeC[v];
-  dfsN[d] = nodeN[v];
+  dfsC[d] = one_point ? pts[l.y] : nodeC[v];
+  dfsN[d] = one_point ? nrm[l.y] : nodeN[v];
   dfsL[d] = l;
 }
 
@@ -878,7 +883,7 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
   }
   k_dfs_scatter<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
       num_nodes, t.n_dfs.get(), t.n_size.get(), t.nodeC.get(), t.nodeN.get(), t.nodeL.get(),
-      t.dfsC.get(), t.dfsN.get(), t.dfsL.get());
+      t.pts.get(), t.nrm.get(), t.dfsC.get(), t.dfsN.get(), t.dfsL.get());
   SBH_HIP(hipGetLastError());
   SBH_HIP(hipStreamSynchronize(s));
 }

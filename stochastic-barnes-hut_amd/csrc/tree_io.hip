@@ -3,7 +3,7 @@
 // a serving process and for checkpoints.  The reference has no counterpart
 // (its trees live only in the Python objects that built them).
 //
-// Format (little-endian), version 1:
+// Format (little-endian), version 2:
 //   char[8]   magic "SBHTREE\0"
 //   u32       version
 //   i32 x5    dim, split_log, leaf_size, max_depth, moments64
@@ -17,6 +17,8 @@
 //   u64       FNV-1a 64 of every byte before it
 // Array lengths follow from n and num_nodes; the loader checks every count,
 // the structure sizes and the checksum before it allocates device memory.
+// Version 2: the depth-first record (dfsC, dfsN) of a one-point leaf holds
+// that point and its normal, not the leaf's moment.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -29,7 +31,7 @@ namespace sbh {
 namespace {
 
 constexpr char kMagic[8] = {'S', 'B', 'H', 'T', 'R', 'E', 'E', '\0'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
 constexpr uint32_t kEndHost = 0xFFFFFFFFu;      // rope terminator
 constexpr uint32_t kLeafBitHost = 0x80000000u;  // leaf flag in a walk record
 
@@ -111,6 +113,10 @@ void validate_structure(const DeviceTree& t, const std::vector<const uint8_t*>& 
   const auto nodeL = host_copy<uint4>(sec[15], nn);
   const auto skip = host_copy<uint32_t>(sec[17], nn);
   const auto dfsL = host_copy<uint4>(sec[22], nn);
+  const auto dfsC = host_copy<float4>(sec[20], nn);
+  const auto dfsN = host_copy<float4>(sec[21], nn);
+  const auto pts = host_copy<float4>(sec[0], n);
+  const auto nrm = host_copy<float4>(sec[1], n);
   const auto perm = host_copy<uint32_t>(sec[2], n);
   for (uint32_t i = 0; i < n; ++i) SBH_REQUIRE(perm[i] < n, "serialized tree: bad leaf order");
   SBH_REQUIRE(start[0] == 0 && end[0] == n, "serialized tree: root must hold every point");
@@ -134,9 +140,13 @@ void validate_structure(const DeviceTree& t, const std::vector<const uint8_t*>& 
                 "serialized tree: bad single-child chain target");
     const uint4 d = dfsL[v];
     SBH_REQUIRE(d.z == kEndHost || (d.z > v && d.z < nn), "serialized tree: bad rope");
-    if (d.w & kLeafBitHost)
+    if (d.w & kLeafBitHost) {
       SBH_REQUIRE((uint64_t)d.y + (d.w & ~kLeafBitHost) <= n, "serialized tree: bad leaf record");
-    else
+      if (d.w == (kLeafBitHost | 1u))
+        SBH_REQUIRE(!std::memcmp(&dfsC[v], &pts[d.y], sizeof(float4)) &&
+                        !std::memcmp(&dfsN[v], &nrm[d.y], sizeof(float4)),
+                    "serialized tree: one-point leaf record does not hold its point");
+    } else
       SBH_REQUIRE(d.y == v + 1 && d.y < nn && d.w >= 1 && d.w <= max_children,
                   "serialized tree: bad depth-first record");
   }

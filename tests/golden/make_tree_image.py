@@ -1,5 +1,5 @@
 """Writes tests/golden/tree_cube300_k2.sbht: the image (sbh_tree_save format,
-version 1) of the octree k=2 the library builds on the GPU for 300 points of
+version 2) of the octree k=2 the library builds on the GPU for 300 points of
 the seeded unit-cube workload.  Run on a machine with a GPU:
 
     python tests/golden/make_tree_image.py [output path]

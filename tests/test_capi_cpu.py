@@ -199,6 +199,9 @@ def test_golden_tree_image_validation_on_host():
              ("n_skip", 0, im["num_nodes"], "chain"),      # chain target out of range
              ("perm", 0, 10**6, "leaf order"),
              ("n_end", 0, 5, "")]                           # root no longer holds every point
+    # a one-point leaf's depth-first record must hold its point
+    one = int(np.flatnonzero(im["dfsL"][:, 3] == (0x80000000 | 1))[0])
+    cases.append(("dfsC", 16 * one, im["dfsC"][one, 0].view(np.uint32) ^ 1, "one-point leaf"))
     for name, off, val, msg in cases:
         bad = bytearray(img)
         at = im["offsets"][name] + off

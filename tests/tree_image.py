@@ -1,4 +1,4 @@
-"""Reader and writer of the tree image format (version 1) written by
+"""Reader and writer of the tree image format (version 2) written by
 sbh_tree_save / sbh_tree_serialize, following its documented layout.  Used by
 the tests to check images against the oracle's tree and to make damaged
 images."""
@@ -9,6 +9,7 @@ import struct
 import numpy as np
 
 MAGIC = b"SBHTREE\0"
+VERSION = 2
 HEADER = struct.Struct("<8sI5i2Ii f6f I")
 
 # (name, dtype, values per element, length basis) in file order
@@ -41,7 +42,7 @@ def parse(img: bytes) -> dict:
     """Header fields, level table and arrays (with their byte offsets)."""
     (magic, version, dim, k, leaf, max_depth, m64, n, nn, depth, alpha,
      *corners, levels) = HEADER.unpack_from(img, 0)
-    assert magic == MAGIC and version == 1
+    assert magic == MAGIC and version == VERSION
     off = HEADER.size
     level_begin = np.frombuffer(img, np.uint32, levels + 1, off)
     off += 4 * (levels + 1)
