@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
 // Fast-mode estimator with sorted samples and the per-level node cache, for
 // trees whose nodes have at most 64 children.
 template <int D, int K>
-__global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_stochastic_sorted(
     TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
     const uint32_t* __restrict__ order, float* __restrict__ out, int32_t* __restrict__ counts,
@@ -220,10 +220,10 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
   constexpr uint32_t kWaves = kThreads / 64;
   __shared__ float4 sSrc[kThreads];
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
-  __shared__ uint4 sRec[kThreads];
+  __shared__ uint2 sRec[kThreads];
   __shared__ uint2 sKeys[kWaves * kChunk];
-  __shared__ uint4 sInfo[kWaves * kCacheLevels];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];
+  __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t wbase = threadIdx.x & ~63u;
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
@@ -232,9 +232,9 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_sorted(
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const uint32_t cl = wave * kCacheLevels;
   const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
-                      sKeys + wave * kChunk, sInfo + cl, sDiff + cl * 64};
+                      sKeys + wave * kChunk, sDiff + wave * kCacheLevels * 64,
+                      sNext + wave * (kCacheLevels + 1) * 3};
   const LaneResult r =
       stochastic_estimate_sorted<D, K>(t, spl, key0, key1, g, live, x, y, z, lds, counts != nullptr);
   if (live) {

@@ -522,9 +522,12 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 // paths), and diff of the most recent node at each of the top kCacheLevels
 // levels is kept in LDS, keyed by node.  A sample whose path passes a cached
 // node reads it instead of evaluating the node's children (its opening
-// ratio is recomputed and its children's starts read from memory, which
-// keeps the cache small enough for six levels at full occupancy).  Only the order in which the samples' contributions
-// are summed differs from the per-sample loop.
+// ratio is recomputed, which keeps the cache small enough for six levels at
+// seven wavefronts per SIMD).  Each cached node also records the step the
+// latest sample took from it (the chosen child's point range and the node
+// stepped to), so a sample whose point falls in the same child steps
+// without reading the tree.  Only the order in which the samples'
+// contributions are summed differs from the per-sample loop.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kChunk = 256;        // samples drawn and sorted together
 constexpr uint32_t kCacheLevels = 6;    // path levels 1..6 cached
@@ -534,11 +537,12 @@ constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 struct SortedLds {
   float4* src;        // 64 staging slots: term sources
   float4* nrm;        // 64 staging slots: normals (winding number only)
-  uint4* rec;         // 64 staging slots: children's walk records
+  uint2* rec;         // 64 staging slots: children's {first point, leaf flag | count}
   uint2* keys;        // kChunk samples: {sampled point, level-1 roulette word with the
                       // sample's index in the chunk in its low 8 bits}
-  uint4* cinfo;       // kCacheLevels: {node, point terms, node terms, 0}
   float* cdiff;       // kCacheLevels x 64: diff of the cached node, per lane
+  uint4* nrec;        // (kCacheLevels + 1) x 3: step records (node and walk record,
+                      // centroid, normal)
 };
 
 // Sorts keys[0, n) by .x, n a power of two <= kChunk (bitonic network;
@@ -591,8 +595,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     node_accumulate<D, K, EXACT>(t, 0, x, y, z, v, cnt, cnt_nodes);
     return {v, cnt, cnt_nodes};
   }
-  if (lane < kCacheLevels) lds.cinfo[lane] = make_uint4(kNoNode, 0u, 0u, 0u);
-  wave_lds_sync();
+  // the cached nodes' identities and term counts live in one register:
+  // lane l < kCacheLevels holds level l + 1's node, lane 8 + l its point
+  // terms and lane 16 + l its node terms (read back with readlane)
+  static_assert(kCacheLevels + 1 <= 8, "a register holds eight lanes per field");
+  uint32_t cinfo = lane < kCacheLevels ? kNoNode : 0u;
 
   // Evaluates the children of v (at most 64) for every live lane: returns
   // their sum; *start = this lane's child's start (~0 past the last child);
@@ -615,7 +622,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       }
       lds.src[lane] = src;
       if (kWN) lds.nrm[lane] = nv;
-      lds.rec[lane] = l;
+      lds.rec[lane] = make_uint2(l.y, l.w);
     }
     const uint64_t multi = __ballot(np_lane > 1);
     wave_lds_sync();
@@ -630,10 +637,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         }
       } else {
         for (uint32_t j = 0; j < nch; ++j) {
-          const uint4 l = lds.rec[j];
-          const uint32_t np = l.w & ~kLeafBit;
-          if ((l.w & kLeafBit) && np > 1) {
-            for (uint32_t pp = l.y; pp < l.y + np; ++pp)
+          const uint2 l = lds.rec[j];
+          const uint32_t np = l.y & ~kLeafBit;
+          if ((l.y & kLeafBit) && np > 1) {
+            for (uint32_t pp = l.x; pp < l.x + np; ++pp)
               r = accum<D, K, EXACT>(r, pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
           } else {
             const float4 src = lds.src[j];
@@ -660,6 +667,56 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     return r;
   };
 
+  // The step taken from each cached node by the latest sample through it
+  // (record r = 1..kCacheLevels for path level r, record 0 for the
+  // subdomain's root): the chosen child's point range [lo, hi) in lanes
+  // 24 + r and 32 + r of `cinfo`, and in LDS the node stepped to (single-child
+  // chains skipped) with its walk record, centroid and normal.  A sorted
+  // sample whose point lies in the same child takes the step without a load.
+  constexpr uint32_t kNoRecord = 0xFFu;
+  auto range_has = [&](uint32_t r, uint32_t idx) {
+    return idx >= (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(24 + r)) &&
+           idx < (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(32 + r));
+  };
+  auto take = [&](uint32_t r, uint32_t& cur, uint32_t& cur_cnt, uint4& curL, float4& curC,
+                  float4& curN) {
+    const uint4 a = lds.nrec[3 * r], c = lds.nrec[3 * r + 1];
+    cur = uniform_u32(a.x);
+    cur_cnt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(32 + r)) -
+              (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(24 + r));
+    curL = make_uint4(a.y, uniform_u32(a.z), 0u, uniform_u32(a.w));
+    curC = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z),
+                       __uint_as_float(c.w));
+    if (kWN) {
+      const uint4 n = lds.nrec[3 * r + 2];
+      curN = make_float4(__uint_as_float(n.x), __uint_as_float(n.y), __uint_as_float(n.z),
+                         __uint_as_float(n.w));
+    } else {
+      curN = curC;
+    }
+  };
+  auto step = [&](uint32_t chosen, uint32_t r, uint32_t& cur, uint32_t& cur_cnt, uint4& curL,
+                  float4& curC, float4& curN) {
+    const uint32_t lo = n_start[chosen], hi = n_end[chosen];
+    cur = n_skip[chosen];
+    cur_cnt = hi - lo;
+    curL = nodeL[cur];
+    curC = nodeC[cur];
+    curN = kWN ? nodeN[cur] : curC;
+    if (r != kNoRecord) {
+      cinfo = lane == 24 + r ? lo : lane == 32 + r ? hi : cinfo;
+      if (lane == 0) {
+        lds.nrec[3 * r] = make_uint4(cur, curL.x, curL.y, curL.w);
+        lds.nrec[3 * r + 1] = make_uint4(__float_as_uint(curC.x), __float_as_uint(curC.y),
+                                         __float_as_uint(curC.z), __float_as_uint(curC.w));
+        if (kWN)
+          lds.nrec[3 * r + 2] = make_uint4(__float_as_uint(curN.x), __float_as_uint(curN.y),
+                                           __float_as_uint(curN.z), __float_as_uint(curN.w));
+      }
+      wave_lds_sync();
+    }
+  };
+
   float bh = 0.0f, mlcv = 0.0f;
   const uint32_t nsub = rl.w, first = rl.y;
   for (uint32_t si = 0; si < nsub; ++si) {
@@ -667,6 +724,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     node_accumulate<D, K, EXACT>(t, S, x, y, z, bh, cnt, cnt_nodes);
     const uint4 Sl = nodeL[S];
     if (Sl.w & kLeafBit) continue;  // the subdomain's points were summed exactly
+    cinfo = lane == 32 ? 0u : cinfo;  // record 0 belongs to the previous subdomain: empty it
     const uint32_t sstart = n_start[S];
     const uint32_t nS = n_end[S] - sstart;
     const float4 SC = nodeC[S];
@@ -706,14 +764,16 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         const uint32_t idx = uniform_u32(key.x);
         const uint32_t w1 = uniform_u32(key.y);
         const uint32_t ss = (uint32_t)s0 + (w1 & 0xFFu);
+        uint32_t cur, cur_cnt;
+        uint4 curL;
+        float4 curC, curN;
         // level 0 from the subdomain's cache
-        const uint32_t below0 = (uint32_t)__popcll(__ballot(lane < nch0 && c0_start <= idx));
-        const uint32_t chosen0 = uniform_u32(f0 + below0 - 1);
-        uint32_t cur = n_skip[chosen0];
-        uint32_t cur_cnt = n_end[chosen0] - n_start[chosen0];
-        uint4 curL = nodeL[cur];
-        float4 curC = nodeC[cur];
-        float4 curN = kWN ? nodeN[cur] : curC;
+        if (range_has(0, idx)) {
+          take(0, cur, cur_cnt, curL, curC, curN);
+        } else {
+          const uint32_t below0 = (uint32_t)__popcll(__ballot(lane < nch0 && c0_start <= idx));
+          step(uniform_u32(f0 + below0 - 1), 0, cur, cur_cnt, curL, curC, curN);
+        }
         float event_prob = 1.0f;
         float wd_all = 0.0f;
         float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
@@ -738,7 +798,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           const uint32_t slot = level - 1;
           const uint32_t cslot = min(slot, kCacheLevels - 1) * 64 + lane;  // in-bounds address
           const bool hit = slot < kCacheLevels &&
-                           uniform_u32(lds.cinfo[min(slot, kCacheLevels - 1)].x) == cur;
+                           (uint32_t)__builtin_amdgcn_readlane(
+                               (int)cinfo, (int)min(slot, kCacheLevels - 1)) == cur;
           const float ffr = path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
           ++level;
           const float tp = traverse_prob<EXACT>(cur_beta, ffr);
@@ -753,23 +814,26 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           if (__ballot(alive) == 0) break;
           const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
           float diff;
-          uint32_t below, pt, nd;
+          uint32_t below = 0, pt, nd;
+          bool known = false;  // the next node is the one recorded for this cached node
           if (hit) {
             diff = lds.cdiff[cslot];
-            const uint32_t cs = lane < nch ? n_start[f + lane] : 0xFFFFFFFFu;
-            below = (uint32_t)__popcll(__ballot(cs <= idx));
-            const uint4 ci = lds.cinfo[slot];
-            pt = uniform_u32(ci.y);
-            nd = uniform_u32(ci.z);
+            pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(8 + slot));
+            nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(16 + slot));
+            known = range_has(slot + 1, idx);
+            if (!known) {
+              const uint32_t cs = lane < nch ? n_start[f + lane] : 0xFFFFFFFFu;
+              below = (uint32_t)__popcll(__ballot(cs <= idx));
+            }
           } else {
             uint32_t cs;
             const float r = children_sum(f, nch, &cs, &pt, &nd);
             diff = r - term<D, K, EXACT>(curC, curN, alpha, alpha_k, x, y, z);
             below = (uint32_t)__popcll(__ballot(cs <= idx));
             if (slot < kCacheLevels) {
+              // each lane reads back only its own diff: no wavefront sync
               lds.cdiff[cslot] = diff;
-              if (lane == 0) lds.cinfo[slot] = make_uint4(cur, pt, nd, 0u);
-              wave_lds_sync();
+              cinfo = lane == slot ? cur : lane == 8 + slot ? pt : lane == 16 + slot ? nd : cinfo;
             }
           }
           if (alive) {
@@ -778,12 +842,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             cnt_nodes += (int32_t)nd;
           }
           cur_beta = fmaxf(cur_beta, ffr);
-          const uint32_t chosen = uniform_u32(f + below - 1);
-          cur = n_skip[chosen];
-          cur_cnt = n_end[chosen] - n_start[chosen];
-          curL = nodeL[cur];
-          curC = nodeC[cur];
-          curN = kWN ? nodeN[cur] : curC;
+          if (known)
+            take(slot + 1, cur, cur_cnt, curL, curC, curN);
+          else
+            step(uniform_u32(f + below - 1), slot < kCacheLevels ? slot + 1 : kNoRecord, cur,
+                 cur_cnt, curL, curC, curN);
         }
         sub_total += wd_all;
       }

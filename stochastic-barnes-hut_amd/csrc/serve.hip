@@ -86,8 +86,9 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
   __shared__ uint2 sAux[kServeThreads];
   constexpr uint32_t kWaves = kServeThreads / 64;
   __shared__ uint2 sKeys[kWaves * kChunk];
-  __shared__ uint4 sInfo[kWaves * kCacheLevels];
+  __shared__ uint2 sRec2[kServeThreads];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];
+  __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   while (true) {
@@ -105,9 +106,9 @@ __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restri
     load_query(r.qx, r.qy, r.qz, live ? slot : 0, dim3 ? 3 : 2, x, y, z);
     const bool stoch = variant & 1;
     const SamplerLds c{sSrc + wbase, sNrm + wbase, sRec + wbase, sAux + wbase};
-    const uint32_t cl = (threadIdx.x >> 6) * kCacheLevels;
-    const SortedLds sc{sSrc + wbase, sNrm + wbase, sRec + wbase, sKeys + (threadIdx.x >> 6) * kChunk,
-                       sInfo + cl, sDiff + cl * 64};
+    const uint32_t wave = threadIdx.x >> 6;
+    const SortedLds sc{sSrc + wbase, sNrm + wbase, sRec2 + wbase, sKeys + wave * kChunk,
+                       sDiff + wave * kCacheLevels * 64, sNext + wave * (kCacheLevels + 1) * 3};
     float v;
     switch (variant >> 1) {
       case 0:
