@@ -186,7 +186,7 @@ int sbh_tree_stream(sbh_tree* tree, void** stream);
 /*
  * Tree images.  No counterpart in the reference (its trees exist only inside
  * the Python objects that built them, include/py_tree_gpu.h:23-81): a built
- * tree is written to a byte image (format version 1: header, every node and
+ * tree is written to a byte image (format version 2: header, every node and
  * point array, FNV-1a checksum) and loaded back resident on the current GPU
  * without rebuilding, e.g. for restarts of a serving process.  A loaded tree
  * evaluates exactly as the tree that was saved.  Loading checks the checksum,
