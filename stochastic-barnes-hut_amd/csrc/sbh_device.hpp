@@ -100,25 +100,31 @@ __device__ __forceinline__ float ratio_exact(float4 c, float diam, float qx, flo
   return __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz)) / diam;
 }
 
-// ratio >= beta, decided exactly as ratio_exact would decide it.  The
-// hardware sqrt and reciprocal (within about 1 ulp each) settle every case
-// farther than 1e-6 from the threshold; the rest take the correctly rounded
-// path.
+// ratio >= beta, decided exactly as ratio_exact would decide it, in two
+// parts: the hardware sqrt and reciprocal (within about 1 ulp each) settle
+// every case farther than 1e-6 (relative) from beta (1 = far, 0 = near,
+// -1 = undecided); the correctly rounded formula decides the rest.
 // inv_diam = hardware reciprocal of diam, computed by the caller.
 template <int D>
-__device__ __forceinline__ bool opens_far(float4 c, float diam, float inv_diam, float beta,
-                                          float qx, float qy, float qz) {
+__device__ __forceinline__ int opens_far_fast(float4 c, float inv_diam, float beta, float qx,
+                                              float qy, float qz) {
 #pragma clang fp contract(off)
   float dx = qx - c.x;
   float dy = qy - c.y;
   float dz = (D == 3) ? (qz - c.z) : 0.0f;
   float d2 = sqnorm_exact<D>(dx, dy, dz);
   float r = __builtin_amdgcn_sqrtf(d2) * inv_diam;
-  if (d2 >= 1e-30f) {
-    if (r >= beta * 1.000001f) return true;
-    if (r <= beta * 0.999999f) return false;
-  }
-  return __builtin_sqrtf(d2) / diam >= beta;
+  const bool ok = d2 >= 1e-30f;
+  return (ok && r >= beta * 1.000001f) ? 1 : (ok && r <= beta * 0.999999f) ? 0 : -1;
+}
+template <int D>
+__device__ __forceinline__ bool opens_far_exact(float4 c, float diam, float beta, float qx,
+                                                float qy, float qz) {
+#pragma clang fp contract(off)
+  float dx = qx - c.x;
+  float dy = qy - c.y;
+  float dz = (D == 3) ? (qz - c.z) : 0.0f;
+  return __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz)) / diam >= beta;
 }
 
 // ---------------------------------------------------------------------------

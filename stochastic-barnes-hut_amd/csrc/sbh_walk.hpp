@@ -124,40 +124,45 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   uint4 l = nodeL[0];
   while (v != kEnd) {
     v = uniform_u32(v);
-    if (!active && live && resume == v) active = true;
+    active = active || (live && resume == v);
     const bool leaf = (l.w & kLeafBit) != 0;
     // the record (c, l) of v was loaded at the end of the previous step, as
-    // soon as v was known
+    // soon as v was known.  The step is branch-free per lane (selects, not
+    // exec-mask branches): the walk's control runs on the scalar unit, which
+    // is the busier one.
+    // every field of the record is read on every path, so it arrives in one
+    // scalar load
     const float diam = __uint_as_float(l.x);
     const float inv_diam = __builtin_amdgcn_rcpf(diam);
     bool reject = false;
-    if (active && !leaf) reject = !opens_far<D>(c, diam, inv_diam, beta, x, y, z);
+    if (!leaf) {
+      int far = opens_far_fast<D>(c, inv_diam, beta, x, y, z);
+      if (__ballot(active && far < 0))  // a lane within 1e-6 of beta: decide it exactly
+        far = far < 0 ? (int)opens_far_exact<D>(c, diam, beta, x, y, z) : far;
+      reject = active && far == 0;
+    }
     const uint64_t rej = __ballot(reject);
     const bool accept = active && ((rej & my_group) == 0);
-    if (accept) {
-      if (leaf) {
-        const uint32_t np = l.w & ~kLeafBit;
-        if (np == 1) {
-          // a one-point leaf's record holds the point
-          acc = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
-        } else {
-          for (uint32_t j = l.y; j < l.y + np; ++j)
-            acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
-        }
-        cnt += (int32_t)np;
-      } else {
-        acc = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
-        cnt_nodes += 1;
+    const uint32_t np = leaf ? (l.w & ~kLeafBit) : 0u;
+    if (np > 1) {
+      if (accept) {
+        for (uint32_t j = l.y; j < l.y + np; ++j)
+          acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
       }
+    } else if (__ballot(accept)) {
+      // an internal node's moment, or a one-point leaf's point (its record
+      // holds the point)
+      const float with = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
+      acc = accept ? with : acc;
     }
+    cnt += accept ? (int32_t)np : 0;
+    cnt_nodes += (accept && !leaf) ? 1 : 0;
     uint32_t next;
     if (rej == 0) {
       next = l.z;
     } else {
-      if (accept) {
-        active = false;
-        resume = l.z;
-      }
+      resume = accept ? l.z : resume;
+      active = active && !accept;
       next = l.y;
     }
     next = uniform_u32(next);
