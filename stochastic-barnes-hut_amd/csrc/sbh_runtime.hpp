@@ -137,13 +137,18 @@ struct DeviceTree {
   DevBuf<uint32_t> n_dfs;
   DevBuf<float4> dfsC, dfsN;
   DevBuf<uint4> dfsL;
+  // Term source of each node as a child on a sampled path: a one-point
+  // leaf's point and normal, any other node's moment (derived from the
+  // arrays above; not part of a tree image).
+  DevBuf<float4> kidC, kidN;
 
   size_t device_bytes() const {
     return pts.bytes() + nrm.bytes() + perm.bytes() + n_start.bytes() + n_end.bytes() +
            n_parent.bytes() + n_first.bytes() + n_nchild.bytes() + n_next.bytes() +
            n_depth.bytes() + n_lo.bytes() + n_hi.bytes() + n_diam.bytes() + nodeC.bytes() +
            nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
-           n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes();
+           n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
+           kidN.bytes();
   }
 };
 
@@ -157,5 +162,7 @@ void build_device_tree(DeviceTree& t, const float* points, const float* normals,
 // any device memory is written; the loader creates t.stream if it is unset.
 void serialize_tree(const DeviceTree& t, std::vector<uint8_t>& out);
 void deserialize_tree(DeviceTree& t, const uint8_t* data, size_t size);
+// Fills kidC / kidN from the node and point arrays (after a build or a load).
+void derive_child_sources(DeviceTree& t);
 
 }  // namespace sbh

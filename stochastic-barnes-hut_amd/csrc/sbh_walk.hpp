@@ -23,6 +23,8 @@ struct TreeView {
   const uint32_t* n_skip;
   const float4* pts;
   const float4* nrm;
+  const float4* kidC;  // term source of each node as a child (one-point leaf: its point)
+  const float4* kidN;
   float alpha;    // smooth-distance exponent
   float alpha_k;  // alpha * log2(e), for exp2
 };
@@ -613,20 +615,14 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
                           uint32_t* nd) -> float {
     uint32_t cstart = 0xFFFFFFFFu, np_lane = 0;
     if (lane < nch) {
+      // independent loads: the child's record, range start and term source
+      // (a one-point leaf's point; a leaf of several points is summed below)
       const uint32_t c = f + lane;
       const uint4 l = nodeL[c];
       cstart = n_start[c];
-      float4 src, nv = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (l.w & kLeafBit) {
-        np_lane = l.w & ~kLeafBit;
-        src = pts[l.y];
-        if (kWN) nv = nrm[l.y];
-      } else {
-        src = nodeC[c];
-        if (kWN) nv = nodeN[c];
-      }
-      lds.src[lane] = src;
-      if (kWN) lds.nrm[lane] = nv;
+      lds.src[lane] = t.kidC[c];
+      if (kWN) lds.nrm[lane] = t.kidN[c];
+      if (l.w & kLeafBit) np_lane = l.w & ~kLeafBit;
       lds.rec[lane] = make_uint2(l.y, l.w);
     }
     const uint64_t multi = __ballot(np_lane > 1);

@@ -564,6 +564,19 @@ __global__ void k_dfs_positions(uint32_t lb, uint32_t count, const uint32_t* n_f
   }
 }
 
+// A sampled path sums a one-point leaf child through its point; the child
+// records give every child's term source in one load.
+__global__ void k_child_sources(uint32_t nn, const float4* nodeC, const float4* nodeN,
+                                const uint4* nodeL, const float4* pts, const float4* nrm,
+                                float4* kidC, float4* kidN) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const uint4 l = nodeL[v];
+  const bool one_point = l.w == (kLeafBit | 1u);
+  kidC[v] = one_point ? pts[l.y] : nodeC[v];
+  kidN[v] = one_point ? nrm[l.y] : nodeN[v];
+}
+
 // The walk never reads a leaf's moment (leaves are summed point by point), so
 // a one-point leaf's depth-first record holds the point itself and the walk
 // sums it without a further load.
@@ -885,10 +898,23 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
       num_nodes, t.n_dfs.get(), t.n_size.get(), t.nodeC.get(), t.nodeN.get(), t.nodeL.get(),
       t.pts.get(), t.nrm.get(), t.dfsC.get(), t.dfsN.get(), t.dfsL.get());
   SBH_HIP(hipGetLastError());
+  derive_child_sources(t);
   SBH_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace
+
+void derive_child_sources(DeviceTree& t) {
+  const uint32_t nn = t.num_nodes;
+  t.kidC.alloc(nn);
+  t.kidN.alloc(nn);
+  if (nn == 0) return;
+  k_child_sources<<<blocks_for(nn, 256), 256, 0, t.stream>>>(nn, t.nodeC.get(), t.nodeN.get(),
+                                                              t.nodeL.get(), t.pts.get(),
+                                                              t.nrm.get(), t.kidC.get(),
+                                                              t.kidN.get());
+  SBH_HIP(hipGetLastError());
+}
 
 size_t exclusive_scan_scratch(uint32_t n) { return scan_scratch_size(n); }
 

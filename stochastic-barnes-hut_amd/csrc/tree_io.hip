@@ -264,6 +264,7 @@ void deserialize_tree(DeviceTree& t, const uint8_t* data, size_t size) {
                              t.stream));
     ++i;
   });
+  derive_child_sources(t);
   SBH_HIP(hipStreamSynchronize(t.stream));
 }
 
