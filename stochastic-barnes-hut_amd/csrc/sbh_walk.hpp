@@ -804,14 +804,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           const float ffr = path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
           ++level;
           const float tp = traverse_prob<EXACT>(cur_beta, ffr);
-          if (alive) {
-            if (p > tp) {
-              event_prob *= 1.0f - tp;
-              alive = false;
-            } else {
-              event_prob *= tp;
-            }
-          }
+          // roulette, as selects (no per-lane branches)
+          const bool stop = p > tp;
+          event_prob = alive ? event_prob * (stop ? 1.0f - tp : tp) : event_prob;
+          alive = alive && !stop;
           if (__ballot(alive) == 0) break;
           const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
           float diff;
@@ -837,11 +833,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
               cinfo = lane == slot ? cur : lane == 8 + slot ? pt : lane == 16 + slot ? nd : cinfo;
             }
           }
-          if (alive) {
-            wd_all += level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
-            cnt += (int32_t)pt;
-            cnt_nodes += (int32_t)nd;
-          }
+          const float wd = level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
+          wd_all = alive ? wd_all + wd : wd_all;
+          cnt += alive ? (int32_t)pt : 0;
+          cnt_nodes += alive ? (int32_t)nd : 0;
           cur_beta = fmaxf(cur_beta, ffr);
           if (known)
             take(slot + 1, cur, cur_cnt, curL, curC, curN);
