@@ -552,23 +552,23 @@ struct SortedLds {
                       // centroid, normal)
 };
 
-// Sorts keys[0, n) by .x, n a power of two <= kChunk (bitonic network;
-// every lane takes up to kChunk / 64 compare-exchanges per stage).
+// Sorts keys[0, n) by .x, n a power of two <= kChunk (bitonic network).
+// Each lane takes up to kChunk / 128 of a stage's n / 2 compare-exchanges,
+// addressed by pair: pair p joins i = p with a 0 inserted at bit log2(j) and
+// i + j, and both slots are written back (no per-lane branch on the order).
 __device__ __forceinline__ void wave_sort_keys(uint2* keys, uint32_t n) {
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 2; k <= n; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
 #pragma unroll
-      for (uint32_t m = 0; m < kChunk / 64; ++m) {
-        const uint32_t i = lane + 64 * m;
-        const uint32_t ixj = i ^ j;
-        if (i < n && ixj > i) {
-          const uint2 a = keys[i], b = keys[ixj];
-          const bool asc = (i & k) == 0;
-          if ((a.x > b.x) == asc) {
-            keys[i] = b;
-            keys[ixj] = a;
-          }
+      for (uint32_t m = 0; m < kChunk / 128; ++m) {
+        const uint32_t p = lane + 64 * m;
+        if (p < n / 2) {
+          const uint32_t i = (p & (j - 1)) | ((p & ~(j - 1)) << 1);
+          const uint2 a = keys[i], b = keys[i + j];
+          const bool swap = (a.x > b.x) == ((i & k) == 0);
+          keys[i] = swap ? b : a;
+          keys[i + j] = swap ? a : b;
         }
       }
       wave_lds_sync();
@@ -627,8 +627,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     }
     const uint64_t multi = __ballot(np_lane > 1);
     wave_lds_sync();
+    // every lane sums (a lane past the last query holds query 0's position
+    // and its sums are dropped), so the loop needs no lane mask
     float r = 0.0f;
-    if (live) {
+    {
       if (multi == 0) {
 #pragma unroll 4
         for (uint32_t j = 0; j < nch; ++j) {
@@ -776,14 +778,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           step(uniform_u32(f0 + below0 - 1), 0, cur, cur_cnt, curL, curC, curN);
         }
         float event_prob = 1.0f;
-        float wd_all = 0.0f;
+        float wd_all = live ? 0.0f + wd0 : 0.0f;
         float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
         bool alive = live;
-        if (live) {
-          wd_all += wd0;
-          cnt += (int32_t)pt0;
-          cnt_nodes += (int32_t)nd0;
-        }
+        cnt += live ? (int32_t)pt0 : 0;
+        cnt_nodes += live ? (int32_t)nd0 : 0;
         Philox4 blk{0u, 0u, 0u, w1};  // block 0: only its level-1 word is used here
         uint32_t blk_id = 0;
         uint32_t level = 1;
