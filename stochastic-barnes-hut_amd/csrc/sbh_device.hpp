@@ -12,6 +12,13 @@ constexpr int kWave = 64;
 constexpr uint32_t kEnd = 0xFFFFFFFFu;   // rope terminator
 constexpr uint32_t kLeafBit = 0x80000000u;
 
+// A depth-first walk record: centroid and mass, then {diameter bits, first
+// child or first point, rope, child count | leaf flag and point count}.
+struct alignas(32) WalkRecord {
+  float4 c;
+  uint4 l;
+};
+
 // include/kernel.h:24-28 (bound kernels) and :61-83 (unbound: Helmholtz,
 // and the gravity gradient as one scalar kernel per axis)
 enum Kernel : int {

@@ -23,6 +23,7 @@ struct TreeView {
   const uint32_t* n_skip;
   const float4* pts;
   const float4* nrm;
+  const WalkRecord* dfsR;  // depth-first walk records, centroid and record together
   const float4* kidC;  // term source of each node as a child (one-point leaf: its point)
   const float4* kidN;
   float alpha;    // smooth-distance exponent
@@ -114,16 +115,15 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   } else {
     my_group = 1ull << lane;
   }
-  const float4* __restrict__ nodeC = t.dfsC;
-  const uint4* __restrict__ nodeL = t.dfsL;
+  const WalkRecord* __restrict__ rec = t.dfsR;
   const float4* __restrict__ nodeN = t.dfsN;
   float acc = 0.0f;
   int32_t cnt = 0, cnt_nodes = 0;
   bool active = live;
   uint32_t resume = kEnd;
   uint32_t v = 0;
-  float4 c = nodeC[0];
-  uint4 l = nodeL[0];
+  float4 c = rec[0].c;
+  uint4 l = rec[0].l;
   while (v != kEnd) {
     v = uniform_u32(v);
     active = active || (live && resume == v);
@@ -169,8 +169,9 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
     }
     next = uniform_u32(next);
     if (next != kEnd) {
-      c = nodeC[next];
-      l = nodeL[next];
+      const WalkRecord r = rec[next];
+      c = r.c;
+      l = r.l;
     }
     v = next;
   }

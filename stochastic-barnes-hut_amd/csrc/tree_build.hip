@@ -568,13 +568,15 @@ __global__ void k_dfs_positions(uint32_t lb, uint32_t count, const uint32_t* n_f
 // records give every child's term source in one load.
 __global__ void k_child_sources(uint32_t nn, const float4* nodeC, const float4* nodeN,
                                 const uint4* nodeL, const float4* pts, const float4* nrm,
-                                float4* kidC, float4* kidN) {
+                                const float4* dfsC, const uint4* dfsL, float4* kidC,
+                                float4* kidN, WalkRecord* dfsR) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   const uint4 l = nodeL[v];
   const bool one_point = l.w == (kLeafBit | 1u);
   kidC[v] = one_point ? pts[l.y] : nodeC[v];
   kidN[v] = one_point ? nrm[l.y] : nodeN[v];
+  dfsR[v] = WalkRecord{dfsC[v], dfsL[v]};
 }
 
 // The walk never reads a leaf's moment (leaves are summed point by point), so
@@ -908,11 +910,11 @@ void derive_child_sources(DeviceTree& t) {
   const uint32_t nn = t.num_nodes;
   t.kidC.alloc(nn);
   t.kidN.alloc(nn);
+  t.dfsR.alloc(nn);
   if (nn == 0) return;
-  k_child_sources<<<blocks_for(nn, 256), 256, 0, t.stream>>>(nn, t.nodeC.get(), t.nodeN.get(),
-                                                              t.nodeL.get(), t.pts.get(),
-                                                              t.nrm.get(), t.kidC.get(),
-                                                              t.kidN.get());
+  k_child_sources<<<blocks_for(nn, 256), 256, 0, t.stream>>>(
+      nn, t.nodeC.get(), t.nodeN.get(), t.nodeL.get(), t.pts.get(), t.nrm.get(), t.dfsC.get(),
+      t.dfsL.get(), t.kidC.get(), t.kidN.get(), t.dfsR.get());
   SBH_HIP(hipGetLastError());
 }
 
