@@ -14,7 +14,8 @@ namespace sbh {
 inline TreeView view_of(const DeviceTree& t) {
   return TreeView{t.nodeC.get(), t.nodeL.get(), t.nodeN.get(), t.dfsC.get(), t.dfsL.get(),
                   t.dfsN.get(), t.n_start.get(), t.n_end.get(), t.n_skip.get(), t.pts.get(),
-                  t.nrm.get(), t.dfsR.get(), t.kidC.get(), t.kidN.get(), t.alpha,
+                  t.nrm.get(), t.dfsR.get(), t.bfsR.get(), t.stepR.get(), t.kidC.get(),
+                  t.kidN.get(), t.alpha,
                   t.alpha * 1.4426950408889634f};
 }
 

@@ -146,6 +146,10 @@ struct DeviceTree {
   // The depth-first walk records (dfsC, dfsL) interleaved, 32 bytes a node,
   // so that a walk step is one scalar load (derived, not in a tree image).
   DevBuf<WalkRecord> dfsR;
+  // Breadth-first records for the sampler's steps: {nodeC, nodeL} together,
+  // and {range start, range end, single-child chain target, 0} (derived).
+  DevBuf<WalkRecord> bfsR;
+  DevBuf<uint4> stepR;
 
   size_t device_bytes() const {
     return pts.bytes() + nrm.bytes() + perm.bytes() + n_start.bytes() + n_end.bytes() +
@@ -153,7 +157,7 @@ struct DeviceTree {
            n_depth.bytes() + n_lo.bytes() + n_hi.bytes() + n_diam.bytes() + nodeC.bytes() +
            nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
            n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
-           kidN.bytes() + dfsR.bytes();
+           kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes();
   }
 };
 
@@ -167,8 +171,8 @@ void build_device_tree(DeviceTree& t, const float* points, const float* normals,
 // any device memory is written; the loader creates t.stream if it is unset.
 void serialize_tree(const DeviceTree& t, std::vector<uint8_t>& out);
 void deserialize_tree(DeviceTree& t, const uint8_t* data, size_t size);
-// Fills kidC / kidN and dfsR from the node, point and depth-first arrays
-// (after a build or a load).
+// Fills kidC / kidN, dfsR, bfsR and stepR from the node, point and
+// depth-first arrays (after a build or a load).
 void derive_child_sources(DeviceTree& t);
 
 }  // namespace sbh

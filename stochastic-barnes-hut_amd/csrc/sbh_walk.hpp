@@ -24,6 +24,8 @@ struct TreeView {
   const float4* pts;
   const float4* nrm;
   const WalkRecord* dfsR;  // depth-first walk records, centroid and record together
+  const WalkRecord* bfsR;  // nodeC and nodeL together (breadth-first)
+  const uint4* stepR;      // {range start, range end, chain target, 0}
   const float4* kidC;  // term source of each node as a child (one-point leaf: its point)
   const float4* kidN;
   float alpha;    // smooth-distance exponent
@@ -701,11 +703,13 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   };
   auto step = [&](uint32_t chosen, uint32_t r, uint32_t& cur, uint32_t& cur_cnt, uint4& curL,
                   float4& curC, float4& curN) {
-    const uint32_t lo = n_start[chosen], hi = n_end[chosen];
-    cur = n_skip[chosen];
+    const uint4 sr = t.stepR[chosen];
+    const uint32_t lo = sr.x, hi = sr.y;
+    cur = sr.z;
     cur_cnt = hi - lo;
-    curL = nodeL[cur];
-    curC = nodeC[cur];
+    const WalkRecord cr = t.bfsR[cur];
+    curL = cr.l;
+    curC = cr.c;
     curN = kWN ? nodeN[cur] : curC;
     if (r != kNoRecord) {
       cinfo = lane == 24 + r ? lo : lane == 32 + r ? hi : cinfo;

@@ -568,8 +568,10 @@ __global__ void k_dfs_positions(uint32_t lb, uint32_t count, const uint32_t* n_f
 // records give every child's term source in one load.
 __global__ void k_child_sources(uint32_t nn, const float4* nodeC, const float4* nodeN,
                                 const uint4* nodeL, const float4* pts, const float4* nrm,
-                                const float4* dfsC, const uint4* dfsL, float4* kidC,
-                                float4* kidN, WalkRecord* dfsR) {
+                                const float4* dfsC, const uint4* dfsL, const uint32_t* n_start,
+                                const uint32_t* n_end, const uint32_t* n_skip, float4* kidC,
+                                float4* kidN, WalkRecord* dfsR, WalkRecord* bfsR,
+                                uint4* stepR) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   const uint4 l = nodeL[v];
@@ -577,6 +579,8 @@ __global__ void k_child_sources(uint32_t nn, const float4* nodeC, const float4* 
   kidC[v] = one_point ? pts[l.y] : nodeC[v];
   kidN[v] = one_point ? nrm[l.y] : nodeN[v];
   dfsR[v] = WalkRecord{dfsC[v], dfsL[v]};
+  bfsR[v] = WalkRecord{nodeC[v], l};
+  stepR[v] = make_uint4(n_start[v], n_end[v], n_skip[v], 0u);
 }
 
 // The walk never reads a leaf's moment (leaves are summed point by point), so
@@ -911,10 +915,13 @@ void derive_child_sources(DeviceTree& t) {
   t.kidC.alloc(nn);
   t.kidN.alloc(nn);
   t.dfsR.alloc(nn);
+  t.bfsR.alloc(nn);
+  t.stepR.alloc(nn);
   if (nn == 0) return;
   k_child_sources<<<blocks_for(nn, 256), 256, 0, t.stream>>>(
       nn, t.nodeC.get(), t.nodeN.get(), t.nodeL.get(), t.pts.get(), t.nrm.get(), t.dfsC.get(),
-      t.dfsL.get(), t.kidC.get(), t.kidN.get(), t.dfsR.get());
+      t.dfsL.get(), t.n_start.get(), t.n_end.get(), t.n_skip.get(), t.kidC.get(), t.kidN.get(),
+      t.dfsR.get(), t.bfsR.get(), t.stepR.get());
   SBH_HIP(hipGetLastError());
 }
 
