@@ -146,7 +146,8 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       reject = active && far == 0;
     }
     const uint64_t rej = __ballot(reject);
-    const bool accept = active && ((rej & my_group) == 0);
+    // per query, a lane's own test decides; a vote group needs every lane
+    const bool accept = active && (GROUP == 1 ? !reject : (rej & my_group) == 0);
     const uint32_t np = leaf ? (l.w & ~kLeafBit) : 0u;
     if (np > 1) {
       if (accept) {
