@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
 // Fast-mode estimator with sorted samples and the per-level node cache, for
 // trees whose nodes have at most 64 children.
 template <int D, int K>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_stochastic_sorted(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == WINDING_NUMBER ? 6 : 7, K == WINDING_NUMBER ? 6 : 7))) void k_stochastic_sorted(
     TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
     const uint32_t* __restrict__ order, float* __restrict__ out, int32_t* __restrict__ counts,
