@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
 
 // Fast-mode estimator with sorted samples and the per-level node cache, for
 // trees whose nodes have at most 64 children.
-template <int D, int K>
+template <int D, int K, bool COUNT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == WINDING_NUMBER ? 6 : 7, K == WINDING_NUMBER ? 6 : 7))) void k_stochastic_sorted(
     TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
@@ -236,10 +236,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == W
                       sKeys + wave * kChunk, sDiff + wave * kCacheLevels * 64,
                       sNext + wave * (kCacheLevels + 1) * 3};
   const LaneResult r =
-      stochastic_estimate_sorted<D, K>(t, spl, key0, key1, g, live, x, y, z, lds, counts != nullptr);
+      stochastic_estimate_sorted<D, K, COUNT>(t, spl, key0, key1, g, live, x, y, z, lds);
   if (live) {
     out[qi] = r.value;
-    if (counts) {
+    if (COUNT && counts) {
       if (split_counts) {
         counts[qi] = r.nodes;
         counts[nq + qi] = r.points;
@@ -460,8 +460,12 @@ void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, bool per_query, c
     k_stochastic_query<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
         view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
         counts, split);
+  else if (!EXACT && small_fanout && counts)
+    k_stochastic_sorted<D, K, true><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+        view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
+        counts, split);
   else if (!EXACT && small_fanout)
-    k_stochastic_sorted<D, K><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+    k_stochastic_sorted<D, K, false><<<nblk(nq, kThreads), kThreads, 0, s>>>(
         view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
         counts, split);
   else

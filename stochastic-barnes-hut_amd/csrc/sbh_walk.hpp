@@ -580,12 +580,13 @@ __device__ __forceinline__ void wave_sort_keys(uint2* keys, uint32_t n) {
   }
 }
 
-template <int D, int K>
+// COUNT: also count the terms each query's estimate sums (the counting
+// mode); without it the counts are not formed at all.
+template <int D, int K, bool COUNT>
 __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView& t, int spl,
                                                                  uint32_t key0, uint32_t key1,
                                                                  uint32_t g, bool live, float x,
-                                                                 float y, float z, SortedLds lds,
-                                                                 bool count = true) {
+                                                                 float y, float z, SortedLds lds) {
 #pragma clang fp contract(off)
   constexpr bool EXACT = false;
   constexpr bool kWN = (K == WINDING_NUMBER);
@@ -659,7 +660,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     }
     *pt = 0;
     *nd = 0;
-    if (count) {
+    if constexpr (COUNT) {
       uint32_t a = np_lane, b = (lane < nch && np_lane == 0) ? 1u : 0u;
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
@@ -787,8 +788,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         float wd_all = live ? 0.0f + wd0 : 0.0f;
         float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
         bool alive = live;
-        cnt += live ? (int32_t)pt0 : 0;
-        cnt_nodes += live ? (int32_t)nd0 : 0;
+        if constexpr (COUNT) {
+          cnt += live ? (int32_t)pt0 : 0;
+          cnt_nodes += live ? (int32_t)nd0 : 0;
+        }
         Philox4 blk{0u, 0u, 0u, w1};  // block 0: only its level-1 word is used here
         uint32_t blk_id = 0;
         uint32_t level = 1;
@@ -820,8 +823,12 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           bool known = false;  // the next node is the one recorded for this cached node
           if (hit) {
             diff = lds.cdiff[cslot];
-            pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(8 + slot));
-            nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(16 + slot));
+            if constexpr (COUNT) {
+              pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(8 + slot));
+              nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(16 + slot));
+            } else {
+              pt = nd = 0;
+            }
             known = range_has(slot + 1, idx);
             if (!known) {
               const uint32_t cs = lane < nch ? n_start[f + lane] : 0xFFFFFFFFu;
@@ -835,13 +842,18 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             if (slot < kCacheLevels) {
               // each lane reads back only its own diff: no wavefront sync
               lds.cdiff[cslot] = diff;
-              cinfo = lane == slot ? cur : lane == 8 + slot ? pt : lane == 16 + slot ? nd : cinfo;
+              if constexpr (COUNT)
+                cinfo = lane == slot ? cur : lane == 8 + slot ? pt : lane == 16 + slot ? nd : cinfo;
+              else
+                cinfo = lane == slot ? cur : cinfo;
             }
           }
           const float wd = level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
           wd_all = alive ? wd_all + wd : wd_all;
-          cnt += alive ? (int32_t)pt : 0;
-          cnt_nodes += alive ? (int32_t)nd : 0;
+          if constexpr (COUNT) {
+            cnt += alive ? (int32_t)pt : 0;
+            cnt_nodes += alive ? (int32_t)nd : 0;
+          }
           cur_beta = fmaxf(cur_beta, ffr);
           if (known)
             take(slot + 1, cur, cur_cnt, curL, curC, curN);

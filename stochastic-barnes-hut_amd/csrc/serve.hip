@@ -66,8 +66,8 @@ __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, ui
   // the same estimator as a one-shot evaluation of the request picks, so a
   // served result equals the one-shot result
   if (stochastic && uniform_u32((uint32_t)r.sorted))
-    return stochastic_estimate_sorted<D, K>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
-                                            sorted_lds, false)
+    return stochastic_estimate_sorted<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x,
+                                                   y, z, sorted_lds)
         .value;
   if (stochastic)
     return stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
