@@ -143,7 +143,7 @@ __global__ void k_reduce_splits_f64(const double* partial, uint32_t nsplit, uint
   out[i] = s;
 }
 
-template <int D, int K, int GROUP, bool EXACT>
+template <int D, int K, int GROUP, bool EXACT, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_bh(TreeView t, float beta,
                                                   const float* __restrict__ qx,
                                                   const float* __restrict__ qy,
@@ -156,10 +156,10 @@ __global__ __launch_bounds__(kThreads) void k_bh(TreeView t, float beta,
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const LaneResult r = bh_walk<D, K, GROUP, EXACT>(t, beta, live, x, y, z);
+  const LaneResult r = bh_walk<D, K, GROUP, EXACT, COUNT>(t, beta, live, x, y, z);
   if (live) {
     out[qi] = r.value;
-    if (counts) {
+    if (COUNT && counts) {
       if (split_counts) {
         counts[qi] = r.nodes;
         counts[nq + qi] = r.points;
@@ -437,15 +437,17 @@ void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, cons
                int split, hipStream_t s) {
   uint32_t grid = nblk(nq, kThreads);
   const TreeView tv = view_of(t);
-#define SBH_LAUNCH_BH(G)                                                                       \
-  k_bh<D, K, G, EXACT><<<grid, kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq, order, out, counts, \
-                                                  split)
+#define SBH_LAUNCH_BH(G, COUNT)                                                      \
+  k_bh<D, K, G, EXACT, COUNT><<<grid, kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq, order, out, \
+                                                         counts, split)
   if (group == 64)
-    SBH_LAUNCH_BH(64);
+    SBH_LAUNCH_BH(64, true);
   else if (group == 32)
-    SBH_LAUNCH_BH(32);
-  else
-    SBH_LAUNCH_BH(1);
+    SBH_LAUNCH_BH(32, true);
+  else if (counts || EXACT)
+    SBH_LAUNCH_BH(1, true);
+  else  // the fast per-query walk without counts (evaluation and serving)
+    SBH_LAUNCH_BH(1, false);
 #undef SBH_LAUNCH_BH
   SBH_HIP(hipGetLastError());
 }

@@ -105,7 +105,8 @@ __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
 // correctly rounded operations, so every lane opens exactly the nodes the
 // reference walk opens.
 // ---------------------------------------------------------------------------
-template <int D, int K, int GROUP, bool EXACT>
+// COUNT: also count the terms (the counting mode).
+template <int D, int K, int GROUP, bool EXACT, bool COUNT = true>
 __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, bool live, float x,
                                               float y, float z) {
   const uint32_t lane = threadIdx.x & 63;
@@ -160,8 +161,10 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       const float with = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
       acc = accept ? with : acc;
     }
-    cnt += accept ? (int32_t)np : 0;
-    cnt_nodes += (accept && !leaf) ? 1 : 0;
+    if constexpr (COUNT) {
+      cnt += accept ? (int32_t)np : 0;
+      cnt_nodes += (accept && !leaf) ? 1 : 0;
+    }
     uint32_t next;
     if (rej == 0) {
       next = l.z;

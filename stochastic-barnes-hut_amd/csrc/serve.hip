@@ -73,7 +73,7 @@ __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, ui
     return stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
                                             lds, false)
         .value;
-  return bh_walk<D, K, 1, false>(r.tree, r.beta, live, x, y, z).value;
+  return bh_walk<D, K, 1, false, false>(r.tree, r.beta, live, x, y, z).value;
 }
 
 __global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restrict__ reqs,
