@@ -108,21 +108,22 @@ __device__ __forceinline__ float ratio_exact(float4 c, float diam, float qx, flo
 }
 
 // ratio >= beta, decided exactly as ratio_exact would decide it, in two
-// parts: the hardware sqrt and reciprocal (within about 1 ulp each) settle
-// every case farther than 1e-6 (relative) from beta (1 = far, 0 = near,
-// -1 = undecided); the correctly rounded formula decides the rest.
+// parts: a fast estimate (fused squares, hardware sqrt and reciprocal; within
+// 7e-7 relative of ratio_exact) settles every case farther than 2e-6 from
+// beta (1 = far, 0 = near, -1 = undecided); the correctly rounded formula
+// decides the rest.
 // inv_diam = hardware reciprocal of diam, computed by the caller.
 template <int D>
 __device__ __forceinline__ int opens_far_fast(float4 c, float inv_diam, float beta, float qx,
                                               float qy, float qz) {
-#pragma clang fp contract(off)
   float dx = qx - c.x;
   float dy = qy - c.y;
   float dz = (D == 3) ? (qz - c.z) : 0.0f;
-  float d2 = sqnorm_exact<D>(dx, dy, dz);
+  float d2 = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
+                      : __builtin_fmaf(dx, dx, dy * dy);
   float r = __builtin_amdgcn_sqrtf(d2) * inv_diam;
   const bool ok = d2 >= 1e-30f;
-  return (ok && r >= beta * 1.000001f) ? 1 : (ok && r <= beta * 0.999999f) ? 0 : -1;
+  return (ok && r >= beta * 1.000002f) ? 1 : (ok && r <= beta * 0.999998f) ? 0 : -1;
 }
 template <int D>
 __device__ __forceinline__ bool opens_far_exact(float4 c, float diam, float beta, float qx,

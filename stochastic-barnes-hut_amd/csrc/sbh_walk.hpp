@@ -142,7 +142,7 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
     bool reject = false;
     if (!leaf) {
       int far = opens_far_fast<D>(c, inv_diam, beta, x, y, z);
-      if (__ballot(active && far < 0))  // a lane within 1e-6 of beta: decide it exactly
+      if (__ballot(active && far < 0))  // a lane within 2e-6 of beta: decide it exactly
         far = far < 0 ? (int)opens_far_exact<D>(c, diam, beta, x, y, z) : far;
       reject = active && far == 0;
     }
