@@ -307,4 +307,18 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Element i of an array no kernel writes while it runs, read through the
+// constant address space: with a wave-uniform i this is a scalar load.
+template <typename T>
+__device__ __forceinline__ T load_invariant(const T* p, uint32_t i) {
+  static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+  using Word = const __attribute__((address_space(4))) uint32_t;
+  Word* w = (Word*)(p + i);
+  T out;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (uint32_t k = 0; k < sizeof(T) / 4; ++k) o[k] = w[k];
+  return out;
+}
+
 }  // namespace sbh

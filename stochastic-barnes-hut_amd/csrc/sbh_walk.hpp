@@ -602,6 +602,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   const uint32_t* __restrict__ n_skip = t.n_skip;
   const float4* __restrict__ pts = t.pts;
   const float4* __restrict__ nrm = t.nrm;
+  // read-only for the whole launch: wave-uniform reads become scalar loads
+  const uint4* __restrict__ stepR = t.stepR;
+  const WalkRecord* __restrict__ bfsR = t.bfsR;
+  const float4* __restrict__ kidC = t.kidC;
+  const float4* __restrict__ kidN = t.kidN;
   const float alpha = t.alpha, alpha_k = t.alpha_k;
   int32_t cnt = 0, cnt_nodes = 0;
   const uint4 rl = nodeL[0];
@@ -628,8 +633,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       const uint32_t c = f + lane;
       const uint4 l = nodeL[c];
       cstart = n_start[c];
-      lds.src[lane] = t.kidC[c];
-      if (kWN) lds.nrm[lane] = t.kidN[c];
+      lds.src[lane] = kidC[c];
+      if (kWN) lds.nrm[lane] = kidN[c];
       if (l.w & kLeafBit) np_lane = l.w & ~kLeafBit;
       lds.rec[lane] = make_uint2(l.y, l.w);
     }
@@ -708,11 +713,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   };
   auto step = [&](uint32_t chosen, uint32_t r, uint32_t& cur, uint32_t& cur_cnt, uint4& curL,
                   float4& curC, float4& curN) {
-    const uint4 sr = t.stepR[chosen];
+    const uint4 sr = load_invariant(stepR, chosen);
     const uint32_t lo = sr.x, hi = sr.y;
     cur = sr.z;
     cur_cnt = hi - lo;
-    const WalkRecord cr = t.bfsR[cur];
+    const WalkRecord cr = load_invariant(bfsR, cur);
     curL = cr.l;
     curC = cr.c;
     curN = kWN ? nodeN[cur] : curC;
