@@ -17,8 +17,9 @@
 // (bf16), small against d2 once d >= 2r (fp32) or d >= 4r (bf16).  Pairs
 // closer than that (the near field) are recomputed directly in fp32.  The
 // remaining per-pair work (sqrt, add, reciprocal or exp2, fma) runs on the
-// VALU.  Partial sums per lane are kept for 4 x 4 query rows and reduced over
-// the 16 lanes that share them at the end.
+// VALU.  Partial sums per lane are kept for 4 x 4 query rows per stage of
+// sources, reduced over the 16 lanes that share them, and accumulated in
+// float64 by one lane per row.
 //
 // Results match the direct formulation to ~1e-6 relative (checked against
 // the oracle and float64 in the GPU tests).
@@ -93,7 +94,7 @@ __device__ __forceinline__ float far_term(float d2, float m, float alpha_k) {
 }
 
 template <int K, bool BF16>
-__global__ __launch_bounds__(kThreads) void k_brute_mfma(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_brute_mfma(
     const float4* __restrict__ pts, uint32_t n, float alpha_k, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
     const uint32_t* __restrict__ order, uint32_t per_split, double* __restrict__ partial,
@@ -140,11 +141,10 @@ __global__ __launch_bounds__(kThreads) void k_brute_mfma(
     for (int v = 0; v < 4; ++v) cinit[t][v] = wq[wbase + 16 * t + 4 * k4 + v].w;
   }
 
-  double acc64[4][4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc64[t][v] = 0.0;
+  // float64 running sum of query row (16t + 4(lane / 16) + v) for t, v with
+  // 4t + v = lane % 16: each stage's fp32 partial sums are reduced over the 16
+  // lanes that share a row and added here
+  double acc64 = 0.0;
 
   const uint32_t s0 = blockIdx.y * per_split;
   const uint32_t s1 = min(n, s0 + per_split);
@@ -180,64 +180,70 @@ __global__ __launch_bounds__(kThreads) void k_brute_mfma(
                      -2.0f * bf16_round(py - yh), -2.0f * zh, -2.0f * bf16_round(pz - zh), sh,
                      sm, bf16_round(s2 - sh - sm));
       } else {
-        b = k4 == 0 ? -2.0f * px
-          : k4 == 1 ? -2.0f * py
-          : k4 == 2 ? -2.0f * pz
-                    : px * px + py * py + pz * pz;
+        // every lane forms both candidates and selects (no lane branches)
+        const float axis = k4 == 0 ? px : k4 == 1 ? py : pz;
+        const float s2 = px * px + py * py + pz * pz;
+        b = k4 < 3 ? -2.0f * axis : s2;
       }
+      f32x4 d[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        f32x4 d;
         if constexpr (BF16)
-          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab[t], bb, cinit[t], 0, 0, 0);
+          d[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab[t], bb, cinit[t], 0, 0, 0);
         else
-          d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b, cinit[t], 0, 0, 0);
+          d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b, cinit[t], 0, 0, 0);
+      }
+      float dmin = d[0][0];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          float d2 = d[v];
-          if (d2 < near2) {
-            // near field: direct fp32 distance
-            const float4 qq = wq[wbase + 16 * t + 4 * k4 + v];
-            const float dx = p.x - qq.x, dy = p.y - qq.y, dz = p.z - qq.z;
-            d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dmin = fminf(dmin, d[t][v]);
+      if (!__ballot(dmin < near2 || (K != GRAVITY && !valid))) {
+        // the whole tile is in every query's far field (the common case for
+        // Morton-ordered queries and leaf-ordered sources): the expansion's
+        // distances are used as they are
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[t][v] += far_term<K>(d[t][v], p.w, alpha_k);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float d2 = d[t][v];
+            if (d2 < near2) {
+              // near field: direct fp32 distance
+              const float4 qq = wq[wbase + 16 * t + 4 * k4 + v];
+              const float dx = p.x - qq.x, dy = p.y - qq.y, dz = p.z - qq.z;
+              d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            }
+            float term = far_term<K>(fmaxf(d2, 0.0f), p.w, alpha_k);
+            if (K != GRAVITY && !valid) term = 0.0f;
+            acc[t][v] += term;
           }
-          float term = far_term<K>(fmaxf(d2, 0.0f), p.w, alpha_k);
-          if (K != GRAVITY && !valid) term = 0.0f;
-          acc[t][v] += term;
         }
       }
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc64[t][v] += (double)acc[t][v];
-  }
-
-  // reduce over the 16 lanes that hold the same query rows (same lane / 16)
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double s = acc64[t][v];
-#pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
-      acc64[t][v] = s;
-    }
-  if (i16 == 0) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const uint32_t row = 16 * t + 4 * k4 + v;
-        const uint32_t s = base + row;
-        if (s < nq) {
-          const uint32_t q = order[s];
-          if (partial)
-            partial[(size_t)blockIdx.y * nq + q] = acc64[t][v];
-          else
-            out[q] = (float)acc64[t][v];
-        }
+        float sum = acc[t][v];
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 16);
+        if (i16 == (uint32_t)(4 * t + v)) acc64 += (double)sum;
       }
+  }
+
+  const uint32_t row = 16 * (i16 >> 2) + 4 * k4 + (i16 & 3);
+  const uint32_t srow = base + row;
+  if (srow < nq) {
+    const uint32_t q = order[srow];
+    if (partial)
+      partial[(size_t)blockIdx.y * nq + q] = acc64;
+    else
+      out[q] = (float)acc64;
   }
 }
 
