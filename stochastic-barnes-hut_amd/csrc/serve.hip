@@ -76,7 +76,7 @@ __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, ui
   return bh_walk<D, K, 1, false, false>(r.tree, r.beta, live, x, y, z).value;
 }
 
-__global__ __launch_bounds__(kServeThreads) void k_serve(const ReqDesc* __restrict__ reqs,
+__global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_serve(const ReqDesc* __restrict__ reqs,
                                                           const Task* __restrict__ tasks,
                                                           uint32_t ntasks,
                                                           uint32_t* __restrict__ next_task) {
