@@ -12,8 +12,10 @@ constexpr int kWave = 64;
 constexpr uint32_t kEnd = 0xFFFFFFFFu;   // rope terminator
 constexpr uint32_t kLeafBit = 0x80000000u;
 
-// A depth-first walk record: centroid and mass, then {diameter bits, first
-// child or first point, rope, child count | leaf flag and point count}.
+// A walk record: centroid and mass, then {diameter bits, first child or
+// first point, rope, child count | leaf flag and point count}.  The
+// breadth-first copy the fast sampler reads holds the diameter's hardware
+// reciprocal in place of the diameter.
 struct alignas(32) WalkRecord {
   float4 c;
   uint4 l;

@@ -817,7 +817,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           const bool hit = slot < kCacheLevels &&
                            (uint32_t)__builtin_amdgcn_readlane(
                                (int)cinfo, (int)min(slot, kCacheLevels - 1)) == cur;
-          const float ffr = path_ratio<D, EXACT>(curC, __uint_as_float(curL.x), x, y, z);
+          // the breadth-first records hold the diameter's hardware reciprocal
+          const float ffr = ratio_fast<D>(curC, __uint_as_float(curL.x), x, y, z);
           ++level;
           const float tp = traverse_prob<EXACT>(cur_beta, ffr);
           // roulette, as selects (no per-lane branches)

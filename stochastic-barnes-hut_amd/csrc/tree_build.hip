@@ -579,7 +579,11 @@ __global__ void k_child_sources(uint32_t nn, const float4* nodeC, const float4* 
   kidC[v] = one_point ? pts[l.y] : nodeC[v];
   kidN[v] = one_point ? nrm[l.y] : nodeN[v];
   dfsR[v] = WalkRecord{dfsC[v], dfsL[v]};
-  bfsR[v] = WalkRecord{nodeC[v], l};
+  // the fast sampler's opening ratios use the hardware reciprocal of the
+  // diameter (never the diameter itself): its records hold the reciprocal
+  bfsR[v] = WalkRecord{nodeC[v],
+                       make_uint4(__float_as_uint(__builtin_amdgcn_rcpf(__uint_as_float(l.x))), l.y,
+                                  l.z, l.w)};
   stepR[v] = make_uint4(n_start[v], n_end[v], n_skip[v], 0u);
 }
 
