@@ -219,6 +219,9 @@ def test_barnes_hut(sbh_gpu, vote, kernel, dim, k):
             if exact and kernel != "smoothdist":
                 # same terms, same order, correctly rounded: identical sums
                 np.testing.assert_array_equal(got, ref)
+            if not exact:
+                # the launch without counts computes the same values
+                np.testing.assert_array_equal(fn(g, q, beta, vote_group=vote)[0], got)
 
 
 def test_barnes_hut_sorted_queries_identical(sbh_gpu):
@@ -244,8 +247,10 @@ def test_stochastic_same_seed(sbh_gpu, kernel, dim, k):
         assert err.max() <= TOL, (spl, seed, err.max())
         if kernel != "smoothdist":
             np.testing.assert_array_equal(got, ref)
-        _check_fast(fn(g, q, spl, seed=seed, return_counts=True), ref, rcnt, kernel,
-                    f"{kernel} d{dim} k{k} spl{spl}")
+        counted = fn(g, q, spl, seed=seed, return_counts=True)
+        _check_fast(counted, ref, rcnt, kernel, f"{kernel} d{dim} k{k} spl{spl}")
+        # the launch without counts computes the same values
+        np.testing.assert_array_equal(fn(g, q, spl, seed=seed)[0], counted[0])
 
 
 def _check_fast(res, ref, rcnt, kernel, label):
