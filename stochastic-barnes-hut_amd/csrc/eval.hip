@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
 // Fast-mode estimator with sorted samples and the per-level node cache, for
 // trees whose nodes have at most 64 children.
 template <int D, int K, bool COUNT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == WINDING_NUMBER ? 6 : 7, K == WINDING_NUMBER ? 6 : 7))) void k_stochastic_sorted(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_stochastic_sorted(
     TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
     const uint32_t* __restrict__ order, float* __restrict__ out, int32_t* __restrict__ counts,
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == W
   __shared__ float4 sSrc[kThreads];
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
   __shared__ uint2 sRec[kThreads];
-  __shared__ uint2 sKeys[kWaves * kChunk];
+  __shared__ uint2 sKeys[kWaves * chunk_for<K>()];
   __shared__ float sDiff[kWaves * kCacheLevels * 64];
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t wave = threadIdx.x >> 6;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == W
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
   const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
-                      sKeys + wave * kChunk, sDiff + wave * kCacheLevels * 64,
+                      sKeys + wave * chunk_for<K>(), sDiff + wave * kCacheLevels * 64,
                       sNext + wave * (kCacheLevels + 1) * 3};
   const LaneResult r =
       stochastic_estimate_sorted<D, K, COUNT>(t, spl, key0, key1, g, live, x, y, z, lds);

@@ -543,7 +543,13 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 // without reading the tree.  Only the order in which the samples'
 // contributions are summed differs from the per-sample loop.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kChunk = 256;        // samples drawn and sorted together
+constexpr uint32_t kChunk = 256;        // samples drawn and sorted together (at most)
+// The winding-number kernels stage normals too; half-size chunks keep their
+// LDS within seven workgroups per CU.
+template <int K>
+constexpr uint32_t chunk_for() {
+  return K == WINDING_NUMBER ? kChunk / 2 : kChunk;
+}
 constexpr uint32_t kCacheLevels = 6;    // path levels 1..6 cached
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
@@ -563,12 +569,13 @@ struct SortedLds {
 // Each lane takes up to kChunk / 128 of a stage's n / 2 compare-exchanges,
 // addressed by pair: pair p joins i = p with a 0 inserted at bit log2(j) and
 // i + j, and both slots are written back (no per-lane branch on the order).
+template <uint32_t CHUNK>
 __device__ __forceinline__ void wave_sort_keys(uint2* keys, uint32_t n) {
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 2; k <= n; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
 #pragma unroll
-      for (uint32_t m = 0; m < kChunk / 128; ++m) {
+      for (uint32_t m = 0; m < CHUNK / 128; ++m) {
         const uint32_t p = lane + 64 * m;
         if (p < n / 2) {
           const uint32_t i = (p & (j - 1)) | ((p & ~(j - 1)) << 1);
@@ -593,6 +600,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
 #pragma clang fp contract(off)
   constexpr bool EXACT = false;
   constexpr bool kWN = (K == WINDING_NUMBER);
+  constexpr uint32_t kCh = chunk_for<K>();
   const uint32_t lane = threadIdx.x & 63;
   const float4* __restrict__ nodeC = t.nodeC;
   const uint4* __restrict__ nodeL = t.nodeL;
@@ -755,14 +763,14 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     const float wd0 = level_weight<EXACT>((float)nS, 1.0f, (float)nS,
                                           r0sum - term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z));
     float sub_total = 0.0f;
-    for (int s0 = 0; s0 < spl; s0 += (int)kChunk) {
-      const uint32_t ncur = (uint32_t)min((int)kChunk, spl - s0);
+    for (int s0 = 0; s0 < spl; s0 += (int)kCh) {
+      const uint32_t ncur = (uint32_t)min((int)kCh, spl - s0);
       // draw the chunk's sampled points (Philox block 0 of each sample); the
       // level-1 roulette uses only the word's top 24 bits (rr_uniform), so
       // its low 8 bits carry the sample's index within the chunk
       static_assert(kChunk <= 256, "the sample index must fit in 8 bits");
 #pragma unroll
-      for (uint32_t m = 0; m < kChunk / 64; ++m) {
+      for (uint32_t m = 0; m < kCh / 64; ++m) {
         const uint32_t i = lane + 64 * m;
         uint2 key = make_uint2(0xFFFFFFFFu, 0u);
         if (i < ncur) {
@@ -776,7 +784,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       // sort the drawn samples (and sentinels up to the next power of two)
       uint32_t nsort = 1;
       while (nsort < ncur) nsort <<= 1;
-      wave_sort_keys(lds.keys, nsort);
+      wave_sort_keys<kCh>(lds.keys, nsort);
       for (uint32_t i = 0; i < ncur; ++i) {
         const uint2 key = lds.keys[i];
         const uint32_t idx = uniform_u32(key.x);
