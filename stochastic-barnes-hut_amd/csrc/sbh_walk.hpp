@@ -536,7 +536,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 // paths), and diff of the most recent node at each of the top kCacheLevels
 // levels is kept in LDS, keyed by node.  A sample whose path passes a cached
 // node reads it instead of evaluating the node's children (its opening
-// ratio is recomputed, which keeps the cache small enough for six levels at
+// ratio is recomputed, which keeps the cache small enough for seven levels at
 // seven wavefronts per SIMD).  Each cached node also records the step the
 // latest sample took from it (the chosen child's point range and the node
 // stepped to), so a sample whose point falls in the same child steps
@@ -550,7 +550,7 @@ template <int K>
 constexpr uint32_t chunk_for() {
   return K == WINDING_NUMBER ? kChunk / 2 : kChunk;
 }
-constexpr uint32_t kCacheLevels = 6;    // path levels 1..6 cached
+constexpr uint32_t kCacheLevels = 7;    // path levels 1..7 cached
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 // This wavefront's LDS for the fast estimator.
