@@ -184,6 +184,51 @@ int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* o
 int sbh_tree_stream(sbh_tree* tree, void** stream);
 
 /*
+ * Double build.  The reference's USE_DOUBLE configuration (include/utils.h:
+ * 13-29: FLOAT = double for every point, bound, moment, term and result,
+ * literals FP(x) = x, PI = M_PI), selected per tree instead of at compile
+ * time.  The tree is built from float64 inputs with the same partition,
+ * breadth-first numbering and leaf order rules in float64; evaluations take
+ * float64 queries and return float64 results, each term and sum with one
+ * rounding per operation in the reference's order.  Stochastic paths use the
+ * same Philox streams as the float build; the roulette uniform is the whole
+ * 32-bit word, ((w + 1) * 2^-32).  A double-build tree is evaluated by
+ * sbh_eval_double only (sbh_eval, sbh_eval_f64, sbh_query_order, tree images
+ * and the serving engine take float-build trees and refuse it).
+ */
+typedef struct sbh_eval_params_double {
+  double beta;                   /* Barnes-Hut opening ratio (1/theta); +inf = exact */
+  int32_t samples_per_subdomain; /* stochastic: samples per root child (>= 1) */
+  uint64_t seed;                 /* stochastic: Philox key */
+  int32_t vote_group;            /* Barnes-Hut: 1, 32 or 64 (as sbh_eval_params) */
+  int32_t flags;                 /* SBH_EVAL_DEVICE_POINTERS, _SPLIT_COUNTS,
+                                    _PER_QUERY_PATHS, _SMOOTH_DISTANCE;
+                                    _EXACT_TERMS is accepted and changes nothing */
+} sbh_eval_params_double;
+/* PyOctree / PyQuadtree + PyOctreeGPU / PyQuadtreeGPU of a USE_DOUBLE build
+ * (include/py_tree.h:30-74, include/py_tree_gpu.h:39-80); arguments as
+ * sbh_tree_create, in float64. */
+int sbh_tree_create_double(int32_t dim, int32_t split_log, int32_t leaf_size,
+                           int32_t max_depth, int64_t n, const double* points,
+                           const double* normals, const double* masses,
+                           const double* min_corner, const double* max_corner, double alpha,
+                           sbh_tree** out, float* build_ms);
+/* *is_double = 1 for a tree made by sbh_tree_create_double, else 0. */
+int sbh_tree_is_double(const sbh_tree* tree, int32_t* is_double);
+/* A double-build tree's float64 node arrays (breadth-first order; NULL or
+ * sized num_nodes, x dim for vectors).  The index arrays come from
+ * sbh_tree_nodes, whose float outputs must then be NULL (moments64 holds the
+ * same float64 moments). */
+int sbh_tree_nodes_double(const sbh_tree* tree, double* min_corner, double* max_corner,
+                          double* diameter, double* centroid, double* mass, double* normal);
+/* evalBruteForce_* / evalBarnesHut_* / multiLevelPrefixControlVariateSampler_*
+ * (src/sbh_gpu.cu:212-550) of a USE_DOUBLE build: results q doubles; counts
+ * and kernel_ms as sbh_eval.  Brute force is the reference's Kahan sum in
+ * tree order. */
+int sbh_eval_double(sbh_tree* tree, int32_t method, int32_t kernel,
+                    const sbh_eval_params_double* params, const double* queries, int64_t q,
+                    double* results, int32_t* counts, float* kernel_ms);
+/*
  * Tree images.  No counterpart in the reference (its trees exist only inside
  * the Python objects that built them, include/py_tree_gpu.h:23-81): a built
  * tree is written to a byte image (format version 2: header, every node and

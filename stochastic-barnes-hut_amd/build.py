@@ -1,5 +1,6 @@
 """Build script of the package: compiles libsbh_hip.so (HIP, gfx950) and the
-two pybind11 plugin modules `sbh` and `sbh_gpu` in place, next to this file.
+pybind11 plugin modules `sbh`, `sbh_gpu` and `sbh_gpu64` (the double build)
+in place, next to this file.
 
     python stochastic-barnes-hut_amd/build.py
 
@@ -19,7 +20,8 @@ CSRC = os.path.join(PKG, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["tree_build.hip", "eval.hip", "brute_mfma.hip", "serve.hip", "tree_io.hip", "capi.hip"]
+HIP_SOURCES = ["tree_build.hip", "eval.hip", "eval_double.hip", "brute_mfma.hip", "serve.hip",
+               "tree_io.hip", "capi.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared",
     # no packed-fp32 instructions in generated code
@@ -67,6 +69,7 @@ def build_all() -> list[str]:
     outs = [build_library()]
     outs.append(build_module("sbh", "py_sbh.cpp", link_lib=True))
     outs.append(build_module("sbh_gpu", "py_sbh_gpu.cpp", link_lib=True))
+    outs.append(build_module("sbh_gpu64", "py_sbh_gpu64.cpp", link_lib=True))
     return outs
 
 

@@ -29,6 +29,7 @@ struct sbh_tree {
   sbh::DevBuf<int32_t> counts;
   sbh::DevBuf<double> partial;
   sbh::DevBuf<double> out64;
+  sbh::DevBuf<double> qd;  // double build: queries
   sbh::DevBuf<uint32_t> order;
   sbh::DevBuf<uint32_t> sort_scratch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -71,10 +72,41 @@ void ensure(sbh::DevBuf<T>& b, size_t n) {
   if (b.size() < n) b.alloc(n);
 }
 
-bool all_finite(const float* p, size_t n) {
+template <typename T>
+bool all_finite(const T* p, size_t n) {
   for (size_t i = 0; i < n; ++i)
     if (!std::isfinite(p[i])) return false;
   return true;
+}
+
+void require_float_build(const sbh::DeviceTree& t, const char* what) {
+  SBH_REQUIRE(!t.is_double, std::string(what) +
+                                " takes a float-build tree; double-build trees are evaluated by "
+                                "sbh_eval_double");
+}
+
+// Shape and input checks shared by the float and the double build.
+template <typename T>
+void check_tree_inputs(int32_t dim, int32_t split_log, int32_t leaf_size, int32_t max_depth,
+                       int64_t n, const T* points, const T* normals, const T* masses,
+                       const T* min_corner, const T* max_corner, T alpha) {
+  SBH_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SBH_REQUIRE(split_log >= 1 && split_log <= 3, "split_log must be 1, 2 or 3");
+  SBH_REQUIRE(leaf_size >= 1, "leaf_size must be >= 1");
+  SBH_REQUIRE(max_depth >= 0 && max_depth <= 1000, "max_depth must be in [0, 1000]");
+  SBH_REQUIRE(n >= 1, "a tree needs at least one point");
+  SBH_REQUIRE(n < (int64_t)0x7FFFFFFF, "too many points (limit 2^31 - 1)");
+  SBH_REQUIRE(points && masses && min_corner && max_corner, "NULL input array");
+  SBH_REQUIRE(all_finite(points, (size_t)n * dim), "points must be finite");
+  SBH_REQUIRE(!normals || all_finite(normals, (size_t)n * dim), "normals must be finite");
+  for (int64_t i = 0; i < n; ++i)
+    SBH_REQUIRE(std::isfinite(masses[i]) && masses[i] > (T)0,
+                "masses must be finite and > 0 (node centroids divide by the mass sum)");
+  for (int a = 0; a < dim; ++a)
+    SBH_REQUIRE(std::isfinite(min_corner[a]) && std::isfinite(max_corner[a]) &&
+                    min_corner[a] < max_corner[a],
+                "min_corner must be finite and below max_corner on every axis");
+  SBH_REQUIRE(std::isfinite(alpha), "alpha must be finite");
 }
 
 }  // namespace
@@ -101,23 +133,8 @@ int sbh_tree_create(int32_t dim, int32_t split_log, int32_t leaf_size, int32_t m
   return guarded([&] {
     SBH_REQUIRE(out, "out is NULL");
     *out = nullptr;
-    SBH_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
-    SBH_REQUIRE(split_log >= 1 && split_log <= 3, "split_log must be 1, 2 or 3");
-    SBH_REQUIRE(leaf_size >= 1, "leaf_size must be >= 1");
-    SBH_REQUIRE(max_depth >= 0 && max_depth <= 1000, "max_depth must be in [0, 1000]");
-    SBH_REQUIRE(n >= 1, "a tree needs at least one point");
-    SBH_REQUIRE(n < (int64_t)0x7FFFFFFF, "too many points (limit 2^31 - 1)");
-    SBH_REQUIRE(points && masses && min_corner && max_corner, "NULL input array");
-    SBH_REQUIRE(all_finite(points, (size_t)n * dim), "points must be finite");
-    SBH_REQUIRE(!normals || all_finite(normals, (size_t)n * dim), "normals must be finite");
-    for (int64_t i = 0; i < n; ++i)
-      SBH_REQUIRE(std::isfinite(masses[i]) && masses[i] > 0.0f,
-                  "masses must be finite and > 0 (node centroids divide by the mass sum)");
-    for (int a = 0; a < dim; ++a)
-      SBH_REQUIRE(std::isfinite(min_corner[a]) && std::isfinite(max_corner[a]) &&
-                      min_corner[a] < max_corner[a],
-                  "min_corner must be finite and below max_corner on every axis");
-    SBH_REQUIRE(std::isfinite(alpha), "alpha must be finite");
+    check_tree_inputs<float>(dim, split_log, leaf_size, max_depth, n, points, normals, masses,
+                             min_corner, max_corner, alpha);
     require_device();
     sbh_tree* h = new sbh_tree();
     try {
@@ -142,6 +159,53 @@ int sbh_tree_create(int32_t dim, int32_t split_log, int32_t leaf_size, int32_t m
       throw;
     }
     *out = h;
+  });
+}
+
+int sbh_tree_create_double(int32_t dim, int32_t split_log, int32_t leaf_size,
+                           int32_t max_depth, int64_t n, const double* points,
+                           const double* normals, const double* masses,
+                           const double* min_corner, const double* max_corner, double alpha,
+                           sbh_tree** out, float* build_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    check_tree_inputs<double>(dim, split_log, leaf_size, max_depth, n, points, normals, masses,
+                              min_corner, max_corner, alpha);
+    require_device();
+    sbh_tree* h = new sbh_tree();
+    try {
+      sbh::DeviceTree& t = h->t;
+      t.is_double = true;
+      t.dim = dim;
+      t.split_log = split_log;
+      t.leaf_size = leaf_size;
+      t.max_depth = max_depth;
+      t.n = (uint32_t)n;
+      t.alpha64 = alpha;
+      t.alpha = (float)alpha;
+      for (int a = 0; a < dim; ++a) {
+        t.lo64[a] = min_corner[a];
+        t.hi64[a] = max_corner[a];
+        t.lo[a] = (float)min_corner[a];
+        t.hi[a] = (float)max_corner[a];
+      }
+      SBH_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+      SBH_HIP(hipEventCreate(&h->ev0));
+      SBH_HIP(hipEventCreate(&h->ev1));
+      sbh::build_device_tree_double(t, points, normals, masses, build_ms);
+    } catch (...) {
+      sbh_tree_destroy(h);
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int sbh_tree_is_double(const sbh_tree* tree, int32_t* is_double) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && is_double, "NULL argument");
+    *is_double = tree->t.is_double ? 1 : 0;
   });
 }
 
@@ -178,6 +242,7 @@ int sbh_tree_stream(sbh_tree* tree, void** stream) {
 int sbh_tree_serialize(const sbh_tree* tree, void* buffer, int64_t capacity, int64_t* size) {
   return guarded([&] {
     SBH_REQUIRE(tree && size, "NULL argument");
+    SBH_REQUIRE(!tree->t.is_double, "tree images hold float-build trees");
     std::vector<uint8_t> img;
     sbh::serialize_tree(tree->t, img);
     *size = (int64_t)img.size();
@@ -225,6 +290,7 @@ int sbh_tree_deserialize(const void* buffer, int64_t size, sbh_tree** out) {
 int sbh_tree_save(const sbh_tree* tree, const char* path) {
   return guarded([&] {
     SBH_REQUIRE(tree && path, "NULL argument");
+    SBH_REQUIRE(!tree->t.is_double, "tree images hold float-build trees");
     std::vector<uint8_t> img;
     sbh::serialize_tree(tree->t, img);
     FILE* f = std::fopen(path, "wb");
@@ -273,6 +339,10 @@ int sbh_tree_nodes(const sbh_tree* tree, uint32_t* start, uint32_t* end, uint32_
   return guarded([&] {
     SBH_REQUIRE(tree, "tree is NULL");
     const sbh::DeviceTree& t = tree->t;
+    SBH_REQUIRE(!t.is_double || !(min_corner || max_corner || diameter || centroid || mass ||
+                                  normal),
+                "a double-build tree's bounds, diameters and moments are float64: "
+                "sbh_tree_nodes_double");
     const size_t nn = t.num_nodes;
     const int d = t.dim;
     hipStream_t s = t.stream;
@@ -323,10 +393,50 @@ int sbh_tree_nodes(const sbh_tree* tree, uint32_t* start, uint32_t* end, uint32_
   });
 }
 
+int sbh_tree_nodes_double(const sbh_tree* tree, double* min_corner, double* max_corner,
+                          double* diameter, double* centroid, double* mass, double* normal) {
+  return guarded([&] {
+    SBH_REQUIRE(tree, "tree is NULL");
+    const sbh::DeviceTree& t = tree->t;
+    SBH_REQUIRE(t.is_double, "sbh_tree_nodes_double takes a double-build tree");
+    const size_t nn = t.num_nodes;
+    const int d = t.dim;
+    hipStream_t s = t.stream;
+    std::vector<sbh::dvec4> lo, hi, c, nr;
+    auto fetch = [&](std::vector<sbh::dvec4>& v, const sbh::dvec4* src) {
+      v.resize(nn);
+      SBH_HIP(hipMemcpyAsync(v.data(), src, nn * sizeof(sbh::dvec4), hipMemcpyDeviceToHost, s));
+    };
+    if (min_corner) fetch(lo, t.n_lo64.get());
+    if (max_corner) fetch(hi, t.n_hi64.get());
+    if (centroid || mass) fetch(c, t.nodeC64.get());
+    if (normal) fetch(nr, t.nodeN64.get());
+    if (diameter)
+      SBH_HIP(hipMemcpyAsync(diameter, t.n_diam64.get(), nn * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    SBH_HIP(hipStreamSynchronize(s));
+    auto put = [&](double* dst, const std::vector<sbh::dvec4>& v) {
+      if (!dst) return;
+      for (size_t i = 0; i < nn; ++i) {
+        dst[i * d] = v[i].x;
+        dst[i * d + 1] = v[i].y;
+        if (d == 3) dst[i * d + 2] = v[i].z;
+      }
+    };
+    put(min_corner, lo);
+    put(max_corner, hi);
+    put(centroid, c);
+    put(normal, nr);
+    if (mass)
+      for (size_t i = 0; i < nn; ++i) mass[i] = c[i].w;
+  });
+}
+
 int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* order,
                     int32_t flags) {
   return guarded([&] {
     SBH_REQUIRE(tree && queries && order, "NULL argument");
+    require_float_build(tree->t, "sbh_query_order");
     SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
     if (q == 0) return;
     sbh::DeviceTree& t = tree->t;
@@ -355,6 +465,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
              const float* queries, int64_t q, float* results, int32_t* counts, float* kernel_ms) {
   return guarded([&] {
     SBH_REQUIRE(tree && params, "NULL argument");
+    require_float_build(tree->t, "sbh_eval");
     SBH_REQUIRE(method == SBH_BRUTE_FORCE || method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
                 "unknown method");
     SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
@@ -464,6 +575,7 @@ int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q
                  double* results, int32_t flags, float* kernel_ms) {
   return guarded([&] {
     SBH_REQUIRE(tree, "tree is NULL");
+    require_float_build(tree->t, "sbh_eval_f64");
     SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
     SBH_REQUIRE(kernel != SBH_GRAVITY_GRAD_Z || tree->t.dim == 3,
                 "the gradient's z component needs a 3D tree");
@@ -498,6 +610,87 @@ int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q
   });
 }
 
+int sbh_eval_double(sbh_tree* tree, int32_t method, int32_t kernel,
+                    const sbh_eval_params_double* params, const double* queries, int64_t q,
+                    double* results, int32_t* counts, float* kernel_ms) {
+  return guarded([&] {
+    SBH_REQUIRE(tree && params, "NULL argument");
+    sbh::DeviceTree& t = tree->t;
+    SBH_REQUIRE(t.is_double, "sbh_eval_double takes a double-build tree (sbh_tree_create_double)");
+    SBH_REQUIRE(method == SBH_BRUTE_FORCE || method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
+                "unknown method");
+    SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
+    SBH_REQUIRE(kernel != SBH_GRAVITY_GRAD_Z || t.dim == 3,
+                "the gradient's z component needs a 3D tree");
+    SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
+    if (kernel_ms) *kernel_ms = 0.0f;
+    if (q == 0) return;
+    SBH_REQUIRE(queries && results, "NULL queries or results");
+    const sbh_eval_params_double& p = *params;
+    SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SPLIT_COUNTS |
+                             SBH_EVAL_EXACT_TERMS | SBH_EVAL_PER_QUERY_PATHS |
+                             SBH_EVAL_SMOOTH_DISTANCE)) == 0,
+                "unsupported eval flags for a double-build tree (query sorting and the "
+                "matrix-core forms are float-build options)");
+    SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
+                "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
+    if (method == SBH_BARNES_HUT) {
+      SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
+      SBH_REQUIRE(p.vote_group == 1 || p.vote_group == 32 || p.vote_group == 64,
+                  "vote_group must be 1, 32 or 64");
+    }
+    if (method == SBH_STOCHASTIC)
+      SBH_REQUIRE(p.samples_per_subdomain >= 1, "samples_per_subdomain must be >= 1");
+    hipStream_t s = t.stream;
+    const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
+    const int split = (p.flags & SBH_EVAL_SPLIT_COUNTS) ? 1 : 0;
+    const uint32_t nq = (uint32_t)q;
+    const size_t ncnt = split ? 2ull * nq : nq;
+    const size_t qvals = (size_t)nq * t.dim;
+    const double* dq = queries;
+    double* dout = results;
+    int32_t* dcnt = counts;
+    if (!dev) {
+      SBH_REQUIRE(all_finite(queries, qvals), "queries must be finite");
+      ensure(tree->qd, qvals);
+      ensure(tree->out64, nq);
+      SBH_HIP(hipMemcpyAsync(tree->qd.get(), queries, qvals * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+      dq = tree->qd.get();
+      dout = tree->out64.get();
+      if (counts) {
+        ensure(tree->counts, ncnt);
+        dcnt = tree->counts.get();
+      }
+    }
+    SBH_HIP(hipEventRecord(tree->ev0, s));
+    if (method == SBH_BRUTE_FORCE)
+      sbh::run_brute_double(t, kernel, dq, nq, dout, s);
+    else if (method == SBH_BARNES_HUT)
+      sbh::run_bh_double(t, kernel, p.beta, p.vote_group, dq, nq, dout, dcnt, split, s);
+    else
+      sbh::run_stochastic_double(t, kernel, p.samples_per_subdomain, p.seed,
+                                 (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, dout, dcnt,
+                                 split, s);
+    SBH_HIP(hipEventRecord(tree->ev1, s));
+    if (method == SBH_BRUTE_FORCE && dcnt) {
+      // every query evaluates every point; split counts have no node terms
+      std::vector<int32_t> c(ncnt, (int32_t)t.n);
+      if (split) std::fill(c.begin(), c.begin() + nq, 0);
+      SBH_HIP(hipMemcpyAsync(dcnt, c.data(), ncnt * 4, hipMemcpyHostToDevice, s));
+      SBH_HIP(hipStreamSynchronize(s));
+    }
+    if (p.flags & SBH_EVAL_SMOOTH_DISTANCE) sbh::run_smooth_distance_double(dout, nq, t.alpha64, s);
+    if (!dev) {
+      SBH_HIP(hipMemcpyAsync(results, dout, nq * sizeof(double), hipMemcpyDeviceToHost, s));
+      if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, ncnt * 4, hipMemcpyDeviceToHost, s));
+    }
+    SBH_HIP(hipEventSynchronize(tree->ev1));
+    if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));
+    SBH_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int sbh_server_create(int64_t budget_bytes, int32_t max_inflight, sbh_server** out) {
   return guarded([&] {
     SBH_REQUIRE(out, "out is NULL");
@@ -517,6 +710,7 @@ int sbh_server_destroy(sbh_server* server) {
 int sbh_server_add_tree(sbh_server* server, sbh_tree* tree, int32_t* tree_id) {
   return guarded([&] {
     SBH_REQUIRE(server && tree && tree_id, "NULL argument");
+    SBH_REQUIRE(!tree->t.is_double, "the serving engine holds float-build trees");
     *tree_id = server->s->add_tree(&tree->t);
   });
 }

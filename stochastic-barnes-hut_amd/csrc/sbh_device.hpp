@@ -264,6 +264,83 @@ __device__ __forceinline__ double kernel_f64(float4 src, float4 nrm, float alpha
 }
 
 // ---------------------------------------------------------------------------
+// Double build: the reference compiled with USE_DOUBLE (include/utils.h:13-29,
+// FLOAT = double, literals FP(x) = x, PI = M_PI).  Points, cell bounds,
+// moments, terms, opening ratios, roulette and sums in float64, one rounding
+// per operation in the reference's order (no contraction).
+// ---------------------------------------------------------------------------
+struct alignas(32) dvec4 {
+  double x, y, z, w;
+};
+constexpr double kPiD = 3.14159265358979323846;
+
+template <int D>
+__device__ __forceinline__ double sqnorm_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  if constexpr (D == 3) {
+    return dx * dx + (dy * dy + dz * dz);
+  } else {
+    return dx * dx + dy * dy;
+  }
+}
+
+// include/kernel.h:31-83 in float64; src = {x, y, z, m}, nrm = {nx, ny, nz, *}
+template <int D, int K>
+__device__ __forceinline__ double kernel_double(const dvec4& src, const dvec4& nrm, double alpha,
+                                                double qx, double qy, double qz) {
+#pragma clang fp contract(off)
+  double dx = src.x - qx;
+  double dy = src.y - qy;
+  double dz = (D == 3) ? (src.z - qz) : 0.0;
+  if constexpr (K == GRAVITY) {
+    double dist = __builtin_sqrt(sqnorm_exact<D>(dx, dy, dz));
+    return -src.w / (dist + 1e-1);
+  } else if constexpr (K == WINDING_NUMBER) {
+    if constexpr (D == 3) {
+      double dist = __builtin_sqrt(sqnorm_exact<3>(dx, dy, dz));
+      double qdn = nrm.x * dx + (nrm.y * dy + nrm.z * dz);
+      return src.w * qdn / (4.0 * kPiD * (dist * dist * dist + 1e-5));
+    } else {
+      double dsq = sqnorm_exact<2>(dx, dy, dz);
+      double qdn = nrm.x * dx + nrm.y * dy;
+      return src.w * qdn / (2.0 * kPiD * (dsq + 1e-5));
+    }
+  } else if constexpr (is_helmholtz(K)) {
+    double dist = __builtin_sqrt(sqnorm_exact<D>(dx, dy, dz));
+    const double freq = (K == HELMHOLTZ_X) ? 2.0 : 1.0;
+    double arg = freq * 2.0 * kPiD * dist;
+    double c = (K == HELMHOLTZ_X) ? cos(arg) : sin(arg);
+    return src.w * c / (dist + 1e-1);
+  } else if constexpr (is_gradient(K)) {
+    double dist = __builtin_sqrt(sqnorm_exact<D>(dx, dy, dz));
+    double t = dist + 1e-1;
+    double s = -src.w / (t * t * t);
+    double comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
+    return s * comp;
+  } else {
+    double dist = __builtin_sqrt(sqnorm_exact<D>(dx, dy, dz));
+    return exp(-alpha * dist);
+  }
+}
+
+// Opening ratio ||q - c|| / diam (include/tree3d.h:134-140) in float64.
+template <int D>
+__device__ __forceinline__ double ratio_double(const dvec4& c, double diam, double qx, double qy,
+                                               double qz) {
+#pragma clang fp contract(off)
+  double dx = qx - c.x;
+  double dy = qy - c.y;
+  double dz = (D == 3) ? (qz - c.z) : 0.0;
+  return __builtin_sqrt(sqnorm_exact<D>(dx, dy, dz)) / diam;
+}
+
+// Russian-roulette uniform in (0, 1] of the double build: all 32 bits of the
+// word (the float build keeps the top 24).
+__device__ __forceinline__ double rr_uniform_double(uint32_t w) {
+  return ((double)w + 1.0) * 0x1p-32;
+}
+
+// ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel Random Numbers: As
 // Easy as 1, 2, 3", SC'11).  10 rounds, key bumped by the Weyl constants
 // between rounds.

@@ -46,6 +46,17 @@ void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, boo
 // Smooth distance from its sum, in place: w <- -log(w) / alpha.
 void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s);
 
+// Double-build trees (eval_double.hip): brute force (Kahan, tree order),
+// Barnes-Hut (vote group 1, 32 or 64) and the stochastic estimator in float64.
+void run_brute_double(const DeviceTree& t, int kernel, const double* q, uint32_t nq, double* out,
+                      hipStream_t s);
+void run_bh_double(const DeviceTree& t, int kernel, double beta, int group, const double* q,
+                   uint32_t nq, double* out, int32_t* counts, int split, hipStream_t s);
+void run_stochastic_double(const DeviceTree& t, int kernel, int spl, uint64_t seed,
+                           bool per_query, const double* q, uint32_t nq, double* out,
+                           int32_t* counts, int split, hipStream_t s);
+void run_smooth_distance_double(double* w, uint32_t n, double alpha, hipStream_t s);
+
 // Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)
 // uint32 values.
 size_t sort_scratch_u32(uint32_t nq);

@@ -151,13 +151,29 @@ struct DeviceTree {
   DevBuf<WalkRecord> bfsR;
   DevBuf<uint4> stepR;
 
+  // Double build (the reference's USE_DOUBLE configuration): points, cell
+  // bounds, diameters and moments in float64 instead of the float arrays
+  // above; the index arrays (ranges, children, ropes, n_skip, perm) and the
+  // float64 moments n_m64 are shared.  A double tree has no depth-first or
+  // derived records and no fp32 point / node arrays.
+  bool is_double = false;
+  double alpha64 = 1.0;
+  double lo64[3] = {0, 0, 0};
+  double hi64[3] = {0, 0, 0};
+  DevBuf<dvec4> pts64, nrm64;      // {x, y, z, mass}, {nx, ny, nz, 0}
+  DevBuf<dvec4> n_lo64, n_hi64;    // cell bounds
+  DevBuf<double> n_diam64;
+  DevBuf<dvec4> nodeC64, nodeN64;  // {cx, cy, cz, mass}, {nx, ny, nz, 0}
+
   size_t device_bytes() const {
     return pts.bytes() + nrm.bytes() + perm.bytes() + n_start.bytes() + n_end.bytes() +
            n_parent.bytes() + n_first.bytes() + n_nchild.bytes() + n_next.bytes() +
            n_depth.bytes() + n_lo.bytes() + n_hi.bytes() + n_diam.bytes() + nodeC.bytes() +
            nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
            n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
-           kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes();
+           kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes() + pts64.bytes() +
+           nrm64.bytes() + n_lo64.bytes() + n_hi64.bytes() + n_diam64.bytes() +
+           nodeC64.bytes() + nodeN64.bytes();
   }
 };
 
@@ -165,6 +181,9 @@ struct DeviceTree {
 // Host pointers; blocks until the build is complete.
 void build_device_tree(DeviceTree& t, const float* points, const float* normals,
                        const float* masses, float* build_ms);
+// The double build of the same tree (t.is_double set by the caller).
+void build_device_tree_double(DeviceTree& t, const double* points, const double* normals,
+                              const double* masses, float* build_ms);
 
 // Byte image of a built tree (tree_io.hip), and the tree back from one: the
 // image is checked (checksum, sizes, and every index the walks follow) before

@@ -607,7 +607,6 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   const float4* __restrict__ nodeN = t.nodeN;
   const uint32_t* __restrict__ n_start = t.n_start;
   const uint32_t* __restrict__ n_end = t.n_end;
-  const uint32_t* __restrict__ n_skip = t.n_skip;
   const float4* __restrict__ pts = t.pts;
   const float4* __restrict__ nrm = t.nrm;
   // read-only for the whole launch: wave-uniform reads become scalar loads

@@ -32,16 +32,41 @@ constexpr int kChunk = 1024;
 constexpr int kChunkThreads = 256;
 constexpr int kMaxCells = 512;  // 3D with split_log 3
 
-// float -> int with x86 cvttss2si semantics (out of range / NaN -> INT_MIN),
-// the conversion behind the reference's cast<int>() (include/aabb.h:34-41).
+// Point / bound types of the float build and of the double build.
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<float> {
+  using type = float4;
+};
+template <>
+struct VecOf<double> {
+  using type = dvec4;
+};
+__host__ __device__ inline float4 make_vec(float x, float y, float z, float w) {
+  return make_float4(x, y, z, w);
+}
+__host__ __device__ inline dvec4 make_vec(double x, double y, double z, double w) {
+  return dvec4{x, y, z, w};
+}
+__device__ __forceinline__ float sqrt_rn(float v) { return __builtin_sqrtf(v); }
+__device__ __forceinline__ double sqrt_rn(double v) { return __builtin_sqrt(v); }
+
+// float / double -> int with x86 cvttss2si / cvttsd2si semantics (out of
+// range / NaN -> INT_MIN), the conversion behind the reference's cast<int>()
+// (include/aabb.h:34-41).
 __device__ __forceinline__ int trunc_x86(float v) {
   return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : INT_MIN;
 }
+__device__ __forceinline__ int trunc_x86(double v) {
+  return (v >= -2147483648.0 && v < 2147483648.0) ? (int)v : INT_MIN;
+}
 
-__device__ __forceinline__ int quant_axis(float p, float lo, float hi, int dsplit) {
+template <typename T>
+__device__ __forceinline__ int quant_axis(T p, T lo, T hi, int dsplit) {
 #pragma clang fp contract(off)
-  float num = (float)dsplit * (p - lo);
-  float den = hi - lo;
+  T num = (T)dsplit * (p - lo);
+  T den = hi - lo;
   int v = trunc_x86(num / den);
   v = v < dsplit - 1 ? v : dsplit - 1;
   v = v > 0 ? v : 0;
@@ -50,8 +75,8 @@ __device__ __forceinline__ int quant_axis(float p, float lo, float hi, int dspli
 
 // Child cell of point p in box [lo, hi): raster index ((z << k) + y) << k + x
 // (include/tree3d.h:62-65).
-template <int D>
-__device__ __forceinline__ uint32_t cell_of(float4 p, float4 lo, float4 hi, int k) {
+template <int D, typename V>
+__device__ __forceinline__ uint32_t cell_of(V p, V lo, V hi, int k) {
   int ds = 1 << k;
   uint32_t x = quant_axis(p.x, lo.x, hi.x, ds);
   uint32_t y = quant_axis(p.y, lo.y, hi.y, ds);
@@ -64,29 +89,29 @@ __device__ __forceinline__ uint32_t cell_of(float4 p, float4 lo, float4 hi, int 
 }
 
 // Sub-box of cell `cell` (include/tree3d.h:269-280): min + coord * inc, + inc.
-template <int D>
-__device__ __forceinline__ void cell_box(float4 lo, float4 hi, int k, uint32_t cell, float4& clo,
-                                         float4& chi, float& diam) {
+template <int D, typename V, typename T>
+__device__ __forceinline__ void cell_box(V lo, V hi, int k, uint32_t cell, V& clo, V& chi,
+                                         T& diam) {
 #pragma clang fp contract(off)
   int ds = 1 << k;
-  float fds = (float)ds;
+  T fds = (T)ds;
   uint32_t mask = (uint32_t)ds - 1;
-  float ix = (float)(cell & mask);
-  float iy = (float)((cell >> k) & mask);
-  float iz = (float)((cell >> (2 * k)) & mask);
-  float incx = (hi.x - lo.x) / fds;
-  float incy = (hi.y - lo.y) / fds;
-  float incz = (D == 3) ? (hi.z - lo.z) / fds : 0.0f;
+  T ix = (T)(cell & mask);
+  T iy = (T)((cell >> k) & mask);
+  T iz = (T)((cell >> (2 * k)) & mask);
+  T incx = (hi.x - lo.x) / fds;
+  T incy = (hi.y - lo.y) / fds;
+  T incz = (D == 3) ? (hi.z - lo.z) / fds : (T)0;
   clo.x = lo.x + ix * incx;
   clo.y = lo.y + iy * incy;
-  clo.z = (D == 3) ? lo.z + iz * incz : 0.0f;
-  clo.w = 0.0f;
+  clo.z = (D == 3) ? lo.z + iz * incz : (T)0;
+  clo.w = (T)0;
   chi.x = clo.x + incx;
   chi.y = clo.y + incy;
-  chi.z = (D == 3) ? clo.z + incz : 0.0f;
-  chi.w = 0.0f;
-  float dx = chi.x - clo.x, dy = chi.y - clo.y, dz = chi.z - clo.z;
-  diam = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));
+  chi.z = (D == 3) ? clo.z + incz : (T)0;
+  chi.w = (T)0;
+  T dx = chi.x - clo.x, dy = chi.y - clo.y, dz = chi.z - clo.z;
+  diam = sqrt_rn(sqnorm_exact<D>(dx, dy, dz));
 }
 
 // Exclusive scan of one value per thread over a block of NT threads.
@@ -181,10 +206,10 @@ __device__ uint32_t rank_zone(const uint16_t* dig, uint32_t a, uint32_t b, uint3
 }
 
 // One workgroup per internal node of at most kSmallMax points.
-template <int D>
+template <int D, typename V>
 __global__ __launch_bounds__(kSmallThreads) void k_partition_small(
-    const uint32_t* list, const uint32_t* n_start, const uint32_t* n_end, const float4* n_lo,
-    const float4* n_hi, uint32_t* perm, const float4* src_pts, int k, uint32_t* n_nchild,
+    const uint32_t* list, const uint32_t* n_start, const uint32_t* n_end, const V* n_lo,
+    const V* n_hi, uint32_t* perm, const V* src_pts, int k, uint32_t* n_nchild,
     uint4* temp_child) {
   __shared__ uint32_t lperm[kSmallMax];
   __shared__ uint16_t ldig[kSmallMax];
@@ -197,7 +222,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_partition_small(
   const uint32_t node = list[blockIdx.x];
   const uint32_t s0 = n_start[node];
   const uint32_t n = n_end[node] - s0;
-  const float4 lo = n_lo[node], hi = n_hi[node];
+  const V lo = n_lo[node], hi = n_hi[node];
   const uint32_t B = 1u << (D * k);
 
   for (uint32_t i = tid; i < B; i += kSmallThreads) hist[i] = 0;
@@ -259,17 +284,17 @@ struct LargeCtx {
   uint32_t* Rg;          // [N]
 };
 
-template <int D>
-__global__ __launch_bounds__(kChunkThreads) void k_large_digits(LargeCtx cx, const float4* n_lo,
-                                                                 const float4* n_hi,
+template <int D, typename V>
+__global__ __launch_bounds__(kChunkThreads) void k_large_digits(LargeCtx cx, const V* n_lo,
+                                                                 const V* n_hi,
                                                                  const uint32_t* perm,
-                                                                 const float4* src_pts, int k) {
+                                                                 const V* src_pts, int k) {
   __shared__ uint32_t lh[kMaxCells];
   const uint4 ch = cx.chunks[blockIdx.x];
   const uint32_t B = 1u << (D * k);
   for (uint32_t i = threadIdx.x; i < B; i += kChunkThreads) lh[i] = 0;
   __syncthreads();
-  const float4 lo = n_lo[ch.w], hi = n_hi[ch.w];
+  const V lo = n_lo[ch.w], hi = n_hi[ch.w];
   for (uint32_t p = ch.y + threadIdx.x; p < ch.z; p += kChunkThreads) {
     uint32_t d = cell_of<D>(src_pts[perm[p]], lo, hi, k);
     cx.digit[p] = (uint16_t)d;
@@ -397,11 +422,11 @@ __global__ __launch_bounds__(kChunkThreads) void k_large_swap(LargeCtx cx, uint3
 
 // Children of the level's internal nodes get consecutive breadth-first
 // numbers: node order, then cell order (include/tree3d.h:400-411).
-template <int D>
+template <int D, typename V, typename T>
 __global__ void k_emit(uint32_t lb, uint32_t count, uint32_t next_base, const uint32_t* cbase, int k,
                        uint32_t* n_start, uint32_t* n_end, uint32_t* n_parent, uint32_t* n_first,
-                       const uint32_t* n_nchild, uint32_t* n_next, int32_t* n_depth, float4* n_lo,
-                       float4* n_hi, float* n_diam, const uint4* temp_child) {
+                       const uint32_t* n_nchild, uint32_t* n_next, int32_t* n_depth, V* n_lo,
+                       V* n_hi, T* n_diam, const uint4* temp_child) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   uint32_t v = lb + i;
@@ -409,7 +434,7 @@ __global__ void k_emit(uint32_t lb, uint32_t count, uint32_t next_base, const ui
   if (nch == 0) return;
   uint32_t base = next_base + cbase[i];
   n_first[v] = base;
-  float4 lo = n_lo[v], hi = n_hi[v];
+  V lo = n_lo[v], hi = n_hi[v];
   int32_t dep = n_depth[v] + 1;
   uint32_t vnext = n_next[v];
   uint32_t s0 = n_start[v];
@@ -420,8 +445,8 @@ __global__ void k_emit(uint32_t lb, uint32_t count, uint32_t next_base, const ui
     n_end[cidx] = tc.z;
     n_parent[cidx] = v;
     n_depth[cidx] = dep;
-    float4 clo, chi;
-    float diam;
+    V clo, chi;
+    T diam;
     cell_box<D>(lo, hi, k, tc.x, clo, chi, diam);
     n_lo[cidx] = clo;
     n_hi[cidx] = chi;
@@ -430,8 +455,9 @@ __global__ void k_emit(uint32_t lb, uint32_t count, uint32_t next_base, const ui
   }
 }
 
-__global__ void k_gather_points(uint32_t n, const uint32_t* perm, const float4* src_pts,
-                                const float4* src_nrm, float4* pts, float4* nrm) {
+template <typename V>
+__global__ void k_gather_points(uint32_t n, const uint32_t* perm, const V* src_pts,
+                                const V* src_nrm, V* pts, V* nrm) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t j = perm[i];
@@ -527,14 +553,83 @@ __global__ void k_moments(uint32_t lb, uint32_t count, const uint32_t* n_start,
   }
 }
 
+// Node moments of the double build: the fp32 recurrence above in float64
+// (include/tree3d.h:255-264, :310-327 with FLOAT = double).  The same values
+// go to n_m64, the float64 moments every tree exports.
+template <int D>
+__global__ void k_moments_double(uint32_t lb, uint32_t count, const uint32_t* n_start,
+                                 const uint32_t* n_end, const uint32_t* n_first,
+                                 const uint32_t* n_nchild, const dvec4* pts, const dvec4* nrm,
+                                 dvec4* nodeC, dvec4* nodeN, double* m64, uint32_t* n_skip,
+                                 uint32_t* n_size) {
+#pragma clang fp contract(off)
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t v = lb + i;
+  uint32_t nch = n_nchild[v];
+  double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
+  if (nch == 0) {
+    for (uint32_t j = n_start[v]; j < n_end[v]; ++j) {
+      const dvec4 p = pts[j];
+      const dvec4 q = nrm[j];
+      mass += p.w;
+      cx += p.w * p.x;
+      cy += p.w * p.y;
+      cz += p.w * p.z;
+      nx += p.w * q.x;
+      ny += p.w * q.y;
+      nz += p.w * q.z;
+    }
+    n_skip[v] = v;
+    n_size[v] = 1;
+  } else {
+    uint32_t f = n_first[v];
+    for (uint32_t j = 0; j < nch; ++j) {
+      const dvec4 c = nodeC[f + j];
+      const dvec4 q = nodeN[f + j];
+      mass += c.w;
+      cx += c.w * c.x;
+      cy += c.w * c.y;
+      cz += c.w * c.z;
+      nx += c.w * q.x;
+      ny += c.w * q.y;
+      nz += c.w * q.z;
+    }
+    n_skip[v] = (nch == 1) ? n_skip[f] : v;
+    uint32_t sz = 1;
+    for (uint32_t j = 0; j < nch; ++j) sz += n_size[f + j];
+    n_size[v] = sz;
+  }
+  cx /= mass;
+  cy /= mass;
+  cz /= mass;
+  nx /= mass;
+  ny /= mass;
+  nz /= mass;
+  nodeC[v] = dvec4{cx, cy, cz, mass};
+  nodeN[v] = dvec4{nx, ny, nz, 0.0};
+  double* o = m64 + 8ull * v;
+  o[0] = cx;
+  o[1] = cy;
+  o[2] = cz;
+  o[3] = mass;
+  o[4] = nx;
+  o[5] = ny;
+  o[6] = nz;
+  o[7] = 0.0;
+}
+
+// The walk record {diameter bits, first child | first point, rope, info}; a
+// double tree keeps its diameters in n_diam64 and records their fp32 rounding.
+template <typename T>
 __global__ void k_finalize(uint32_t nn, const uint32_t* n_start, const uint32_t* n_end,
                            const uint32_t* n_first, const uint32_t* n_nchild,
-                           const uint32_t* n_next, const float* n_diam, uint4* nodeL) {
+                           const uint32_t* n_next, const T* n_diam, uint4* nodeL) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   uint32_t nch = n_nchild[v];
   uint4 r;
-  r.x = __float_as_uint(n_diam[v]);
+  r.x = __float_as_uint((float)n_diam[v]);
   r.z = n_next[v];
   if (nch == 0) {
     r.y = n_start[v];
@@ -609,13 +704,13 @@ __global__ void k_dfs_scatter(uint32_t nn, const uint32_t* n_dfs, const uint32_t
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
-template <int D>
-__global__ void k_set_root(uint32_t n, float4 lo, float4 hi, uint32_t* n_start, uint32_t* n_end,
-                           uint32_t* n_parent, int32_t* n_depth, uint32_t* n_next, float4* n_lo,
-                           float4* n_hi, float* n_diam) {
+template <int D, typename V, typename T>
+__global__ void k_set_root(uint32_t n, V lo, V hi, uint32_t* n_start, uint32_t* n_end,
+                           uint32_t* n_parent, int32_t* n_depth, uint32_t* n_next, V* n_lo,
+                           V* n_hi, T* n_diam) {
 #pragma clang fp contract(off)
-  float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
-  float diam = __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz));  // include/tree3d.h:249
+  T dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  T diam = sqrt_rn(sqnorm_exact<D>(dx, dy, dz));  // include/tree3d.h:249
   n_start[0] = 0;
   n_end[0] = n;
   n_parent[0] = 0;
@@ -693,25 +788,55 @@ void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
-template <int D>
-void build_impl(DeviceTree& t, const float* points, const float* normals, const float* masses) {
+// The arrays a build of scalar type T writes: the fp32 set, or the float64
+// set of the double build.
+template <typename T>
+struct BuildArrays;
+template <>
+struct BuildArrays<float> {
+  static DevBuf<float4>& lo(DeviceTree& t) { return t.n_lo; }
+  static DevBuf<float4>& hi(DeviceTree& t) { return t.n_hi; }
+  static DevBuf<float>& diam(DeviceTree& t) { return t.n_diam; }
+  static DevBuf<float4>& pts(DeviceTree& t) { return t.pts; }
+  static DevBuf<float4>& nrm(DeviceTree& t) { return t.nrm; }
+  static const float* root_lo(const DeviceTree& t) { return t.lo; }
+  static const float* root_hi(const DeviceTree& t) { return t.hi; }
+};
+template <>
+struct BuildArrays<double> {
+  static DevBuf<dvec4>& lo(DeviceTree& t) { return t.n_lo64; }
+  static DevBuf<dvec4>& hi(DeviceTree& t) { return t.n_hi64; }
+  static DevBuf<double>& diam(DeviceTree& t) { return t.n_diam64; }
+  static DevBuf<dvec4>& pts(DeviceTree& t) { return t.pts64; }
+  static DevBuf<dvec4>& nrm(DeviceTree& t) { return t.nrm64; }
+  static const double* root_lo(const DeviceTree& t) { return t.lo64; }
+  static const double* root_hi(const DeviceTree& t) { return t.hi64; }
+};
+
+template <int D, typename T>
+void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masses) {
+  using V = typename VecOf<T>::type;
+  using A = BuildArrays<T>;
   hipStream_t s = t.stream;
   const uint32_t N = t.n;
   const int k = t.split_log;
   const uint32_t B = 1u << (D * k);
+  DevBuf<V>& n_lo = A::lo(t);
+  DevBuf<V>& n_hi = A::hi(t);
+  DevBuf<T>& n_diam = A::diam(t);
 
   // Inputs as {x, y, z, m} / {nx, ny, nz, 0} rows in input order.
-  std::vector<float4> hp(N), hn(N);
+  std::vector<V> hp(N), hn(N);
   for (uint32_t i = 0; i < N; ++i) {
-    hp[i] = make_float4(points[(size_t)i * D], points[(size_t)i * D + 1],
-                        D == 3 ? points[(size_t)i * D + 2] : 0.f, masses[i]);
+    hp[i] = make_vec(points[(size_t)i * D], points[(size_t)i * D + 1],
+                     D == 3 ? points[(size_t)i * D + 2] : (T)0, masses[i]);
     if (normals)
-      hn[i] = make_float4(normals[(size_t)i * D], normals[(size_t)i * D + 1],
-                          D == 3 ? normals[(size_t)i * D + 2] : 0.f, 0.f);
+      hn[i] = make_vec(normals[(size_t)i * D], normals[(size_t)i * D + 1],
+                       D == 3 ? normals[(size_t)i * D + 2] : (T)0, (T)0);
     else
-      hn[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      hn[i] = make_vec((T)0, (T)0, (T)0, (T)0);
   }
-  DevBuf<float4> src_pts(N), src_nrm(N);
+  DevBuf<V> src_pts(N), src_nrm(N);
   h2d(src_pts.get(), hp.data(), N, s);
   h2d(src_nrm.get(), hn.data(), N, s);
   std::vector<uint32_t> iota(N);
@@ -728,9 +853,9 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
   t.n_nchild.alloc(cap);
   t.n_next.alloc(cap);
   t.n_depth.alloc(cap);
-  t.n_lo.alloc(cap);
-  t.n_hi.alloc(cap);
-  t.n_diam.alloc(cap);
+  n_lo.alloc(cap);
+  n_hi.alloc(cap);
+  n_diam.alloc(cap);
 
   auto grow_all = [&](size_t need, size_t keep) {
     if (need <= cap) return;
@@ -742,9 +867,9 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
     t.n_nchild.grow(nc, keep, s);
     t.n_next.grow(nc, keep, s);
     t.n_depth.grow(nc, keep, s);
-    t.n_lo.grow(nc, keep, s);
-    t.n_hi.grow(nc, keep, s);
-    t.n_diam.grow(nc, keep, s);
+    n_lo.grow(nc, keep, s);
+    n_hi.grow(nc, keep, s);
+    n_diam.grow(nc, keep, s);
     cap = nc;
   };
 
@@ -756,11 +881,13 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
   DevBuf<uint16_t> digit(N + 1);
   DevBuf<uint32_t> Lg(N + 1), Rg(N + 1);
 
-  float4 rlo = make_float4(t.lo[0], t.lo[1], D == 3 ? t.lo[2] : 0.f, 0.f);
-  float4 rhi = make_float4(t.hi[0], t.hi[1], D == 3 ? t.hi[2] : 0.f, 0.f);
+  const T* tlo = A::root_lo(t);
+  const T* thi = A::root_hi(t);
+  const V rlo = make_vec(tlo[0], tlo[1], D == 3 ? tlo[2] : (T)0, (T)0);
+  const V rhi = make_vec(thi[0], thi[1], D == 3 ? thi[2] : (T)0, (T)0);
   k_set_root<D><<<1, 1, 0, s>>>(N, rlo, rhi, t.n_start.get(), t.n_end.get(), t.n_parent.get(),
-                                t.n_depth.get(), t.n_next.get(), t.n_lo.get(), t.n_hi.get(),
-                                t.n_diam.get());
+                                t.n_depth.get(), t.n_next.get(), n_lo.get(), n_hi.get(),
+                                n_diam.get());
   SBH_HIP(hipGetLastError());
 
   t.level_begin.clear();
@@ -826,7 +953,7 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
       cx.digit = digit.get();
       cx.Lg = Lg.get();
       cx.Rg = Rg.get();
-      k_large_digits<D><<<nchunks, kChunkThreads, 0, s>>>(cx, t.n_lo.get(), t.n_hi.get(),
+      k_large_digits<D><<<nchunks, kChunkThreads, 0, s>>>(cx, n_lo.get(), n_hi.get(),
                                                           t.perm.get(), src_pts.get(), k);
       SBH_HIP(hipGetLastError());
       k_large_cells<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, large_list.get(), nlarge, B,
@@ -843,7 +970,7 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
     }
     if (nsmall > 0) {
       k_partition_small<D><<<nsmall, kSmallThreads, 0, s>>>(
-          small_list.get(), t.n_start.get(), t.n_end.get(), t.n_lo.get(), t.n_hi.get(),
+          small_list.get(), t.n_start.get(), t.n_end.get(), n_lo.get(), n_hi.get(),
           t.perm.get(), src_pts.get(), k, t.n_nchild.get(), temp_child.get());
       SBH_HIP(hipGetLastError());
     }
@@ -854,8 +981,8 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
     grow_all((size_t)num_nodes + nnext, num_nodes);
     k_emit<D><<<blocks_for(lcount, 128), 128, 0, s>>>(
         lb, lcount, num_nodes, cbase.get(), k, t.n_start.get(), t.n_end.get(), t.n_parent.get(),
-        t.n_first.get(), t.n_nchild.get(), t.n_next.get(), t.n_depth.get(), t.n_lo.get(),
-        t.n_hi.get(), t.n_diam.get(), temp_child.get());
+        t.n_first.get(), t.n_nchild.get(), t.n_next.get(), t.n_depth.get(), n_lo.get(),
+        n_hi.get(), n_diam.get(), temp_child.get());
     SBH_HIP(hipGetLastError());
     lb = num_nodes;
     lcount = nnext;
@@ -866,11 +993,36 @@ void build_impl(DeviceTree& t, const float* points, const float* normals, const 
   t.depth = (int)t.level_begin.size() - 2;
 
   // Points in leaf order.
-  t.pts.alloc(N);
-  t.nrm.alloc(N);
+  DevBuf<V>& pts = A::pts(t);
+  DevBuf<V>& nrm = A::nrm(t);
+  pts.alloc(N);
+  nrm.alloc(N);
   k_gather_points<<<blocks_for(N, 256), 256, 0, s>>>(N, t.perm.get(), src_pts.get(),
-                                                     src_nrm.get(), t.pts.get(), t.nrm.get());
+                                                     src_nrm.get(), pts.get(), nrm.get());
   SBH_HIP(hipGetLastError());
+  if constexpr (sizeof(T) == 8) {
+    // Moments of the double build, deepest level first, and the walk records.
+    t.nodeC64.alloc(num_nodes);
+    t.nodeN64.alloc(num_nodes);
+    t.nodeL.alloc(num_nodes);
+    t.n_m64.alloc((size_t)num_nodes * 8);
+    t.n_skip.alloc(num_nodes);
+    t.n_size.alloc(num_nodes);
+    for (int l = t.depth; l >= 0; --l) {
+      uint32_t b = t.level_begin[l], cnt = t.level_begin[l + 1] - b;
+      k_moments_double<D><<<blocks_for(cnt, 128), 128, 0, s>>>(
+          b, cnt, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(),
+          pts.get(), nrm.get(), t.nodeC64.get(), t.nodeN64.get(), t.n_m64.get(),
+          t.n_skip.get(), t.n_size.get());
+      SBH_HIP(hipGetLastError());
+    }
+    k_finalize<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
+        num_nodes, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(),
+        t.n_next.get(), n_diam.get(), t.nodeL.get());
+    SBH_HIP(hipGetLastError());
+    SBH_HIP(hipStreamSynchronize(s));
+    return;
+  }
 
   // Moments, deepest level first.
   t.nodeC.alloc(num_nodes);
@@ -940,9 +1092,20 @@ void build_device_tree(DeviceTree& t, const float* points, const float* normals,
                        const float* masses, float* build_ms) {
   auto t0 = std::chrono::steady_clock::now();
   if (t.dim == 2)
-    build_impl<2>(t, points, normals, masses);
+    build_impl<2, float>(t, points, normals, masses);
   else
-    build_impl<3>(t, points, normals, masses);
+    build_impl<3, float>(t, points, normals, masses);
+  auto t1 = std::chrono::steady_clock::now();
+  if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+}
+
+void build_device_tree_double(DeviceTree& t, const double* points, const double* normals,
+                              const double* masses, float* build_ms) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (t.dim == 2)
+    build_impl<2, double>(t, points, normals, masses);
+  else
+    build_impl<3, double>(t, points, normals, masses);
   auto t1 = std::chrono::steady_clock::now();
   if (build_ms) *build_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
 }

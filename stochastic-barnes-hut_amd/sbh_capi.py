@@ -35,6 +35,13 @@ class EvalParams(ctypes.Structure):
                 ("flags", ctypes.c_int32)]
 
 
+class EvalParamsDouble(ctypes.Structure):
+    """sbh_eval_params_double (double-build trees)."""
+    _fields_ = [("beta", ctypes.c_double), ("samples_per_subdomain", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("vote_group", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
 class SbhError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -73,6 +80,13 @@ def lib():
         L.sbh_tree_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sbh_tree_layout.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                       ctypes.POINTER(i32), ctypes.POINTER(f32)]
+        f64 = ctypes.c_double
+        L.sbh_tree_create_double.argtypes = [i32, i32, i32, i32, i64, vp, vp, vp, vp, vp, f64,
+                                             ctypes.POINTER(vp), ctypes.POINTER(f32)]
+        L.sbh_tree_is_double.argtypes = [vp, ctypes.POINTER(i32)]
+        L.sbh_tree_nodes_double.argtypes = [vp] + [vp] * 6
+        L.sbh_eval_double.argtypes = [vp, i32, i32, ctypes.POINTER(EvalParamsDouble), vp, i64,
+                                      vp, vp, ctypes.POINTER(f32)]
         _lib = L
     return _lib
 

@@ -213,3 +213,65 @@ def test_golden_tree_image_validation_on_host():
     bad[100] ^= 1
     assert L.sbh_tree_deserialize(bytes(bad), len(bad), ctypes.byref(h)) == C.ERR_INVALID
     assert "checksum" in L.sbh_last_error().decode()
+
+
+def _create_double(lib, dim=3, k=1, leaf=1, depth=100, pts=None, m=None, lo=None, hi=None,
+                   alpha=1.0):
+    n = 0 if pts is None else len(pts)
+    pts = np.zeros((max(n, 1), dim)) if pts is None else np.asarray(pts, np.float64)
+    m = np.ones(max(n, 1)) if m is None else np.asarray(m, np.float64)
+    lo = np.zeros(dim) if lo is None else np.asarray(lo, np.float64)
+    hi = np.ones(dim) if hi is None else np.asarray(hi, np.float64)
+    h = ctypes.c_void_p()
+    f = lib.sbh_tree_create_double
+    f.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_int64] + [ctypes.c_void_p] * 5 + [
+        ctypes.c_double, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]
+    return f(dim, k, leaf, depth, n, pts.ctypes.data, None, m.ctypes.data, lo.ctypes.data,
+             hi.ctypes.data, alpha, ctypes.byref(h), None)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(dim=1), dict(k=4), dict(leaf=0), dict(pts=np.zeros((0, 3))),
+    dict(pts=[[0.5, np.inf, 0.5]]), dict(pts=[[0.5, 0.5, 0.5]], m=[0.0]),
+    dict(pts=[[0.5, 0.5, 0.5]], lo=[0, 0, 1], hi=[1, 1, 1]),
+    dict(pts=[[0.5, 0.5, 0.5]], alpha=float("nan")),
+])
+def test_invalid_double_tree_arguments(kw):
+    lib = ctypes.CDLL(LIB)
+    assert _create_double(lib, **kw) == 1  # SBH_ERR_INVALID, before touching a device
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_valid_double_tree_without_gpu_reports_no_device():
+    lib = ctypes.CDLL(LIB)
+    assert _create_double(lib, pts=[[0.5, 0.5, 0.5]]) == 4  # SBH_ERR_NODEVICE
+
+
+def test_double_build_module_keeps_reference_names():
+    import sbh_gpu64
+
+    assert sbh_gpu64.fp == 64
+    for c in ("QuadtreeGPU1", "QuadtreeGPU2", "QuadtreeGPU3", "OctreeGPU1", "OctreeGPU2",
+              "OctreeGPU3"):
+        assert hasattr(sbh_gpu64, c)
+    for kern in ("gravity", "wn", "smoothdist", "helmholtz_x", "gravity_grad_z"):
+        for fn in ("eval_brute_force_", "eval_barnes_hut_", "multi_level_prefix_control_variate_"):
+            assert hasattr(sbh_gpu64, fn + kern)
+    pts = np.random.default_rng(1).random((20, 3))
+    with pytest.raises(ValueError):  # validated on the host, before any device work
+        sbh_gpu64.OctreeGPU1(pts, np.zeros_like(pts), -np.ones(20), [0, 0, 0], [1, 1, 1])
+    with pytest.raises(ValueError):
+        sbh_gpu64.QuadtreeGPU1(pts, np.zeros_like(pts), np.ones(20), [0, 0], [1, 1])
+
+
+def test_double_eval_validates_on_host():
+    import sbh_capi as C
+
+    L = C.lib()
+    p = C.EvalParamsDouble(2.0, 1, 0, 1, 0)
+    out = np.zeros(4)
+    q = np.zeros(12)
+    assert L.sbh_eval_double(None, C.BARNES_HUT, 0, p, q.ctypes.data, 4, out.ctypes.data, None,
+                             None) == C.ERR_INVALID
+    isd = ctypes.c_int32()
+    assert L.sbh_tree_is_double(None, ctypes.byref(isd)) == C.ERR_INVALID
