@@ -44,6 +44,7 @@ MFMA_FLOPS_PER_PAIR = 8
 # V_MFMA_F32_16X16X16_BF16: 16*16*16 multiply-adds per block; dense bf16 peak
 MFMA_BF16_FLOPS_PER_PAIR = 32
 BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
 # algorithmic flops per pair (add/sub/mul = 1, sqrt/rcp/exp = 1, divide = 2)
 FLOPS_PER_PAIR = {(3, "gravity"): 13, (3, "wn"): 22, (3, "smoothdist"): 12,
                   (2, "gravity"): 10, (2, "wn"): 14, (2, "smoothdist"): 9}
@@ -379,6 +380,7 @@ def main():
                 "stochastic")})
 
     precision = None if args.no_variants else precision_variant(args)
+    double_build = None if args.no_variants else double_build_variant(args)
     serving = None if args.no_serve else serving_bench(args)
     cpu = None
     if not args.no_cpu:
@@ -419,6 +421,7 @@ def main():
         "build_ms": head["build_ms"], "trees": head["trees"],
         "variants": variants,
         "precision_variant": precision,
+        "double_build": double_build,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
@@ -479,6 +482,47 @@ def precision_variant(args):
         del qd, out, ref
         torch.cuda.empty_cache()
     return rows
+
+
+def double_build_variant(args):
+    """The double build (sbh_gpu64, float64 throughout) on the headline
+    workload: Barnes-Hut (theta 1.0, per query) and the estimator (512 samples
+    per subdomain) over all queries in Morton order, brute force on the first
+    2^18 queries; errors against the double brute force on the first
+    err_queries queries."""
+    import sbh_gpu64
+
+    wl = make_workload("plummer3d", args.n, args.q, rmax=args.rmax)
+    pts, q = wl["pts"], wl["queries"]
+    g = sbh_gpu64.OctreeGPU2(pts, np.zeros_like(pts), wl["m"], wl["lo"], wl["hi"])
+    e = min(args.err_queries, len(q))
+    ref, _ = sbh_gpu64.eval_brute_force_gravity(g, q[:e])
+    nb = min(1 << 18, len(q))
+    sbh_gpu64.eval_brute_force_gravity(g, q[:nb])
+    _, bms = sbh_gpu64.eval_brute_force_gravity(g, q[:nb])
+    out = {"workload": wl["desc"] + " (float64 inputs, tree, terms and sums)",
+           "build_ms": g.build_ms,
+           "brute_force": {"kernel_ms": bms, "queries": nb, "queries_per_s": nb / (bms / 1e3),
+                           "tflops": float(len(pts)) * nb * FLOPS_PER_PAIR[(3, "gravity")]
+                           / (bms / 1e3) / 1e12}}
+    out["brute_force"]["frac_fp64_peak"] = out["brute_force"]["tflops"] / FP64_PEAK_TFLOPS
+
+    def timed(fn, *a, **kw):
+        fn(*a, **kw)
+        v, ms = fn(*a, **kw)
+        err = float(np.mean(np.abs(v[:e] - ref) / np.abs(ref)))
+        return {"kernel_ms": ms, "queries_per_s": len(q) / (ms / 1e3), "mean_rel_err": err}
+
+    out["barnes_hut"] = dict(theta=1.0, **timed(sbh_gpu64.eval_barnes_hut_gravity, g, q, 1.0,
+                                                vote_group=1, sort_queries=True))
+    out["stochastic"] = dict(samples_per_subdomain=512, **timed(
+        sbh_gpu64.multi_level_prefix_control_variate_gravity, g, q, 512, seed=1,
+        sort_queries=True))
+    log(f"double build: brute {bms:.1f} ms / {nb} q, BH {out['barnes_hut']['kernel_ms']:.1f} ms "
+        f"(err {out['barnes_hut']['mean_rel_err']:.2e}), stochastic "
+        f"{out['stochastic']['kernel_ms']:.1f} ms (err {out['stochastic']['mean_rel_err']:.2e})")
+    del g
+    return out
 
 
 def profile(wl, args):

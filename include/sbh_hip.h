@@ -201,8 +201,9 @@ typedef struct sbh_eval_params_double {
   int32_t samples_per_subdomain; /* stochastic: samples per root child (>= 1) */
   uint64_t seed;                 /* stochastic: Philox key */
   int32_t vote_group;            /* Barnes-Hut: 1, 32 or 64 (as sbh_eval_params) */
-  int32_t flags;                 /* SBH_EVAL_DEVICE_POINTERS, _SPLIT_COUNTS,
-                                    _PER_QUERY_PATHS, _SMOOTH_DISTANCE;
+  int32_t flags;                 /* SBH_EVAL_DEVICE_POINTERS, _SORT_QUERIES (the
+                                    Morton order of the queries' float rounding),
+                                    _SPLIT_COUNTS, _PER_QUERY_PATHS, _SMOOTH_DISTANCE;
                                     _EXACT_TERMS is accepted and changes nothing */
 } sbh_eval_params_double;
 /* PyOctree / PyQuadtree + PyOctreeGPU / PyQuadtreeGPU of a USE_DOUBLE build

@@ -627,11 +627,11 @@ int sbh_eval_double(sbh_tree* tree, int32_t method, int32_t kernel,
     if (q == 0) return;
     SBH_REQUIRE(queries && results, "NULL queries or results");
     const sbh_eval_params_double& p = *params;
-    SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SPLIT_COUNTS |
-                             SBH_EVAL_EXACT_TERMS | SBH_EVAL_PER_QUERY_PATHS |
-                             SBH_EVAL_SMOOTH_DISTANCE)) == 0,
-                "unsupported eval flags for a double-build tree (query sorting and the "
-                "matrix-core forms are float-build options)");
+    SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
+                             SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS |
+                             SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_SMOOTH_DISTANCE)) == 0,
+                "unsupported eval flags for a double-build tree (the matrix-core forms are "
+                "float-build options)");
     SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
                 "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
     if (method == SBH_BARNES_HUT) {
@@ -663,15 +663,24 @@ int sbh_eval_double(sbh_tree* tree, int32_t method, int32_t kernel,
         dcnt = tree->counts.get();
       }
     }
+    const bool sort = method != SBH_BRUTE_FORCE && (p.flags & SBH_EVAL_SORT_QUERIES) != 0;
+    uint32_t* ord = nullptr;
+    if (sort) {
+      ensure(tree->q, qvals);
+      ensure(tree->order, nq);
+      ensure(tree->sort_scratch, sbh::sort_scratch_u32(nq));
+      ord = tree->order.get();
+    }
     SBH_HIP(hipEventRecord(tree->ev0, s));
+    if (sort) sbh::query_order_double(t, dq, nq, tree->q.get(), ord, tree->sort_scratch.get(), s);
     if (method == SBH_BRUTE_FORCE)
       sbh::run_brute_double(t, kernel, dq, nq, dout, s);
     else if (method == SBH_BARNES_HUT)
-      sbh::run_bh_double(t, kernel, p.beta, p.vote_group, dq, nq, dout, dcnt, split, s);
+      sbh::run_bh_double(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, s);
     else
       sbh::run_stochastic_double(t, kernel, p.samples_per_subdomain, p.seed,
-                                 (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, dout, dcnt,
-                                 split, s);
+                                 (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout,
+                                 dcnt, split, s);
     SBH_HIP(hipEventRecord(tree->ev1, s));
     if (method == SBH_BRUTE_FORCE && dcnt) {
       // every query evaluates every point; split counts have no node terms

@@ -34,13 +34,15 @@ struct TreeViewD {
   const uint32_t* n_start;
   const uint32_t* n_end;
   const uint32_t* n_skip;
+  const WalkRecordD* dfsR;  // depth-first walk records (Barnes-Hut)
+  const dvec4* dfsN;
   double alpha;
 };
 
 TreeViewD view_double(const DeviceTree& t) {
   return TreeViewD{t.pts64.get(),  t.nrm64.get(),   t.nodeC64.get(), t.nodeN64.get(),
                    t.n_diam64.get(), t.nodeL.get(), t.n_start.get(), t.n_end.get(),
-                   t.n_skip.get(),  t.alpha64};
+                   t.n_skip.get(),  t.dfsR64.get(),  t.dfsN64.get(),  t.alpha64};
 }
 
 struct LaneD {
@@ -62,22 +64,27 @@ __device__ __forceinline__ void store_result(uint32_t i, uint32_t nq, const Lane
   }
 }
 
-// Adds node v's terms: its points in order if v is a leaf, else its moment.
+// Adds node v's terms to acc: its points in order if v is a leaf, else its
+// moment.  Returns the number of point terms (0: one node term).
 template <int D, int K>
-__device__ __forceinline__ void node_accumulate_d(const TreeViewD& t, uint32_t v, double x,
-                                                  double y, double z, double& acc,
-                                                  int32_t& pts, int32_t& nodes) {
+__device__ __forceinline__ uint32_t node_accumulate_d(const TreeViewD& t, uint32_t v, double x,
+                                                      double y, double z, double& acc) {
 #pragma clang fp contract(off)
   const uint4 l = t.nodeL[v];
   if (l.w & kLeafBit) {
     const uint32_t np = l.w & ~kLeafBit;
     for (uint32_t j = l.y; j < l.y + np; ++j)
       acc = acc + kernel_double<D, K>(t.pts[j], t.nrm[j], t.alpha, x, y, z);
-    pts += (int32_t)np;
-  } else {
-    acc = acc + kernel_double<D, K>(t.nodeC[v], t.nodeN[v], t.alpha, x, y, z);
-    nodes += 1;
+    return np;
   }
+  acc = acc + kernel_double<D, K>(t.nodeC[v], t.nodeN[v], t.alpha, x, y, z);
+  return 0;
+}
+
+// Term counts from node_accumulate_d's return value (counters stay in registers).
+__device__ __forceinline__ void count_terms(uint32_t np, int32_t& pts, int32_t& nodes) {
+  pts += (int32_t)np;
+  nodes += np == 0 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -107,6 +114,9 @@ __global__ __launch_bounds__(kThreads) void k_brute_double(TreeViewD t, uint32_t
       if constexpr (kWN) sn[threadIdx.x] = t.nrm[base + threadIdx.x];
     }
     __syncthreads();
+    // unrolled so that the terms of neighbouring points overlap; the
+    // compensated additions stay in point order
+#pragma unroll 4
     for (uint32_t j = 0; j < m; ++j) {
       const double kv = kernel_double<D, K>(sp[j], kWN ? sn[j] : sp[j], t.alpha, x, y, z);
       const double yk = kv - c;
@@ -120,21 +130,24 @@ __global__ __launch_bounds__(kThreads) void k_brute_double(TreeViewD t, uint32_t
 
 // ---------------------------------------------------------------------------
 // Deterministic Barnes-Hut (include/barnes_hut.h:23-78 per query,
-// include/barnes_hut_gpu.h:19-72 vote) over the breadth-first records and
+// include/barnes_hut_gpu.h:19-72 vote) over the depth-first records and
 // their ropes: the wavefront walks the union of its lanes' traversals; a lane
 // that accepts a node while the wave descends adds its term and sleeps until
 // the wave reaches the node's rope target.  Each lane adds exactly the terms
-// of its own walk, in the same order.
+// of its own walk, in the same order.  The node index is wave-uniform: each
+// step is one 64-byte scalar load, issued as soon as the next node is known.
 // ---------------------------------------------------------------------------
 template <int D, int K, int GROUP>
 __global__ __launch_bounds__(kThreads) void k_bh_double(TreeViewD t, double beta,
                                                         const double* qx, const double* qy,
                                                         const double* qz, uint32_t nq,
-                                                        double* out, int32_t* counts,
-                                                        int split) {
+                                                        const uint32_t* order, double* out,
+                                                        int32_t* counts, int split) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  const bool live = i < nq;
+  constexpr bool kWN = (K == WINDING_NUMBER);
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = slot < nq;
+  const uint32_t i = live ? (order ? order[slot] : slot) : 0u;
   const double x = live ? qx[i] : 0.0;
   const double y = live ? qy[i] : 0.0;
   const double z = (live && D == 3) ? qz[i] : 0.0;
@@ -146,32 +159,59 @@ __global__ __launch_bounds__(kThreads) void k_bh_double(TreeViewD t, double beta
     my_group = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
   else
     my_group = 1ull << lane;
+  const WalkRecordD* __restrict__ rec = t.dfsR;
   double acc = 0.0;
   int32_t npts = 0, nnodes = 0;
   bool active = live;
   uint32_t resume = kEnd;
   uint32_t v = 0;
+  WalkRecordD r = rec[0];
   while (v != kEnd) {
     v = uniform_u32(v);
     active = active || (live && resume == v);
-    const uint4 l = t.nodeL[v];
-    const bool leaf = (l.w & kLeafBit) != 0;
+    const bool leaf = (r.info & kLeafBit) != 0;
     bool reject = false;
     if (!leaf) {
-      const double r = ratio_double<D>(t.nodeC[v], t.diam[v], x, y, z);
-      reject = active && !(r >= beta);
+      // ||q - c|| / diam >= beta, decided as the correctly rounded quotient
+      // decides it: squared distances more than 1e-12 (relative) from
+      // (beta diam)^2 settle it without the square root and the divide,
+      // whose roundings are ~1e-16; the rest (and a (beta diam)^2 outside
+      // the normal range) take the exact formula.
+      const double dx = x - r.c.x, dy = y - r.c.y, dz = (D == 3) ? (z - r.c.z) : 0.0;
+      const double d2 = sqnorm_exact<D>(dx, dy, dz);
+      const double b2 = (beta * r.diam) * (beta * r.diam);
+      const bool in_range = b2 >= 1e-290 && b2 <= 1e290;
+      const bool far = in_range && d2 >= b2 * (1.0 + 1e-12);
+      const bool near = in_range && d2 <= b2 * (1.0 - 1e-12);
+      bool open_ok = far;
+      if (!far && !near) open_ok = __builtin_sqrt(d2) / r.diam >= beta;
+      reject = active && !open_ok;
     }
     const uint64_t rej = __ballot(reject);
     const bool accept = active && (GROUP == 1 ? !reject : (rej & my_group) == 0);
-    if (accept) node_accumulate_d<D, K>(t, v, x, y, z, acc, npts, nnodes);
+    const uint32_t np = leaf ? (r.info & ~kLeafBit) : 0u;
+    if (accept) {
+      if (np > 1) {
+        for (uint32_t j = r.first; j < r.first + np; ++j)
+          acc = acc + kernel_double<D, K>(t.pts[j], t.nrm[j], t.alpha, x, y, z);
+      } else {
+        // an internal node's moment, or a one-point leaf's point (its record holds it)
+        const dvec4 nv = kWN ? t.dfsN[v] : r.c;
+        acc = acc + kernel_double<D, K>(r.c, nv, t.alpha, x, y, z);
+      }
+      npts += (int32_t)np;
+      nnodes += leaf ? 0 : 1;
+    }
     uint32_t next;
     if (rej == 0) {
-      next = l.z;
+      next = r.rope;
     } else {
-      resume = accept ? l.z : resume;
+      resume = accept ? r.rope : resume;
       active = active && !accept;
-      next = l.y;
+      next = r.first;
     }
+    next = uniform_u32(next);
+    if (next != kEnd) r = rec[next];
     v = next;
   }
   if (live) store_result(i, nq, LaneD{acc, npts, nnodes}, out, counts, split);
@@ -195,14 +235,14 @@ __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, u
   const uint4 rl = t.nodeL[0];
   if (rl.w & kLeafBit) {
     double v = 0.0;
-    node_accumulate_d<D, K>(t, 0, x, y, z, v, npts, nnodes);
+    count_terms(node_accumulate_d<D, K>(t, 0, x, y, z, v), npts, nnodes);
     return {v, npts, nnodes};
   }
   double bh = 0.0, mlcv = 0.0;
   const uint32_t nsub = rl.w, first = rl.y;
   for (uint32_t si = 0; si < nsub; ++si) {
     const uint32_t S = first + si;
-    node_accumulate_d<D, K>(t, S, x, y, z, bh, npts, nnodes);
+    count_terms(node_accumulate_d<D, K>(t, S, x, y, z, bh), npts, nnodes);
     const uint4 Sl = t.nodeL[S];
     if (Sl.w & kLeafBit) continue;  // the subdomain's points were summed exactly
     const uint32_t sstart = t.n_start[S];
@@ -211,7 +251,8 @@ __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, u
     const uint32_t f0 = Sl.y, nch0 = Sl.w;
     double r0sum = 0.0;
     int32_t p0 = 0, n0 = 0;
-    for (uint32_t j = 0; j < nch0; ++j) node_accumulate_d<D, K>(t, f0 + j, x, y, z, r0sum, p0, n0);
+    for (uint32_t j = 0; j < nch0; ++j)
+      count_terms(node_accumulate_d<D, K>(t, f0 + j, x, y, z, r0sum), p0, n0);
     const double prev0 = kernel_double<D, K>(t.nodeC[S], t.nodeN[S], t.alpha, x, y, z);
     const double wd0 = (double)nS / (1.0 * (double)nS) * (r0sum - prev0);
     double sub_total = 0.0;
@@ -254,7 +295,7 @@ __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, u
         uint32_t pk = 0;
         for (uint32_t j = 0; j < nch; ++j) {
           const uint32_t c = f + j;
-          node_accumulate_d<D, K>(t, c, x, y, z, cur_result, npts, nnodes);
+          count_terms(node_accumulate_d<D, K>(t, c, x, y, z, cur_result), npts, nnodes);
           pk = t.n_start[c] <= idx ? j : pk;  // children are in point order
         }
         cur = t.n_skip[f + pk];
@@ -279,14 +320,23 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_double(TreeViewD t, int
                                                                 int per_query, const double* qx,
                                                                 const double* qy,
                                                                 const double* qz, uint32_t nq,
+                                                                const uint32_t* order,
                                                                 double* out, int32_t* counts,
                                                                 int split) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= nq) return;
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  if (slot >= nq) return;
+  const uint32_t i = order ? order[slot] : slot;
   const double x = qx[i], y = qy[i], z = D == 3 ? qz[i] : 0.0;
-  const uint32_t g = per_query ? i : i / 64u;
+  // paths: the query's own, or those of its wavefront in launch order
+  const uint32_t g = per_query ? i : slot / 64u;
   store_result(i, nq, stochastic_double<D, K>(t, spl, key0, key1, g, x, y, z), out, counts,
                split);
+}
+
+// float copy of the queries, for their spatial order
+__global__ void k_queries_to_float(const double* q, float* qf, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) qf[i] = (float)q[i];
 }
 
 __global__ void k_smooth_distance_double(double* w, uint32_t n, double alpha) {
@@ -331,28 +381,30 @@ struct BruteL {
 template <int D, int K>
 struct BhL {
   static void run(const DeviceTree& t, double beta, int group, const double* q, uint32_t nq,
-                  double* out, int32_t* counts, int split, hipStream_t s) {
+                  const uint32_t* order, double* out, int32_t* counts, int split,
+                  hipStream_t s) {
     const double* qz = D == 3 ? q + 2ull * nq : nullptr;
     const uint32_t nb = nblk(nq, kThreads);
     if (group == 64)
       k_bh_double<D, K, 64><<<nb, kThreads, 0, s>>>(view_double(t), beta, q, q + nq, qz, nq,
-                                                   out, counts, split);
+                                                   order, out, counts, split);
     else if (group == 32)
       k_bh_double<D, K, 32><<<nb, kThreads, 0, s>>>(view_double(t), beta, q, q + nq, qz, nq,
-                                                   out, counts, split);
+                                                   order, out, counts, split);
     else
       k_bh_double<D, K, 1><<<nb, kThreads, 0, s>>>(view_double(t), beta, q, q + nq, qz, nq,
-                                                  out, counts, split);
+                                                  order, out, counts, split);
   }
 };
 
 template <int D, int K>
 struct StochL {
   static void run(const DeviceTree& t, int spl, uint64_t seed, bool per_query, const double* q,
-                  uint32_t nq, double* out, int32_t* counts, int split, hipStream_t s) {
+                  uint32_t nq, const uint32_t* order, double* out, int32_t* counts, int split,
+                  hipStream_t s) {
     k_stochastic_double<D, K><<<nblk(nq, kThreads), kThreads, 0, s>>>(
         view_double(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), per_query ? 1 : 0, q,
-        q + nq, D == 3 ? q + 2ull * nq : nullptr, nq, out, counts, split);
+        q + nq, D == 3 ? q + 2ull * nq : nullptr, nq, order, out, counts, split);
   }
 };
 
@@ -365,16 +417,25 @@ void run_brute_double(const DeviceTree& t, int kernel, const double* q, uint32_t
 }
 
 void run_bh_double(const DeviceTree& t, int kernel, double beta, int group, const double* q,
-                   uint32_t nq, double* out, int32_t* counts, int split, hipStream_t s) {
-  dispatch<BhL>(t.dim, kernel, t, beta, group, q, nq, out, counts, split, s);
+                   uint32_t nq, const uint32_t* order, double* out, int32_t* counts, int split,
+                   hipStream_t s) {
+  dispatch<BhL>(t.dim, kernel, t, beta, group, q, nq, order, out, counts, split, s);
   SBH_HIP(hipGetLastError());
 }
 
 void run_stochastic_double(const DeviceTree& t, int kernel, int spl, uint64_t seed,
-                           bool per_query, const double* q, uint32_t nq, double* out,
-                           int32_t* counts, int split, hipStream_t s) {
-  dispatch<StochL>(t.dim, kernel, t, spl, seed, per_query, q, nq, out, counts, split, s);
+                           bool per_query, const double* q, uint32_t nq, const uint32_t* order,
+                           double* out, int32_t* counts, int split, hipStream_t s) {
+  dispatch<StochL>(t.dim, kernel, t, spl, seed, per_query, q, nq, order, out, counts, split, s);
   SBH_HIP(hipGetLastError());
+}
+
+void query_order_double(const DeviceTree& t, const double* q, uint32_t nq, float* qf,
+                        uint32_t* order, uint32_t* scratch, hipStream_t s) {
+  const uint32_t n = nq * (uint32_t)t.dim;
+  k_queries_to_float<<<nblk(n, 256), 256, 0, s>>>(q, qf, n);
+  SBH_HIP(hipGetLastError());
+  query_order(t, qf, nq, order, scratch, s);
 }
 
 void run_smooth_distance_double(double* w, uint32_t n, double alpha, hipStream_t s) {

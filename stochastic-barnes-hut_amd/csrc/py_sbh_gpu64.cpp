@@ -10,8 +10,8 @@
 // for the bound kernels (gravity, wn, smoothdist) and the unbound ones of
 // include/kernel.h:61-83.  Inputs are taken as float64 and every evaluation
 // returns (float64[q], kernel milliseconds).  Keyword extensions as in
-// `sbh_gpu`: vote_group (32), seed (0), return_counts, per_query_paths,
-// smooth_distance.  Query sorting and the matrix-core forms are float-build
+// `sbh_gpu`: vote_group (32), seed (0), sort_queries (False), return_counts,
+// per_query_paths, smooth_distance.  The matrix-core forms are float-build
 // options and are not offered here.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -219,22 +219,24 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
         },
         "tree"_a, "queries"_a, "smooth_distance"_a = false);
   m.def(("eval_barnes_hut_" + s).c_str(),
-        [kernel](TreeBase& t, F64Array q, double beta, int vote_group, bool return_counts,
-                 bool smooth_distance) {
+        [kernel](TreeBase& t, F64Array q, double beta, int vote_group, bool sort_queries,
+                 bool return_counts, bool smooth_distance) {
           return run_eval(t, SBH_BARNES_HUT, kernel, q, beta, 1, 0, vote_group, return_counts,
-                          smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0);
-        },
-        "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "return_counts"_a = false,
-        "smooth_distance"_a = false);
-  m.def(("multi_level_prefix_control_variate_" + s).c_str(),
-        [kernel](TreeBase& t, F64Array q, int spl, uint64_t seed, bool return_counts,
-                 bool per_query_paths, bool smooth_distance) {
-          return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.0, spl, seed, 1, return_counts,
-                          (per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0) |
+                          (sort_queries ? SBH_EVAL_SORT_QUERIES : 0) |
                               (smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0));
         },
-        "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "return_counts"_a = false,
-        "per_query_paths"_a = false, "smooth_distance"_a = false);
+        "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "sort_queries"_a = false,
+        "return_counts"_a = false, "smooth_distance"_a = false);
+  m.def(("multi_level_prefix_control_variate_" + s).c_str(),
+        [kernel](TreeBase& t, F64Array q, int spl, uint64_t seed, bool sort_queries,
+                 bool return_counts, bool per_query_paths, bool smooth_distance) {
+          return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.0, spl, seed, 1, return_counts,
+                          (sort_queries ? SBH_EVAL_SORT_QUERIES : 0) |
+                              (per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0) |
+                              (smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0));
+        },
+        "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
+        "return_counts"_a = false, "per_query_paths"_a = false, "smooth_distance"_a = false);
 }
 
 }  // namespace

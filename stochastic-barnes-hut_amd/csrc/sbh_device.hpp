@@ -274,6 +274,15 @@ struct alignas(32) dvec4 {
 };
 constexpr double kPiD = 3.14159265358979323846;
 
+// Depth-first walk record of the double build: centroid and mass (a one-point
+// leaf: its point), diameter, first child (depth-first position) or first
+// point, rope, child count | leaf flag and point count.
+struct alignas(64) WalkRecordD {
+  dvec4 c;
+  double diam;
+  uint32_t first, rope, info, pad;
+};
+
 template <int D>
 __device__ __forceinline__ double sqnorm_exact(double dx, double dy, double dz) {
 #pragma clang fp contract(off)

@@ -50,11 +50,17 @@ void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s);
 // Barnes-Hut (vote group 1, 32 or 64) and the stochastic estimator in float64.
 void run_brute_double(const DeviceTree& t, int kernel, const double* q, uint32_t nq, double* out,
                       hipStream_t s);
+// order (optional): query evaluated in each slot (query_order_double).
 void run_bh_double(const DeviceTree& t, int kernel, double beta, int group, const double* q,
-                   uint32_t nq, double* out, int32_t* counts, int split, hipStream_t s);
+                   uint32_t nq, const uint32_t* order, double* out, int32_t* counts, int split,
+                   hipStream_t s);
 void run_stochastic_double(const DeviceTree& t, int kernel, int spl, uint64_t seed,
-                           bool per_query, const double* q, uint32_t nq, double* out,
-                           int32_t* counts, int split, hipStream_t s);
+                           bool per_query, const double* q, uint32_t nq, const uint32_t* order,
+                           double* out, int32_t* counts, int split, hipStream_t s);
+// Spatial order of float64 queries (their float rounding's Morton order);
+// qf holds nq * dim floats, scratch sort_scratch_u32(nq) values.
+void query_order_double(const DeviceTree& t, const double* q, uint32_t nq, float* qf,
+                        uint32_t* order, uint32_t* scratch, hipStream_t s);
 void run_smooth_distance_double(double* w, uint32_t n, double alpha, hipStream_t s);
 
 // Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)

@@ -154,8 +154,8 @@ struct DeviceTree {
   // Double build (the reference's USE_DOUBLE configuration): points, cell
   // bounds, diameters and moments in float64 instead of the float arrays
   // above; the index arrays (ranges, children, ropes, n_skip, perm) and the
-  // float64 moments n_m64 are shared.  A double tree has no depth-first or
-  // derived records and no fp32 point / node arrays.
+  // float64 moments n_m64 are shared.  A double tree has no fp32 point, node,
+  // depth-first or derived records of its own.
   bool is_double = false;
   double alpha64 = 1.0;
   double lo64[3] = {0, 0, 0};
@@ -164,6 +164,10 @@ struct DeviceTree {
   DevBuf<dvec4> n_lo64, n_hi64;    // cell bounds
   DevBuf<double> n_diam64;
   DevBuf<dvec4> nodeC64, nodeN64;  // {cx, cy, cz, mass}, {nx, ny, nz, 0}
+  // Depth-first walk records of the double build (Barnes-Hut), and the
+  // normal of each record's term source.
+  DevBuf<WalkRecordD> dfsR64;
+  DevBuf<dvec4> dfsN64;
 
   size_t device_bytes() const {
     return pts.bytes() + nrm.bytes() + perm.bytes() + n_start.bytes() + n_end.bytes() +
@@ -173,7 +177,7 @@ struct DeviceTree {
            n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
            kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes() + pts64.bytes() +
            nrm64.bytes() + n_lo64.bytes() + n_hi64.bytes() + n_diam64.bytes() +
-           nodeC64.bytes() + nodeN64.bytes();
+           nodeC64.bytes() + nodeN64.bytes() + dfsR64.bytes() + dfsN64.bytes();
   }
 };
 

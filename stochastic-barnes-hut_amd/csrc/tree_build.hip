@@ -619,6 +619,29 @@ __global__ void k_moments_double(uint32_t lb, uint32_t count, const uint32_t* n_
   o[7] = 0.0;
 }
 
+// Depth-first records of the double build, as k_dfs_scatter writes them for
+// the float build (a one-point leaf's record holds its point).
+__global__ void k_dfs_scatter_double(uint32_t nn, const uint32_t* n_dfs, const uint32_t* n_size,
+                                     const dvec4* nodeC, const dvec4* nodeN, const uint4* nodeL,
+                                     const double* n_diam, const dvec4* pts, const dvec4* nrm,
+                                     WalkRecordD* dfsR, dvec4* dfsN) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  const uint32_t d = n_dfs[v];
+  const uint4 l = nodeL[v];
+  const bool one_point = l.w == (kLeafBit | 1u);
+  const uint32_t rope = d + n_size[v];
+  WalkRecordD r;
+  r.c = one_point ? pts[l.y] : nodeC[v];
+  r.diam = n_diam[v];
+  r.first = (l.w & kLeafBit) ? l.y : d + 1;
+  r.rope = rope < nn ? rope : kEnd;
+  r.info = l.w;
+  r.pad = 0;
+  dfsR[d] = r;
+  dfsN[d] = one_point ? nrm[l.y] : nodeN[v];
+}
+
 // The walk record {diameter bits, first child | first point, rope, info}; a
 // double tree keeps its diameters in n_diam64 and records their fp32 rounding.
 template <typename T>
@@ -1019,6 +1042,20 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
     k_finalize<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
         num_nodes, t.n_start.get(), t.n_end.get(), t.n_first.get(), t.n_nchild.get(),
         t.n_next.get(), n_diam.get(), t.nodeL.get());
+    SBH_HIP(hipGetLastError());
+    t.n_dfs.alloc(num_nodes);
+    t.dfsR64.alloc(num_nodes);
+    t.dfsN64.alloc(num_nodes);
+    k_zero_u32<<<1, 1, 0, s>>>(t.n_dfs.get());
+    for (int l = 0; l <= t.depth; ++l) {
+      uint32_t b = t.level_begin[l], cnt = t.level_begin[l + 1] - b;
+      k_dfs_positions<<<blocks_for(cnt, 128), 128, 0, s>>>(b, cnt, t.n_first.get(),
+                                                            t.n_nchild.get(), t.n_size.get(),
+                                                            t.n_dfs.get());
+    }
+    k_dfs_scatter_double<<<blocks_for(num_nodes, 256), 256, 0, s>>>(
+        num_nodes, t.n_dfs.get(), t.n_size.get(), t.nodeC64.get(), t.nodeN64.get(),
+        t.nodeL.get(), n_diam.get(), pts.get(), nrm.get(), t.dfsR64.get(), t.dfsN64.get());
     SBH_HIP(hipGetLastError());
     SBH_HIP(hipStreamSynchronize(s));
     return;

@@ -124,6 +124,47 @@ def test_double_stochastic(sbh_gpu64, case, per_query):
             _compare(kern, got, ref, np.abs(ref).max() + 1.0)
 
 
+def _launch_order(g32, qs):
+    """The spatial order SBH_EVAL_SORT_QUERIES uses (Morton order of the
+    queries' float rounding in the tree's float-rounded root cell)."""
+    import ctypes
+
+    import sbh_capi as C
+
+    q32 = np.ascontiguousarray(qs.astype(np.float32).T)
+    order = np.empty(len(qs), np.uint32)
+    C.check(C.lib().sbh_query_order(g32.h, q32.ctypes.data, len(qs), order.ctypes.data, 0))
+    return order.astype(np.int64)
+
+
+def test_double_sorted_queries(sbh_gpu64, case):
+    """Sorting the launch changes nothing per query (vote group 1, per-query
+    paths) and regroups the shared forms exactly as the oracle does on the
+    queries taken in launch order."""
+    import sbh_capi as C
+
+    dim, k, qs, g, o = case
+    g32 = C.Tree(o.points, None, o.masses, o.min_corner, o.max_corner, split_log=k)
+    order = _launch_order(g32, qs)
+    assert sorted(order.tolist()) == list(range(len(qs)))
+    for kern in ("gravity", "wn"):
+        bh = getattr(sbh_gpu64, "eval_barnes_hut_" + kern)
+        a, _ = bh(g, qs, 2.0, vote_group=1)
+        b, _ = bh(g, qs, 2.0, vote_group=1, sort_queries=True)
+        _same_bits(a, b)
+        c, _ = bh(g, qs, 2.0, vote_group=32, sort_queries=True)
+        ref = o.barnes_hut(kern, qs[order], 2.0, vote_group=32)
+        _same_bits(c[order], ref)
+        st = getattr(sbh_gpu64, "multi_level_prefix_control_variate_" + kern)
+        a, _ = st(g, qs, 2, seed=5, per_query_paths=True)
+        b, _ = st(g, qs, 2, seed=5, per_query_paths=True, sort_queries=True)
+        _same_bits(a, b)
+        c, _ = st(g, qs, 2, seed=5, sort_queries=True)
+        ref = o.stochastic(kern, qs[order], 2, seed=5, group=64)
+        _same_bits(c[order], ref)
+    g32.close()
+
+
 def test_double_smooth_distance_transform(sbh_gpu64, case):
     dim, k, qs, g, o = case
     w, _ = sbh_gpu64.eval_barnes_hut_smoothdist(g, qs, 2.0, vote_group=1)
