@@ -294,7 +294,7 @@ def main():
                     help="profiling mode: split_log of the stochastic estimator's tree")
     ap.add_argument("--no-sort", action="store_true",
                     help="profiling mode: evaluate the stochastic estimator in input order")
-    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute", "mfma"],
+    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute", "mfma", "double"],
                     default=None,
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
@@ -532,6 +532,21 @@ def profile(wl, args):
     import sbh_capi as C
 
     dev = torch.device("cuda:0")
+    if args.profile_kernel == "double":
+        # the double build: Barnes-Hut at --theta and the estimator at --spl[0],
+        # Morton-sorted, --steps times each
+        import sbh_gpu64
+
+        pts, q = wl["pts"], wl["queries"]
+        g = sbh_gpu64.OctreeGPU2(pts, np.zeros_like(pts), wl["m"], wl["lo"], wl["hi"])
+        for _ in range(args.steps):
+            _, ms = sbh_gpu64.eval_barnes_hut_gravity(g, q, 1.0 / args.theta, vote_group=1,
+                                                      sort_queries=True)
+            log(f"double bh: {ms:.3f} ms")
+            _, ms = sbh_gpu64.multi_level_prefix_control_variate_gravity(
+                g, q, args.spl[0], seed=1, sort_queries=True)
+            log(f"double stochastic: {ms:.3f} ms")
+        return
     if args.copy_mib:
         a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)

@@ -208,3 +208,48 @@ def test_double_tree_refused_by_float_entry_points(sbh_gpu, sbh_gpu64, tmp_path)
     assert L.sbh_eval_double(t32.h, C.BARNES_HUT, 0, p, q.ctypes.data, 100, out.ctypes.data,
                              None, None) == C.ERR_INVALID
     assert "double-build" in L.sbh_last_error().decode()
+
+
+@pytest.mark.parametrize("shape", ["leaf4", "duplicates", "single_point"])
+def test_double_edge_trees(sbh_gpu64, shape):
+    """Leaves of several points, duplicates that reach the depth limit, and a
+    one-point tree: the double build still equals the oracle bit for bit."""
+    rng = np.random.default_rng(77)
+    leaf = 1
+    if shape == "leaf4":
+        pts = rng.random((1500, 3))
+        leaf = 4
+    elif shape == "duplicates":
+        base = rng.random((60, 3))
+        pts = np.concatenate([base, base[:12], base[:5]])
+    else:
+        pts = np.array([[0.3, 0.6, 0.2]])
+    n = len(pts)
+    nrm = rng.normal(size=pts.shape)
+    m = rng.uniform(0.5, 1.5, n)
+    lo, hi = np.full(3, -0.01), np.full(3, 1.01)
+    qs = rng.random((100, 3)) * 1.4 - 0.2
+    print(f"[case] double edge tree {shape} n={n} leaf={leaf}", flush=True)
+    g = sbh_gpu64.OctreeGPU1(pts, nrm, m, lo, hi, 1.0, leaf)
+    o = oracle.Tree(pts, nrm, m, lo, hi, split_log=1, leaf_size=leaf, dtype=np.float64)
+    np.testing.assert_array_equal(g.leaf_order(), o.leaf_order())
+    assert g.info()["depth"] == o.depth
+    for kern in ("gravity", "wn"):
+        _same_bits(getattr(sbh_gpu64, "eval_brute_force_" + kern)(g, qs)[0],
+                   o.brute_force(kern, qs))
+        for group in (1, 64):
+            got, _, cnt = getattr(sbh_gpu64, "eval_barnes_hut_" + kern)(
+                g, qs, 2.0, vote_group=group, return_counts=True)
+            ref, rc = o.barnes_hut(kern, qs, 2.0, vote_group=group, counts=True)
+            _same_bits(got, ref)
+            np.testing.assert_array_equal(cnt, rc)
+        for per_query in (False, True):
+            got, _, cnt = getattr(sbh_gpu64, "multi_level_prefix_control_variate_" + kern)(
+                g, qs, 3, seed=4, return_counts=True, per_query_paths=per_query)
+            ref, rc = o.stochastic(kern, qs, 3, seed=4, group=1 if per_query else 64,
+                                   counts=True)
+            _same_bits(got, ref)
+            np.testing.assert_array_equal(cnt, rc)
+    # no queries: empty results, no launch
+    empty, ms = sbh_gpu64.eval_barnes_hut_gravity(g, np.zeros((0, 3)), 2.0)
+    assert empty.shape == (0,) and ms == 0.0
