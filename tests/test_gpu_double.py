@@ -125,7 +125,7 @@ def test_double_stochastic(sbh_gpu64, case, per_query):
 
 
 def _launch_order(g32, qs):
-    """The spatial order SBH_EVAL_SORT_QUERIES uses (Morton order of the
+    """The spatial order SBH_EVAL_SORT_QUERIES uses (Hilbert order of the
     queries' float rounding in the tree's float-rounded root cell)."""
     import ctypes
 

@@ -7,7 +7,7 @@ N = Q = 2^20, octree k = 2): properties that do not depend on the size.
   theta 1.0 stay under the 1e-2 mean relative error of the metric against
   the float64 all-pairs sum (GPU float64 kernel, pinned against the oracle at
   small sizes) on 16384 queries;
-* Barnes-Hut per query is independent of the query order (Morton-sorted and
+* Barnes-Hut per query is independent of the query order (Hilbert-sorted and
   unsorted launches agree bit for bit);
 * the estimator is a function of (seed, query group): the same call twice
   gives the same bits, another seed gives a different sample.

@@ -14,6 +14,7 @@ import pytest
 
 import oracle
 import workloads as W
+from test_curve_order import hilbert_keys
 
 pytestmark = pytest.mark.gpu
 
@@ -381,28 +382,10 @@ def test_host_tree_constructor(sbh_gpu):
         sbh_gpu.OctreeGPU1(cpu)
 
 
-def _morton_keys(q, lo, hi):
-    """The 63-bit (3D) / 62-bit (2D) Morton keys of the query order, in
-    float64 as the device computes them."""
-    dim = q.shape[1]
-    cells = 2097152.0 if dim == 3 else 2147483648.0
-    bits = 21 if dim == 3 else 31
-    q = q.astype(np.float32).astype(np.float64)
-    lo = np.asarray(lo, np.float32).astype(np.float64)
-    hi = np.asarray(hi, np.float32).astype(np.float64)
-    t = (q - lo) / (hi - lo) * cells
-    c = np.clip(t, 0.0, cells - 1.0).astype(np.uint64)
-    key = np.zeros(len(q), np.uint64)
-    for b in range(bits):
-        for a in range(dim):
-            key |= ((c[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(dim * b + a)
-    return key
-
-
 @pytest.mark.parametrize("dim", [3, 2])
-def test_query_order_is_a_stable_morton_sort(sbh_gpu, dim):
+def test_query_order_is_a_stable_hilbert_sort(sbh_gpu, dim):
     """sbh_query_order (the order SBH_EVAL_SORT_QUERIES evaluates in): a
-    permutation, keys non-decreasing, equal keys in input order."""
+    permutation, Hilbert keys non-decreasing, equal keys in input order."""
     import ctypes
 
     import sbh_capi as C
@@ -418,7 +401,7 @@ def test_query_order_is_a_stable_morton_sort(sbh_gpu, dim):
     order = np.empty(n, np.uint32)
     C.check(C.lib().sbh_query_order(t.h, soa.ctypes.data, n, order.ctypes.data, 0))
     np.testing.assert_array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
-    keys = _morton_keys(q, lo, hi)[order]
+    keys = hilbert_keys(q, lo, hi)[order]
     assert (keys[1:] >= keys[:-1]).all()
     same = keys[1:] == keys[:-1]
     assert (order[1:][same] > order[:-1][same]).all()
