@@ -170,7 +170,7 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
     log(f"  brute force {ms:.1f} ms err {res['brute_force']['mean_rel_err']:.2e} "
         f"{res['brute_force']['tflops']:.1f} TFLOP/s")
     if dim == 3 and kern in ("gravity", "smoothdist"):
-        # matrix-core distances; the timed region includes its Morton query sort
+        # matrix-core distances; the timed region includes its Hilbert query sort
         _, kms = run(t1, C.BRUTE_FORCE, args.brute_steps, 1, flags=C.EVAL_MFMA)
         ms = float(np.mean(kms))
         res["brute_force_mfma"] = {"kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
@@ -228,7 +228,7 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
         res["headline"] = {"wall_ms": wall, "kernel_ms": kms, "mean_rel_err": err_of()}
     if want_counts and head["sorted"]:
         # the estimator kernel alone (for the roofline): the queries permuted
-        # into the Morton order beforehand and evaluated in that order, which
+        # into the Hilbert order beforehand and evaluated in that order, which
         # is exactly the launch a sorted evaluation makes after its sort
         order = torch.empty(nq, dtype=torch.int32, device=dev)
         C.check(C.lib().sbh_query_order(th.h, ctypes.c_void_p(qd.data_ptr()), nq,
@@ -487,7 +487,7 @@ def precision_variant(args):
 def double_build_variant(args):
     """The double build (sbh_gpu64, float64 throughout) on the headline
     workload: Barnes-Hut (theta 1.0, per query) and the estimator (512 samples
-    per subdomain) over all queries in Morton order, brute force on the first
+    per subdomain) over all queries in Hilbert order, brute force on the first
     2^18 queries; errors against the double brute force on the first
     err_queries queries."""
     import sbh_gpu64
@@ -534,7 +534,7 @@ def profile(wl, args):
     dev = torch.device("cuda:0")
     if args.profile_kernel == "double":
         # the double build: Barnes-Hut at --theta and the estimator at --spl[0],
-        # Morton-sorted, --steps times each
+        # Hilbert-sorted, --steps times each
         import sbh_gpu64
 
         pts, q = wl["pts"], wl["queries"]

@@ -63,7 +63,7 @@ typedef struct sbh_tree sbh_tree;
 
 /* sbh_eval flags. */
 #define SBH_EVAL_DEVICE_POINTERS 1 /* queries / results / counts are device pointers */
-#define SBH_EVAL_SORT_QUERIES 2    /* evaluate in spatial (Morton) order; results are
+#define SBH_EVAL_SORT_QUERIES 2    /* evaluate in spatial (Hilbert-curve) order; results are
                                       returned in input order.  Barnes-Hut with
                                       vote_group 1 gives identical values either way */
 #define SBH_EVAL_SPLIT_COUNTS 4    /* counts holds 2q values: node terms in [0, q),
@@ -202,7 +202,7 @@ typedef struct sbh_eval_params_double {
   uint64_t seed;                 /* stochastic: Philox key */
   int32_t vote_group;            /* Barnes-Hut: 1, 32 or 64 (as sbh_eval_params) */
   int32_t flags;                 /* SBH_EVAL_DEVICE_POINTERS, _SORT_QUERIES (the
-                                    Morton order of the queries' float rounding),
+                                    Hilbert order of the queries' float rounding),
                                     _SPLIT_COUNTS, _PER_QUERY_PATHS, _SMOOTH_DISTANCE;
                                     _EXACT_TERMS is accepted and changes nothing */
 } sbh_eval_params_double;

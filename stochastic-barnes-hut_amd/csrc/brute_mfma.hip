@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))
         for (int v = 0; v < 4; ++v) dmin = fminf(dmin, d[t][v]);
       if (!__ballot(dmin < near2 || (K != GRAVITY && !valid))) {
         // the whole tile is in every query's far field (the common case for
-        // Morton-ordered queries and leaf-ordered sources): the expansion's
+        // Hilbert-ordered queries and leaf-ordered sources): the expansion's
         // distances are used as they are
 #pragma unroll
         for (int t = 0; t < 4; ++t)

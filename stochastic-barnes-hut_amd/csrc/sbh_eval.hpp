@@ -57,13 +57,13 @@ void run_bh_double(const DeviceTree& t, int kernel, double beta, int group, cons
 void run_stochastic_double(const DeviceTree& t, int kernel, int spl, uint64_t seed,
                            bool per_query, const double* q, uint32_t nq, const uint32_t* order,
                            double* out, int32_t* counts, int split, hipStream_t s);
-// Spatial order of float64 queries (their float rounding's Morton order);
+// Spatial order of float64 queries (their float rounding's Hilbert order);
 // qf holds nq * dim floats, scratch sort_scratch_u32(nq) values.
 void query_order_double(const DeviceTree& t, const double* q, uint32_t nq, float* qf,
                         uint32_t* order, uint32_t* scratch, hipStream_t s);
 void run_smooth_distance_double(double* w, uint32_t n, double alpha, hipStream_t s);
 
-// Spatial (Morton) order of the queries; scratch holds sort_scratch_u32(nq)
+// Spatial (Hilbert-curve) order of the queries; scratch holds sort_scratch_u32(nq)
 // uint32 values.
 size_t sort_scratch_u32(uint32_t nq);
 void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* order,

@@ -894,7 +894,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
 // 1-query group gets in the wavefront-shared estimator, so a query's result
 // does not depend on which queries share its launch or in what order.  Lanes
 // walk independently, reading node records and children straight from
-// memory (neighbouring queries in Morton order touch the same records).
+// memory (neighbouring queries in Hilbert order touch the same records).
 // ---------------------------------------------------------------------------
 template <int D, int K, bool EXACT>
 __device__ __forceinline__ LaneResult stochastic_estimate_query(const TreeView& t, int spl,
