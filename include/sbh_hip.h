@@ -252,13 +252,15 @@ int sbh_tree_load(const char* path, sbh_tree** out);
 /*
  * Serving.  No counterpart in the reference (it re-copies queries per call
  * and runs synchronously, src/sbh_gpu.cu:212-248): resident trees, requests
- * queued and admitted in arrival order while their working sets (queries +
- * results, (dim + 1) * 4 bytes per query) fit in the server's device budget,
- * each admitted batch run as one work-queue launch, up to max_inflight
- * launches overlapping on separate streams.  Barnes-Hut runs per query
- * (vote_group is ignored), the stochastic estimator keys paths on (seed,
- * 64-query block of the request): a request's values are identical alone and
- * in any mix.  Terms use the fast kernels.
+ * queued and admitted in arrival order while their working sets (queries,
+ * results, their Hilbert-curve order and its sort scratch: about
+ * (dim + 7) * 4 bytes per query) fit in the server's device budget, each
+ * admitted batch run as one work-queue launch over the requests' sorted
+ * queries, up to max_inflight launches overlapping on separate streams.
+ * Barnes-Hut runs per query (vote_group is ignored), the stochastic estimator
+ * keys paths on (seed, 64-query block of the request's sorted order): a
+ * request's values are identical alone and in any mix, and equal those of
+ * sbh_eval with SBH_EVAL_SORT_QUERIES.  Terms use the fast kernels.
  *
  * Trees must outlive the server.  `queries` (column-major dim x q) and
  * `results` (q floats) are host memory that must stay valid until the request

@@ -2,18 +2,21 @@
 //
 // Requests (a query set against one resident tree, Barnes-Hut or stochastic,
 // any kernel) are queued by sbh_server_submit.  A scheduler thread admits
-// queued requests in arrival order while their working sets (queries and
-// results) fit in the server's device arena, and launches every admitted
-// batch as ONE work-queue kernel: each wavefront repeatedly takes the next
-// (request, 64-query block) task and runs the matching walk, whatever the
-// tree, dimension, kernel or method.  Up to `max_inflight` launches run at once
+// queued requests in arrival order while their working sets (queries,
+// results, the queries' spatial order and its sort scratch) fit in the
+// server's device arena, puts each request's queries in Hilbert-curve order
+// (so a wavefront's 64 queries are neighbours and its walks coherent), and
+// launches every admitted batch as ONE work-queue kernel: each wavefront
+// repeatedly takes the next (request, 64-query block of the sorted order)
+// task and runs the matching walk, whatever the tree, dimension, kernel or
+// method.  Up to `max_inflight` launches run at once
 // on separate streams, so requests that arrive while others are on the GPU
 // are admitted as soon as memory frees and their work overlaps on the device.
 //
 // A request's values depend only on its own queries and parameters: the walks
-// are per query (Barnes-Hut without votes) or keyed on (seed, query block
-// within the request) (stochastic), so a result is identical alone and in any
-// mix.
+// are per query (Barnes-Hut without votes) or keyed on (seed, block of the
+// request's sorted order) (stochastic), so a result is identical alone and in
+// any mix, and equal to a one-shot sorted evaluation of the request.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,6 +48,7 @@ struct ReqDesc {
   const float* qx;
   const float* qy;
   const float* qz;
+  const uint32_t* order;  // query evaluated in each slot (Hilbert order)
   float* out;
   uint32_t nq;
   int32_t variant;  // ((dim == 3) * 3 + kernel) * 2 + (method == stochastic)
@@ -63,8 +67,8 @@ template <int D, int K>
 __device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, uint32_t block,
                                            bool live, float x, float y, float z,
                                            SamplerLds lds, SortedLds sorted_lds) {
-  // the same estimator as a one-shot evaluation of the request picks, so a
-  // served result equals the one-shot result
+  // the same estimator as a one-shot sorted evaluation of the request picks,
+  // so a served result equals that one-shot result
   if (stochastic && uniform_u32((uint32_t)r.sorted))
     return stochastic_estimate_sorted<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x,
                                                    y, z, sorted_lds)
@@ -100,10 +104,11 @@ __global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(5
     const ReqDesc& r = reqs[tk.req];
     const uint32_t slot = tk.block * 64 + lane;
     const bool live = slot < r.nq;
+    const uint32_t qi = live ? r.order[slot] : 0u;
     const int variant = (int)uniform_u32((uint32_t)r.variant);
     const int dim3 = variant >= 6;
     float x, y, z;
-    load_query(r.qx, r.qy, r.qz, live ? slot : 0, dim3 ? 3 : 2, x, y, z);
+    load_query(r.qx, r.qy, r.qz, qi, dim3 ? 3 : 2, x, y, z);
     const bool stoch = variant & 1;
     const SamplerLds c{sSrc + wbase, sNrm + wbase, sRec + wbase, sAux + wbase};
     const uint32_t wave = threadIdx.x >> 6;
@@ -130,11 +135,19 @@ __global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(5
         v = serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c, sc);
         break;
     }
-    if (live) r.out[slot] = v;
+    if (live) r.out[qi] = v;
   }
 }
 
 __global__ void k_reset_counter(uint32_t* p) { *p = 0; }
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Device bytes of a request in the arena: queries and results (floats), the
+// launch order (u32), and the scratch of the sort that makes it.
+size_t request_bytes(uint32_t nq, int dim) {
+  return align256((size_t)nq * (dim + 2) * 4) + align256(sort_scratch_u32(nq) * 4);
+}
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -282,7 +295,7 @@ int64_t Server::submit(int32_t tree_id, int32_t method, int32_t kernel, const sb
   r->results = results;
   r->nq = (uint32_t)q;
   r->dim = trees_[tree_id]->dim;
-  r->bytes = ((size_t)r->nq * (r->dim + 1) * sizeof(float) + 255) & ~(size_t)255;
+  r->bytes = request_bytes(r->nq, r->dim);
   if (r->bytes > arena_->size())
     throw Error(kErrBudget, "request working set (" + std::to_string(r->bytes) +
                                 " bytes) exceeds the server's HBM budget");
@@ -461,9 +474,13 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     const DeviceTree& t = *trees_[r->tree_id];
     float* dq = reinterpret_cast<float*>(base + r->off);
     float* dout = dq + (size_t)r->nq * r->dim;
+    uint32_t* dorder = reinterpret_cast<uint32_t*>(dout + r->nq);
+    uint32_t* dscratch =
+        reinterpret_cast<uint32_t*>(base + r->off + align256((size_t)r->nq * (r->dim + 2) * 4));
     const size_t qf = (size_t)r->nq * r->dim;
     std::memcpy(L.stage + so, r->queries, qf * sizeof(float));
     SBH_HIP(hipMemcpyAsync(dq, L.stage + so, qf * sizeof(float), hipMemcpyHostToDevice, s));
+    query_order(t, dq, r->nq, dorder, dscratch, s);
     so += qf;
     L.res_off.push_back(so);
     so += r->nq;
@@ -472,6 +489,7 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     d.qx = dq;
     d.qy = dq + r->nq;
     d.qz = r->dim == 3 ? dq + 2ull * r->nq : nullptr;
+    d.order = dorder;
     d.out = dout;
     d.nq = r->nq;
     d.variant = (((r->dim == 3) ? 3 : 0) + r->kernel) * 2 + (r->method == SBH_STOCHASTIC ? 1 : 0);

@@ -50,7 +50,8 @@ def alone(sbh_gpu, trees, r):
         fn = getattr(sbh_gpu, "eval_barnes_hut_" + kern)
         return fn(tree, r["queries"], r["beta"], vote_group=1)[0]
     fn = getattr(sbh_gpu, "multi_level_prefix_control_variate_" + kern)
-    return fn(tree, r["queries"], r["spl"], seed=r["seed"])[0]
+    # the engine evaluates each request in its queries' spatial order
+    return fn(tree, r["queries"], r["spl"], seed=r["seed"], sort_queries=True)[0]
 
 
 def test_results_identical_alone_and_in_mix(sbh_gpu, trees):
@@ -75,11 +76,17 @@ def test_results_identical_alone_and_in_mix(sbh_gpu, trees):
 def test_admission_by_budget_and_cancel(sbh_gpu, trees):
     kern, tree, lo, hi, dim = trees[0]
     q = 3000
-    per_request = ((q * (dim + 1) * 4 + 255) // 256) * 256
-    srv = sbh_gpu.Server(2 * per_request, 1)
-    tid = srv.add_tree(tree)
     rng = np.random.default_rng(5)
     pts = lo + (hi - lo) * rng.random((q, dim))
+    # one request's working set, as the engine accounts it
+    probe = sbh_gpu.Server(64 << 20, 1)
+    probe.add_tree(tree)
+    probe.submit(0, "stochastic", kern, pts, samples_per_subdomain=1).wait(60.0)
+    per_request = probe.stats()["peak_bytes"]
+    assert q * (dim + 1) * 4 < per_request < q * (dim + 12) * 4
+    del probe
+    srv = sbh_gpu.Server(2 * per_request, 1)
+    tid = srv.add_tree(tree)
     hs = [srv.submit(tid, "stochastic", kern, pts, samples_per_subdomain=8, seed=i)
           for i in range(12)]
     cancelled = srv.cancel(hs[-1])
