@@ -108,7 +108,8 @@ def test_loaded_tree_serves(sbh_gpu):
     tid = srv.add_tree(u)
     r = srv.submit(tid, "stochastic", "gravity", q, samples_per_subdomain=2, seed=9)
     got, _ = r.result()
-    ref, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(t, q, 2, seed=9)
-    # same sample paths (seed, 64-query block); the serving kernel is a
-    # different compilation of the same fast-math terms
+    ref, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(t, q, 2, seed=9,
+                                                                 sort_queries=True)
+    # same sample paths (seed, 64-query block of the sorted order); the serving
+    # kernel is a different compilation of the same fast-math terms
     np.testing.assert_allclose(got, ref, rtol=1e-5)
