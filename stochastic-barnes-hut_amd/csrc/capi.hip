@@ -30,6 +30,8 @@ struct sbh_tree {
   sbh::DevBuf<double> partial;
   sbh::DevBuf<double> out64;
   sbh::DevBuf<double> qd;  // double build: queries
+  sbh::DevBuf<double> memo64;  // double build: the estimator's level-1 memo
+  sbh::DevBuf<int32_t> memo_cnt;
   sbh::DevBuf<uint32_t> order;
   sbh::DevBuf<uint32_t> sort_scratch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -663,6 +665,18 @@ int sbh_eval_double(sbh_tree* tree, int32_t method, int32_t kernel,
         dcnt = tree->counts.get();
       }
     }
+    // the estimator's level-1 memo: 64 doubles (and with counts 128 int32) a
+    // query, kept by the tree and grown on demand; skipped beyond 4 GiB
+    double* memo = nullptr;
+    int32_t* memo_cnt = nullptr;
+    if (method == SBH_STOCHASTIC && (size_t)nq * 64 * sizeof(double) <= ((size_t)4 << 30)) {
+      ensure(tree->memo64, (size_t)nq * 64);
+      memo = tree->memo64.get();
+      if (dcnt) {
+        ensure(tree->memo_cnt, (size_t)nq * 128);
+        memo_cnt = tree->memo_cnt.get();
+      }
+    }
     const bool sort = method != SBH_BRUTE_FORCE && (p.flags & SBH_EVAL_SORT_QUERIES) != 0;
     uint32_t* ord = nullptr;
     if (sort) {
@@ -680,7 +694,7 @@ int sbh_eval_double(sbh_tree* tree, int32_t method, int32_t kernel,
     else
       sbh::run_stochastic_double(t, kernel, p.samples_per_subdomain, p.seed,
                                  (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout,
-                                 dcnt, split, s);
+                                 dcnt, split, memo, memo_cnt, s);
     SBH_HIP(hipEventRecord(tree->ev1, s));
     if (method == SBH_BRUTE_FORCE && dcnt) {
       // every query evaluates every point; split counts have no node terms

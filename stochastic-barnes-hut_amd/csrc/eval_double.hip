@@ -226,10 +226,42 @@ __global__ __launch_bounds__(kThreads) void k_bh_double(TreeViewD t, double beta
 // wavefront (the paths the wave shares) or the query itself (per-query
 // paths).  Level 0 of a subdomain is the same for every sample (its roulette
 // keeps every path, weight 1) and is evaluated once.
+//
+// Level 1 is memoised: the path node there is one of the subdomain's <= 64
+// children, and its diff (children's sum minus its own term) depends only on
+// that node and the query, so each lane keeps the diff of every level-1 node
+// it has evaluated, with the term counts, in a scratch column of its own
+// (memo.diff[j * stride + slot], j = the child's index in the subdomain; a
+// wavefront's lanes have consecutive slots, so these accesses coalesce).  A
+// reused diff is the recomputed one bit for bit, and every sample still adds
+// its own weight in sample order.
 // ---------------------------------------------------------------------------
+// Index of the child of a node (children f .. f + nch - 1, in point order)
+// whose point range holds point idx: the last child starting at or before
+// idx, by bisection.
+__device__ __forceinline__ uint32_t child_holding(const uint32_t* n_start, uint32_t f,
+                                                  uint32_t nch, uint32_t idx) {
+  uint32_t lo = 0, hi = nch;  // n_start[f + lo] <= idx
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (n_start[f + mid] <= idx)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct Memo {
+  double* diff;  // nullptr: no memo
+  int32_t* cnt;  // optional: {point terms, node terms} per entry
+  uint32_t stride;
+  uint32_t slot;
+};
+
 template <int D, int K>
 __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, uint32_t key1,
-                                   uint32_t g, double x, double y, double z) {
+                                   uint32_t g, double x, double y, double z, const Memo& memo) {
 #pragma clang fp contract(off)
   int32_t npts = 0, nnodes = 0;
   const uint4 rl = t.nodeL[0];
@@ -255,6 +287,8 @@ __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, u
       count_terms(node_accumulate_d<D, K>(t, f0 + j, x, y, z, r0sum), p0, n0);
     const double prev0 = kernel_double<D, K>(t.nodeC[S], t.nodeN[S], t.alpha, x, y, z);
     const double wd0 = (double)nS / (1.0 * (double)nS) * (r0sum - prev0);
+    const bool use_memo = memo.diff != nullptr && nch0 <= 64;
+    uint64_t memo_valid = 0;  // level-1 nodes of this subdomain whose diff is stored
     double sub_total = 0.0;
     for (int ss = 0; ss < spl; ++ss) {
       const Philox4 r0 = philox4x32_10((uint32_t)ss, si, g, 0u, key0, key1);
@@ -264,14 +298,14 @@ __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, u
       double event_prob = 1.0;
       double wd_all = 0.0;
       double cur_beta = fmax(cS, fmax(1.0, cS));
-      uint32_t pick = 0;
-      for (uint32_t j = 1; j < nch0; ++j) pick = t.n_start[f0 + j] <= idx ? j : pick;
+      const uint32_t pick = child_holding(t.n_start, f0, nch0, idx);
       uint32_t cur = t.n_skip[f0 + pick];
       wd_all = wd_all + wd0;
       npts += p0;
       nnodes += n0;
       uint32_t level = 1;
       uint4 curL = t.nodeL[cur];
+      bool at1 = true;  // the path node is the level-1 node S's child `pick` leads to
       while (!(curL.w & kLeafBit)) {
         const uint32_t wi = level + 2;
         if ((wi >> 2) != blk_id) {
@@ -291,19 +325,39 @@ __device__ LaneD stochastic_double(const TreeViewD& t, int spl, uint32_t key0, u
         event_prob = event_prob * tp;
         const uint32_t prev = cur;
         const uint32_t f = curL.y, nch = curL.w;
-        double cur_result = 0.0;
-        uint32_t pk = 0;
-        for (uint32_t j = 0; j < nch; ++j) {
-          const uint32_t c = f + j;
-          count_terms(node_accumulate_d<D, K>(t, c, x, y, z, cur_result), npts, nnodes);
-          pk = t.n_start[c] <= idx ? j : pk;  // children are in point order
+        const uint32_t pk = child_holding(t.n_start, f, nch, idx);
+        double diff;
+        const bool memo_here = use_memo && at1;
+        const size_t me = (size_t)pick * memo.stride + memo.slot;
+        if (memo_here && ((memo_valid >> pick) & 1ull)) {
+          diff = memo.diff[me];
+          if (memo.cnt) {
+            npts += memo.cnt[2 * me];
+            nnodes += memo.cnt[2 * me + 1];
+          }
+        } else {
+          double cur_result = 0.0;
+          int32_t pc = 0, nc = 0;
+          for (uint32_t j = 0; j < nch; ++j)
+            count_terms(node_accumulate_d<D, K>(t, f + j, x, y, z, cur_result), pc, nc);
+          const double prev_result =
+              kernel_double<D, K>(t.nodeC[prev], t.nodeN[prev], t.alpha, x, y, z);
+          diff = cur_result - prev_result;
+          npts += pc;
+          nnodes += nc;
+          if (memo_here) {
+            memo.diff[me] = diff;
+            if (memo.cnt) {
+              memo.cnt[2 * me] = pc;
+              memo.cnt[2 * me + 1] = nc;
+            }
+            memo_valid |= 1ull << pick;
+          }
         }
+        at1 = false;
         cur = t.n_skip[f + pk];
         curL = t.nodeL[cur];
         cur_beta = fmax(cur_beta, ffr);
-        const double prev_result =
-            kernel_double<D, K>(t.nodeC[prev], t.nodeN[prev], t.alpha, x, y, z);
-        const double diff = cur_result - prev_result;
         const double n_prev = (double)(t.n_end[prev] - t.n_start[prev]);
         wd_all = wd_all + (double)nS / (event_prob * n_prev) * diff;
       }
@@ -322,14 +376,16 @@ __global__ __launch_bounds__(kThreads) void k_stochastic_double(TreeViewD t, int
                                                                 const double* qz, uint32_t nq,
                                                                 const uint32_t* order,
                                                                 double* out, int32_t* counts,
-                                                                int split) {
+                                                                int split, double* memo_diff,
+                                                                int32_t* memo_cnt) {
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
   if (slot >= nq) return;
   const uint32_t i = order ? order[slot] : slot;
   const double x = qx[i], y = qy[i], z = D == 3 ? qz[i] : 0.0;
   // paths: the query's own, or those of its wavefront in launch order
   const uint32_t g = per_query ? i : slot / 64u;
-  store_result(i, nq, stochastic_double<D, K>(t, spl, key0, key1, g, x, y, z), out, counts,
+  const Memo memo{memo_diff, memo_cnt, nq, slot};
+  store_result(i, nq, stochastic_double<D, K>(t, spl, key0, key1, g, x, y, z, memo), out, counts,
                split);
 }
 
@@ -401,10 +457,11 @@ template <int D, int K>
 struct StochL {
   static void run(const DeviceTree& t, int spl, uint64_t seed, bool per_query, const double* q,
                   uint32_t nq, const uint32_t* order, double* out, int32_t* counts, int split,
-                  hipStream_t s) {
+                  double* memo_diff, int32_t* memo_cnt, hipStream_t s) {
     k_stochastic_double<D, K><<<nblk(nq, kThreads), kThreads, 0, s>>>(
         view_double(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), per_query ? 1 : 0, q,
-        q + nq, D == 3 ? q + 2ull * nq : nullptr, nq, order, out, counts, split);
+        q + nq, D == 3 ? q + 2ull * nq : nullptr, nq, order, out, counts, split, memo_diff,
+        memo_cnt);
   }
 };
 
@@ -425,8 +482,10 @@ void run_bh_double(const DeviceTree& t, int kernel, double beta, int group, cons
 
 void run_stochastic_double(const DeviceTree& t, int kernel, int spl, uint64_t seed,
                            bool per_query, const double* q, uint32_t nq, const uint32_t* order,
-                           double* out, int32_t* counts, int split, hipStream_t s) {
-  dispatch<StochL>(t.dim, kernel, t, spl, seed, per_query, q, nq, order, out, counts, split, s);
+                           double* out, int32_t* counts, int split, double* memo_diff,
+                           int32_t* memo_cnt, hipStream_t s) {
+  dispatch<StochL>(t.dim, kernel, t, spl, seed, per_query, q, nq, order, out, counts, split,
+                   memo_diff, memo_cnt, s);
   SBH_HIP(hipGetLastError());
 }
 

@@ -54,9 +54,12 @@ void run_brute_double(const DeviceTree& t, int kernel, const double* q, uint32_t
 void run_bh_double(const DeviceTree& t, int kernel, double beta, int group, const double* q,
                    uint32_t nq, const uint32_t* order, double* out, int32_t* counts, int split,
                    hipStream_t s);
+// memo_diff (optional): 64 * nq doubles of level-1 memo; memo_cnt (optional,
+// with counts): 128 * nq int32.
 void run_stochastic_double(const DeviceTree& t, int kernel, int spl, uint64_t seed,
                            bool per_query, const double* q, uint32_t nq, const uint32_t* order,
-                           double* out, int32_t* counts, int split, hipStream_t s);
+                           double* out, int32_t* counts, int split, double* memo_diff,
+                           int32_t* memo_cnt, hipStream_t s);
 // Spatial order of float64 queries (their float rounding's Hilbert order);
 // qf holds nq * dim floats, scratch sort_scratch_u32(nq) values.
 void query_order_double(const DeviceTree& t, const double* q, uint32_t nq, float* qf,
