@@ -224,3 +224,61 @@ class Tree:
         check(lib().sbh_eval_f64(self.h, k, soa.ctypes.data, len(q), out.ctypes.data, 0,
                                  ctypes.byref(ms)))
         return out, ms.value
+
+
+class TreeDouble(Tree):
+    """Double-build tree handle (sbh_tree_create_double): float64 inputs,
+    tree, terms and results, evaluated by sbh_eval_double."""
+
+    def __init__(self, points, normals, masses, min_corner, max_corner, alpha=1.0,
+                 split_log=1, leaf_size=1, max_depth=100):
+        L = lib()
+        pts = np.ascontiguousarray(np.asarray(points, np.float64))
+        n, d = pts.shape
+        nrm = None if normals is None else np.ascontiguousarray(
+            np.asarray(normals, np.float64).reshape(n, d))
+        m = np.ascontiguousarray(np.asarray(masses, np.float64).reshape(n))
+        lo = np.ascontiguousarray(np.asarray(min_corner, np.float64).reshape(d))
+        hi = np.ascontiguousarray(np.asarray(max_corner, np.float64).reshape(d))
+        self.dim, self.n, self.split_log = d, n, split_log
+        h = ctypes.c_void_p()
+        ms = ctypes.c_float()
+        check(L.sbh_tree_create_double(d, split_log, leaf_size, max_depth, n, pts.ctypes.data,
+                                       None if nrm is None else nrm.ctypes.data, m.ctypes.data,
+                                       lo.ctypes.data, hi.ctypes.data, float(alpha),
+                                       ctypes.byref(h), ctypes.byref(ms)))
+        self.h = h
+        self.build_ms = ms.value
+
+    def eval_device(self, method, kernel, q_ptr, nq, out_ptr, counts_ptr=0, beta=0.0,
+                    samples_per_subdomain=1, seed=0, vote_group=1, flags=0) -> float:
+        """Evaluate with device pointers (float64 queries column-major d x nq,
+        float64 results); returns the hipEvent-timed kernel milliseconds."""
+        p = EvalParamsDouble(beta, samples_per_subdomain, seed, vote_group,
+                             flags | EVAL_DEVICE_POINTERS)
+        ms = ctypes.c_float()
+        k = KERNELS[kernel] if isinstance(kernel, str) else kernel
+        check(lib().sbh_eval_double(self.h, method, k, ctypes.byref(p), ctypes.c_void_p(q_ptr),
+                                    nq, ctypes.c_void_p(out_ptr),
+                                    ctypes.c_void_p(counts_ptr or None), ctypes.byref(ms)))
+        return ms.value
+
+    def eval_host(self, method, kernel, queries, beta=0.0, samples_per_subdomain=1, seed=0,
+                  vote_group=1, flags=0, counts=False):
+        q = np.asarray(queries, np.float64).reshape(-1, self.dim)
+        soa = np.ascontiguousarray(q.T)
+        nq = len(q)
+        out = np.empty(nq, np.float64)
+        cnt = np.empty(nq * (2 if flags & EVAL_SPLIT_COUNTS else 1), np.int32) if counts else None
+        p = EvalParamsDouble(beta, samples_per_subdomain, seed, vote_group, flags)
+        ms = ctypes.c_float()
+        k = KERNELS[kernel] if isinstance(kernel, str) else kernel
+        check(lib().sbh_eval_double(self.h, method, k, ctypes.byref(p), soa.ctypes.data, nq,
+                                    out.ctypes.data, None if cnt is None else cnt.ctypes.data,
+                                    ctypes.byref(ms)))
+        return (out, ms.value, cnt) if counts else (out, ms.value)
+
+    def eval_f64_device(self, *a, **kw):
+        raise SbhError(ERR_INVALID, "a double-build tree evaluates in float64 already: eval_*")
+
+    eval_f64_host = eval_f64_device

@@ -253,3 +253,28 @@ def test_double_edge_trees(sbh_gpu64, shape):
     # no queries: empty results, no launch
     empty, ms = sbh_gpu64.eval_barnes_hut_gravity(g, np.zeros((0, 3)), 2.0)
     assert empty.shape == (0,) and ms == 0.0
+
+
+def test_double_build_through_ctypes_device_pointers(sbh_gpu64, case):
+    """The ctypes binding's double-build handle with device-resident float64
+    buffers (SBH_EVAL_DEVICE_POINTERS) gives the module's results."""
+    import torch
+
+    import sbh_capi as C
+
+    dim, k, qs, g, o = case
+    t = C.TreeDouble(o.points, o.normals, o.masses, o.min_corner, o.max_corner, alpha=2.0,
+                     split_log=k)
+    dev = torch.device("cuda:0")
+    qd = torch.from_numpy(np.ascontiguousarray(qs.T)).to(dev)
+    out = torch.empty(len(qs), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t.eval_device(C.BARNES_HUT, "gravity", qd.data_ptr(), len(qs), out.data_ptr(), beta=2.0,
+                  vote_group=32)
+    torch.cuda.synchronize()
+    ref, _ = sbh_gpu64.eval_barnes_hut_gravity(g, qs, 2.0, vote_group=32)
+    _same_bits(out.cpu().numpy(), ref)
+    st, _ = t.eval_host(C.STOCHASTIC, "wn", qs, samples_per_subdomain=2, seed=3)
+    ref, _ = sbh_gpu64.multi_level_prefix_control_variate_wn(g, qs, 2, seed=3)
+    _same_bits(st, ref)
+    t.close()
