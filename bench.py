@@ -382,9 +382,10 @@ def main():
     precision = None if args.no_variants else precision_variant(args)
     double_build = None if args.no_variants else double_build_variant(args)
     serving = None if args.no_serve else serving_bench(args)
-    cpu = None
+    cpu = cpu_c1 = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, hs["samples_per_subdomain"], hs["tree_k"], args.cpu_seconds, value)
+        cpu_c1 = cpu_baseline_c1()
 
     line = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1,
@@ -411,6 +412,7 @@ def main():
                             "kind": "per 2^20-query evaluation, wall clock"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_c1": cpu_c1,
         "speedup_vs_brute_force": value / head["brute_force"]["queries_per_s"],
         "brute_force": head["brute_force"],
         "brute_force_mfma": head.get("brute_force_mfma"),
@@ -705,6 +707,75 @@ def cpu_baseline(wl, spl, k, budget_s, gpu_qps):
                       f"(k={k}), one sample path per query; tree build excluded "
                       f"({build_s:.1f} s single-threaded)",
             "gpu_over_cpu": gpu_qps / qps}
+
+
+def cpu_baseline_c1():
+    """CPU baseline leg on the reference's CPU-runnable case (3D gravity, 16384
+    uniform sources with masses U[0.5,1.5], 16384 queries in [-0.1,1.1]^3):
+    brute force, Barnes-Hut (beta 2, leaf sizes 8 and 1, per query) and the
+    stochastic estimator (256 samples per query: octree k=2, 4 per subdomain),
+    the oracle on the host cores (best of 3) beside the GPU, each against the
+    float64 all-pairs sum."""
+    import oracle
+    import sbh_gpu
+    import workloads as W
+
+    threads = min(16, os.cpu_count() or 1)
+    pts, _, m, q = W.uniform_cube(16384, 16384, seed=0)
+    lo, hi = W.bounds(pts)
+    nrm = np.zeros_like(pts)
+    q32 = q.astype(np.float32)
+    nq = len(q32)
+    g1 = sbh_gpu.OctreeGPU1(pts, nrm, m, lo, hi)
+    ref, _ = sbh_gpu.eval_brute_force_f64_gravity(g1, q32)
+
+    def err(v):
+        return float(np.mean(np.abs(np.asarray(v, np.float64) - ref) / np.abs(ref)))
+
+    def cpu(fn):
+        best, v = float("inf"), None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            v = fn()
+            best = min(best, time.perf_counter() - t0)
+        return v, best
+
+    def gpu(fn, *a, **kw):
+        fn(*a, **kw)
+        v, ms = fn(*a, **kw)
+        return v, ms
+
+    rows = {}
+
+    def row(name, cv, cs, gv, gms, **extra):
+        rows[name] = dict(cpu_s=cs, cpu_queries_per_s=nq / cs, cpu_mean_rel_err=err(cv),
+                          gpu_kernel_ms=gms, gpu_queries_per_s=nq / (gms / 1e3),
+                          gpu_mean_rel_err=err(gv), gpu_over_cpu=(nq / (gms / 1e3)) / (nq / cs),
+                          **extra)
+        log(f"C1 {name}: cpu {cs * 1e3:.1f} ms (err {err(cv):.2e}), gpu {gms:.3f} ms "
+            f"(err {err(gv):.2e})")
+
+    o1 = oracle.Tree(pts, None, m, lo, hi, split_log=1)
+    cv, cs = cpu(lambda: o1.brute_force("gravity", q32, nthreads=threads))
+    gv, gms = gpu(sbh_gpu.eval_brute_force_gravity, g1, q32)
+    row("brute_force", cv, cs, gv, gms)
+    for leaf in (8, 1):
+        ot = o1 if leaf == 1 else oracle.Tree(pts, None, m, lo, hi, split_log=1, leaf_size=leaf)
+        gt = g1 if leaf == 1 else sbh_gpu.OctreeGPU1(pts, nrm, m, lo, hi, leaf_size=leaf)
+        cv, cs = cpu(lambda: ot.barnes_hut("gravity", q32, 2.0, vote_group=1, nthreads=threads))
+        gv, gms = gpu(sbh_gpu.eval_barnes_hut_gravity, gt, q32, 2.0, vote_group=1,
+                      sort_queries=True)
+        row(f"barnes_hut_beta2_leaf{leaf}", cv, cs, gv, gms)
+    o2 = oracle.Tree(pts, None, m, lo, hi, split_log=2)
+    g2 = sbh_gpu.OctreeGPU2(pts, nrm, m, lo, hi)
+    cv, cs = cpu(lambda: o2.stochastic("gravity", q32, 4, seed=1, group=1, nthreads=threads))
+    gv, gms = gpu(sbh_gpu.multi_level_prefix_control_variate_gravity, g2, q32, 4, seed=1,
+                  sort_queries=True)
+    row("stochastic_256_per_query", cv, cs, gv, gms,
+        paths="CPU: one sample path per query; GPU: paths shared by a wavefront's 64 queries")
+    return {"workload": "3D gravity, uniform unit cube N=16384 (masses U[0.5,1.5]), Q=16384 "
+                        "queries in [-0.1,1.1]^3, fp32", "cores": threads, "kind": "port",
+            "methods": rows}
 
 
 if __name__ == "__main__":

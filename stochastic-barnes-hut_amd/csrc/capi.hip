@@ -552,10 +552,20 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
         ensure(tree->sort_scratch, sbh::sort_scratch_u32(nq));
         ord = tree->order.get();
       }
+      float* bh_partial = nullptr;
+      if (method == SBH_BARNES_HUT) {
+        const uint32_t ns = sbh::bh_split_count(t, nq, p.vote_group, exact, dcnt != nullptr);
+        if (ns > 1) {
+          // ns * nq floats (rounded up to an even count), then ns * nq int2 counts
+          ensure(tree->partial, ((size_t)ns * nq + 1) / 2 + (dcnt ? (size_t)ns * nq : 0));
+          bh_partial = reinterpret_cast<float*>(tree->partial.get());
+        }
+      }
       SBH_HIP(hipEventRecord(tree->ev0, s));
       if (sort) sbh::query_order(t, dq, nq, ord, tree->sort_scratch.get(), s);
       if (method == SBH_BARNES_HUT)
-        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, exact, s);
+        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, exact,
+                    bh_partial, s);
       else
         sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed,
                             (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout, dcnt,

@@ -170,6 +170,75 @@ __global__ __launch_bounds__(kThreads) void k_bh(TreeView t, float beta,
   }
 }
 
+// Barnes-Hut per query for launches too small to fill the GPU: the walk below
+// the root is split over the root's children (blockIdx.y = child), each task
+// walking one subtree for 64 queries, and the partial sums are added in child
+// order afterwards.  A query whose opening test accepts the root takes the
+// root's moment (in the first child's task) and nothing else, as the whole
+// walk would.  Fast terms only: the association of the sum changes.  With
+// COUNT the tasks also count their terms (pcnt: {nodes, points} per task).
+template <int D, int K, bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_bh_split(TreeView t, float beta,
+                                                        const float* __restrict__ qx,
+                                                        const float* __restrict__ qy,
+                                                        const float* __restrict__ qz, uint32_t nq,
+                                                        const uint32_t* __restrict__ order,
+                                                        float* __restrict__ partial,
+                                                        int2* __restrict__ pcnt) {
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = slot < nq;
+  const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
+  float x, y, z;
+  load_query(qx, qy, qz, qi, D, x, y, z);
+  const WalkRecord root = t.dfsR[0];
+  // the root's opening test, decided exactly as the whole walk decides it
+  const float diam = __uint_as_float(root.l.x);
+  int far = opens_far_fast<D>(root.c, __builtin_amdgcn_rcpf(diam), beta, x, y, z);
+  if (far < 0) far = (int)opens_far_exact<D>(root.c, diam, beta, x, y, z);
+  const bool take_root = far != 0;
+  // depth-first position of the root's child blockIdx.y: follow the sibling ropes
+  uint32_t v0 = 1;
+  for (uint32_t j = 0; j < blockIdx.y; ++j) v0 = uniform_u32(t.dfsR[v0].l.z);
+  const uint32_t vstop = uniform_u32(t.dfsR[v0].l.z);
+  float acc = 0.0f;
+  if (blockIdx.y == 0 && take_root)
+    acc = accum<D, K, false>(acc, root.c, t.dfsN[0], t.alpha, t.alpha_k, x, y, z);
+  const LaneResult r =
+      bh_walk<D, K, 1, false, COUNT>(t, beta, live && !take_root, x, y, z, v0, vstop);
+  if (live) {
+    const size_t e = (size_t)blockIdx.y * nq + slot;
+    partial[e] = take_root ? acc : r.value;
+    if constexpr (COUNT)
+      pcnt[e] = take_root ? make_int2(blockIdx.y == 0 ? 1 : 0, 0) : make_int2(r.nodes, r.points);
+  }
+}
+
+__global__ void k_reduce_bh_split(const float* __restrict__ partial,
+                                  const int2* __restrict__ pcnt, uint32_t nsplit, uint32_t nq,
+                                  const uint32_t* __restrict__ order, float* __restrict__ out,
+                                  int32_t* __restrict__ counts, int split_counts) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nq) return;
+  float s = partial[slot];
+  for (uint32_t k = 1; k < nsplit; ++k) s += partial[(size_t)k * nq + slot];
+  const uint32_t qi = order ? order[slot] : slot;
+  out[qi] = s;
+  if (counts) {
+    int32_t nodes = 0, points = 0;
+    for (uint32_t k = 0; k < nsplit; ++k) {
+      const int2 c = pcnt[(size_t)k * nq + slot];
+      nodes += c.x;
+      points += c.y;
+    }
+    if (split_counts) {
+      counts[qi] = nodes;
+      counts[nq + qi] = points;
+    } else {
+      counts[qi] = nodes + points;
+    }
+  }
+}
+
 template <int D, int K, bool EXACT>
 __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, uint32_t key0,
                                                           uint32_t key1,
@@ -454,9 +523,24 @@ void launch_brute_f64(const DeviceTree& t, const float* qx, const float* qy, con
 template <int D, int K, bool EXACT>
 void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, const float* qy,
                const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
-               int split, hipStream_t s) {
+               int split, float* partial, uint32_t nsplit, hipStream_t s) {
   uint32_t grid = nblk(nq, kThreads);
   const TreeView tv = view_of(t);
+  if (!EXACT && nsplit > 1) {
+    // partial: nsplit * nq floats, then (with counts) nsplit * nq int2
+    int2* pcnt = reinterpret_cast<int2*>(partial + (((size_t)nsplit * nq + 1) & ~(size_t)1));
+    if (counts)
+      k_bh_split<D, K, true><<<dim3(grid, nsplit), kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq,
+                                                                      order, partial, pcnt);
+    else
+      k_bh_split<D, K, false><<<dim3(grid, nsplit), kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq,
+                                                                       order, partial, pcnt);
+    SBH_HIP(hipGetLastError());
+    k_reduce_bh_split<<<nblk(nq, 256), 256, 0, s>>>(partial, pcnt, nsplit, nq, order, out,
+                                                     counts, split);
+    SBH_HIP(hipGetLastError());
+    return;
+  }
 #define SBH_LAUNCH_BH(G, COUNT)                                                      \
   k_bh<D, K, G, EXACT, COUNT><<<grid, kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq, order, out, \
                                                          counts, split)
@@ -573,17 +657,30 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
   });
 }
 
+uint32_t bh_split_count(const DeviceTree& t, uint32_t nq, int group, bool exact, bool counts) {
+  // fewer than four wavefronts per SIMD (1024 SIMDs): split the walk over the
+  // root's children, if there are between 2 and 64 of them
+  (void)counts;  // counted launches split the same way (the values must not depend on it)
+  if (exact || group != 1 || t.is_double || t.level_begin.size() < 3) return 1;
+  const uint32_t nsub = t.level_begin[2] - t.level_begin[1];
+  if (nsub < 2 || nsub > 64 || nq >= 4u * 1024u * 64u) return 1;
+  return nsub;
+}
+
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
             const uint32_t* order, float* out, int32_t* counts, int split, bool exact,
-            hipStream_t s) {
+            float* partial, hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
+  const uint32_t ns = partial ? bh_split_count(t, nq, group, exact, counts != nullptr) : 1;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
     if (exact)
-      launch_bh<D, K, true>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, s);
+      launch_bh<D, K, true>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, partial,
+                            1, s);
     else
-      launch_bh<D, K, false>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, s);
+      launch_bh<D, K, false>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, partial,
+                             ns, s);
   });
 }
 

@@ -34,9 +34,12 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
 // and point terms in counts[nq, 2 nq).  exact: every term with correctly
 // rounded fp32 operations in the reference's order (otherwise hardware
 // sqrt / reciprocal / exp2).
+// partial (optional): bh_split_count(...) * nq floats; launches too small to
+// fill the GPU then split the per-query fast walk over the root's subtrees.
+uint32_t bh_split_count(const DeviceTree& t, uint32_t nq, int group, bool exact, bool counts);
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
             const uint32_t* order, float* out, int32_t* counts, int split, bool exact,
-            hipStream_t s);
+            float* partial, hipStream_t s);
 // per_query: one sample path per query keyed by its index (otherwise the 64
 // queries of a wavefront share their paths, keyed by the wavefront's index).
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,

@@ -105,10 +105,13 @@ __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
 // correctly rounded operations, so every lane opens exactly the nodes the
 // reference walk opens.
 // ---------------------------------------------------------------------------
-// COUNT: also count the terms (the counting mode).
+// COUNT: also count the terms (the counting mode).  [v0, vstop): the walk
+// starts at depth-first position v0 and ends where the ropes reach vstop (the
+// whole tree by default; a subtree's position and its rope walk the subtree).
 template <int D, int K, int GROUP, bool EXACT, bool COUNT = true>
 __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, bool live, float x,
-                                              float y, float z) {
+                                              float y, float z, uint32_t v0 = 0,
+                                              uint32_t vstop = kEnd) {
   const uint32_t lane = threadIdx.x & 63;
   uint64_t my_group;
   if constexpr (GROUP == 64) {
@@ -124,10 +127,10 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   int32_t cnt = 0, cnt_nodes = 0;
   bool active = live;
   uint32_t resume = kEnd;
-  uint32_t v = 0;
-  float4 c = rec[0].c;
-  uint4 l = rec[0].l;
-  while (v != kEnd) {
+  uint32_t v = v0;
+  float4 c = rec[v0].c;
+  uint4 l = rec[v0].l;
+  while (v != vstop) {
     v = uniform_u32(v);
     active = active || (live && resume == v);
     const bool leaf = (l.w & kLeafBit) != 0;
@@ -174,7 +177,7 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       next = l.y;
     }
     next = uniform_u32(next);
-    if (next != kEnd) {
+    if (next != vstop) {
       const WalkRecord r = rec[next];
       c = r.c;
       l = r.l;
