@@ -560,6 +560,14 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
           ensure(tree->partial, ((size_t)ns * nq + 1) / 2 + (dcnt ? (size_t)ns * nq : 0));
           bh_partial = reinterpret_cast<float*>(tree->partial.get());
         }
+      } else {
+        const uint32_t ns = sbh::stochastic_split_count(
+            t, nq, (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, exact);
+        if (ns > 1) {
+          // ns * nq float2 parts, then ns * nq int2 counts
+          ensure(tree->partial, (size_t)ns * nq * (dcnt ? 2 : 1));
+          bh_partial = reinterpret_cast<float*>(tree->partial.get());
+        }
       }
       SBH_HIP(hipEventRecord(tree->ev0, s));
       if (sort) sbh::query_order(t, dq, nq, ord, tree->sort_scratch.get(), s);
@@ -569,7 +577,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       else
         sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed,
                             (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout, dcnt,
-                            split, exact, s);
+                            split, exact, bh_partial, s);
       SBH_HIP(hipEventRecord(tree->ev1, s));
     }
     if (p.flags & SBH_EVAL_SMOOTH_DISTANCE) sbh::run_smooth_distance(dout, nq, t.alpha, s);

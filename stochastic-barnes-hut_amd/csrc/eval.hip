@@ -319,6 +319,79 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
   }
 }
 
+// The sorted-sample estimator for launches too small to fill the GPU: one
+// task per (64 queries, root subdomain) (blockIdx.y = subdomain), each
+// writing its Barnes-Hut and control-variate parts; k_reduce_stoch_split adds
+// the parts in subdomain order.  Same sample paths (keyed by the wavefront's
+// slot block and the subdomain) and terms; the fast terms' fused
+// accumulation rounds differently across the split, so values agree with
+// the whole estimate to fp32 rounding.
+template <int D, int K, bool COUNT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void
+k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
+                          const float* __restrict__ qx, const float* __restrict__ qy,
+                          const float* __restrict__ qz, uint32_t nq,
+                          const uint32_t* __restrict__ order, float2* __restrict__ partial,
+                          int2* __restrict__ pcnt) {
+  constexpr uint32_t kWaves = kThreads / 64;
+  __shared__ float4 sSrc[kThreads];
+  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
+  __shared__ uint2 sRec[kThreads];
+  __shared__ uint2 sKeys[kWaves * chunk_for<K>()];
+  __shared__ float sDiff[kWaves * kCacheLevels * 64];
+  __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t wbase = threadIdx.x & ~63u;
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = slot < nq;
+  const uint32_t g = uniform_u32(slot >> 6);
+  const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
+  float x, y, z;
+  load_query(qx, qy, qz, qi, D, x, y, z);
+  const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
+                      sKeys + wave * chunk_for<K>(), sDiff + wave * kCacheLevels * 64,
+                      sNext + wave * (kCacheLevels + 1) * 3};
+  float bh = 0.0f;
+  const LaneResult r = stochastic_estimate_sorted<D, K, COUNT, true>(
+      t, spl, key0, key1, g, live, x, y, z, lds, blockIdx.y, blockIdx.y + 1, &bh);
+  if (live) {
+    const size_t e = (size_t)blockIdx.y * nq + slot;
+    partial[e] = make_float2(bh, r.value);
+    if constexpr (COUNT) pcnt[e] = make_int2(r.nodes, r.points);
+  }
+}
+
+__global__ void k_reduce_stoch_split(const float2* __restrict__ partial,
+                                     const int2* __restrict__ pcnt, uint32_t nsplit,
+                                     uint32_t nq, const uint32_t* __restrict__ order,
+                                     float* __restrict__ out, int32_t* __restrict__ counts,
+                                     int split_counts) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nq) return;
+  float bh = 0.0f, mlcv = 0.0f;
+  for (uint32_t k = 0; k < nsplit; ++k) {
+    const float2 p = partial[(size_t)k * nq + slot];
+    bh += p.x;
+    mlcv += p.y;
+  }
+  const uint32_t qi = order ? order[slot] : slot;
+  out[qi] = bh + mlcv;
+  if (counts) {
+    int32_t nodes = 0, points = 0;
+    for (uint32_t k = 0; k < nsplit; ++k) {
+      const int2 c = pcnt[(size_t)k * nq + slot];
+      nodes += c.x;
+      points += c.y;
+    }
+    if (split_counts) {
+      counts[qi] = nodes;
+      counts[nq + qi] = points;
+    } else {
+      counts[qi] = nodes + points;
+    }
+  }
+}
+
 // One sample path per query: thread = query, path key = the query's index in
 // the caller's array.
 template <int D, int K, bool EXACT>
@@ -559,9 +632,29 @@ void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, cons
 template <int D, int K, bool EXACT>
 void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, bool per_query, const float* qx,
                   const float* qy, const float* qz, uint32_t nq, const uint32_t* order,
-                  float* out, int32_t* counts, int split, hipStream_t s) {
+                  float* out, int32_t* counts, int split, float* partial, uint32_t nsplit,
+                  hipStream_t s) {
   // nodes have at most 2^(dim * split_log) children
   const bool small_fanout = t.dim * t.split_log <= 6;
+  if (!EXACT && small_fanout && !per_query && nsplit > 1) {
+    // partial: nsplit * nq float2, then (with counts) nsplit * nq int2
+    float2* part = reinterpret_cast<float2*>(partial);
+    int2* pcnt = reinterpret_cast<int2*>(part + (size_t)nsplit * nq);
+    const dim3 grid(nblk(nq, kThreads), nsplit);
+    if (counts)
+      k_stochastic_sorted_split<D, K, true><<<grid, kThreads, 0, s>>>(
+          view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, part,
+          pcnt);
+    else
+      k_stochastic_sorted_split<D, K, false><<<grid, kThreads, 0, s>>>(
+          view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, part,
+          pcnt);
+    SBH_HIP(hipGetLastError());
+    k_reduce_stoch_split<<<nblk(nq, 256), 256, 0, s>>>(part, pcnt, nsplit, nq, order, out,
+                                                        counts, split);
+    SBH_HIP(hipGetLastError());
+    return;
+  }
   if (per_query)
     k_stochastic_query<D, K, EXACT><<<nblk(nq, kThreads), kThreads, 0, s>>>(
         view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
@@ -684,19 +777,30 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
   });
 }
 
+uint32_t stochastic_split_count(const DeviceTree& t, uint32_t nq, bool per_query, bool exact) {
+  // the sorted-sample estimator in a launch of fewer than four wavefronts per
+  // SIMD: one task per root subdomain, if there are between 2 and 64
+  if (exact || per_query || t.is_double || t.dim * t.split_log > 6 || t.level_begin.size() < 3)
+    return 1;
+  const uint32_t nsub = t.level_begin[2] - t.level_begin[1];
+  if (nsub < 2 || nsub > 64 || nq >= 4u * 1024u * 64u) return 1;
+  return nsub;
+}
+
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,
                     const float* q, uint32_t nq, const uint32_t* order, float* out,
-                    int32_t* counts, int split, bool exact, hipStream_t s) {
+                    int32_t* counts, int split, bool exact, float* partial, hipStream_t s) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
+  const uint32_t ns = partial ? stochastic_split_count(t, nq, per_query, exact) : 1;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
     if (exact)
       launch_stoch<D, K, true>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
-                               split, s);
+                               split, partial, 1, s);
     else
       launch_stoch<D, K, false>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
-                                split, s);
+                                split, partial, ns, s);
   });
 }
 

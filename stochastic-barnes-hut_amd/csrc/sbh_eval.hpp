@@ -42,9 +42,13 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
             float* partial, hipStream_t s);
 // per_query: one sample path per query keyed by its index (otherwise the 64
 // queries of a wavefront share their paths, keyed by the wavefront's index).
+// partial (optional): stochastic_split_count(...) * nq * 2 floats (and with
+// counts as many int32 again); launches too small to fill the GPU then run
+// one task per root subdomain.
+uint32_t stochastic_split_count(const DeviceTree& t, uint32_t nq, bool per_query, bool exact);
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,
                     const float* q, uint32_t nq, const uint32_t* order, float* out,
-                    int32_t* counts, int split, bool exact, hipStream_t s);
+                    int32_t* counts, int split, bool exact, float* partial, hipStream_t s);
 
 // Smooth distance from its sum, in place: w <- -log(w) / alpha.
 void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s);

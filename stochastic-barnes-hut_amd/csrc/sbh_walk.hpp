@@ -595,11 +595,17 @@ __device__ __forceinline__ void wave_sort_keys(uint2* keys, uint32_t n) {
 
 // COUNT: also count the terms each query's estimate sums (the counting
 // mode); without it the counts are not formed at all.
-template <int D, int K, bool COUNT>
+// RANGE: only subdomains [si0, si1) of the root; the Barnes-Hut part of the
+// sum goes to *bh_part and the control-variate part is returned (the caller
+// adds the parts of all ranges in subdomain order).
+template <int D, int K, bool COUNT, bool RANGE = false>
 __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView& t, int spl,
                                                                  uint32_t key0, uint32_t key1,
                                                                  uint32_t g, bool live, float x,
-                                                                 float y, float z, SortedLds lds) {
+                                                                 float y, float z, SortedLds lds,
+                                                                 uint32_t si0 = 0,
+                                                                 uint32_t si1 = 0,
+                                                                 float* bh_part = nullptr) {
 #pragma clang fp contract(off)
   constexpr bool EXACT = false;
   constexpr bool kWN = (K == WINDING_NUMBER);
@@ -623,6 +629,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   if (rl.w & kLeafBit) {
     float v = 0.0f;
     node_accumulate<D, K, EXACT>(t, 0, x, y, z, v, cnt, cnt_nodes);
+    if constexpr (RANGE) {
+      *bh_part = v;
+      return {0.0f, cnt, cnt_nodes};
+    }
     return {v, cnt, cnt_nodes};
   }
   // the cached nodes' identities and term counts live in one register:
@@ -747,7 +757,9 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
 
   float bh = 0.0f, mlcv = 0.0f;
   const uint32_t nsub = rl.w, first = rl.y;
-  for (uint32_t si = 0; si < nsub; ++si) {
+  const uint32_t si_begin = RANGE ? si0 : 0u;
+  const uint32_t si_end = RANGE ? (si1 < nsub ? si1 : nsub) : nsub;
+  for (uint32_t si = si_begin; si < si_end; ++si) {
     const uint32_t S = uniform_u32(first + si);
     node_accumulate<D, K, EXACT>(t, S, x, y, z, bh, cnt, cnt_nodes);
     const uint4 Sl = nodeL[S];
@@ -885,6 +897,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       wave_lds_sync();
     }
     mlcv += sub_total / (float)spl;
+  }
+  if constexpr (RANGE) {
+    *bh_part = bh;
+    return {mlcv, cnt, cnt_nodes};
   }
   return {bh + mlcv, cnt, cnt_nodes};
 }
