@@ -294,7 +294,8 @@ def main():
                     help="profiling mode: split_log of the stochastic estimator's tree")
     ap.add_argument("--no-sort", action="store_true",
                     help="profiling mode: evaluate the stochastic estimator in input order")
-    ap.add_argument("--profile-kernel", choices=["stochastic", "bh", "brute", "mfma", "double"],
+    ap.add_argument("--profile-kernel",
+                    choices=["stochastic", "bh", "brute", "mfma", "double", "serve"],
                     default=None,
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
@@ -534,6 +535,12 @@ def profile(wl, args):
     import sbh_capi as C
 
     dev = torch.device("cuda:0")
+    if args.profile_kernel == "serve":
+        # the serving measurement alone (--serve-requests requests)
+        r = serving_bench(args)
+        log(f"serve: burst {r['burst']['queries_per_s'] / 1e6:.1f} M q/s, poisson p50 "
+            f"{r['poisson']['p50_ms']:.1f} ms, p99 {r['poisson']['p99_ms']:.1f} ms")
+        return
     if args.profile_kernel == "double":
         # the double build: Barnes-Hut at --theta and the estimator at --spl[0],
         # Hilbert-sorted, --steps times each
