@@ -405,3 +405,34 @@ def test_query_order_is_a_stable_hilbert_sort(sbh_gpu, dim):
     assert (keys[1:] >= keys[:-1]).all()
     same = keys[1:] == keys[:-1]
     assert (order[1:][same] > order[:-1][same]).all()
+
+
+@pytest.mark.parametrize("dim,k", [(3, 1), (3, 2), (2, 2)])
+def test_small_launches_split_over_root_children(sbh_gpu, dim, k):
+    """Launches too small to fill the GPU split Barnes-Hut and the sorted
+    estimator over the root's children.  Queries far outside the box accept
+    the root itself (its moment, one node term); near queries walk every
+    subtree.  Values agree with the oracle to fp32 rounding, counts exactly."""
+    pts, nrm, m, q, lo, hi = _cube(3000, 300, dim, 61 + k, "gravity")
+    d = q - 0.5
+    far = 0.5 + 60.0 * d / np.linalg.norm(d, axis=1, keepdims=True)  # ratio to the root >> beta
+    qs = np.concatenate([q, far])
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, k)
+    for beta in (1.5, 3.0):
+        got, _, cnt = sbh_gpu.eval_barnes_hut_gravity(g, qs, beta, vote_group=1,
+                                                      return_counts=True)
+        ref, rcnt = o.barnes_hut("gravity", qs, beta, vote_group=1, counts=True)
+        np.testing.assert_array_equal(cnt, rcnt)
+        assert (rcnt[len(q):] == 1).all()  # the far queries take the root's moment
+        assert rel_err(got, ref, "gravity").max() <= TOL
+        # without counts: the same split, the same values
+        np.testing.assert_array_equal(
+            sbh_gpu.eval_barnes_hut_gravity(g, qs, beta, vote_group=1)[0], got)
+    st, _, scnt = sbh_gpu.multi_level_prefix_control_variate_gravity(g, qs, 3, seed=5,
+                                                                     return_counts=True)
+    np.testing.assert_array_equal(
+        sbh_gpu.multi_level_prefix_control_variate_gravity(g, qs, 3, seed=5)[0], st)
+    ref, rcnt = o.stochastic("gravity", qs, 3, seed=5, group=64, counts=True)
+    same = scnt == rcnt
+    assert np.count_nonzero(~same) <= max(2, len(same) // 100)
+    assert rel_err(st[same], ref[same], "gravity").max() <= TOL
