@@ -24,7 +24,7 @@ def gpu_available() -> bool:
 
 @pytest.fixture(scope="session")
 def sbh_gpu():
-    """The sbh_gpu plugin; on a GPU box it must be the native HIP build."""
+    """The sbh_gpu plugin; with a GPU visible it must be the native HIP build."""
     import sbh_gpu as m
     assert m.device_count() >= 1, "no GPU visible to libsbh_hip.so"
     return m
