@@ -5,10 +5,10 @@
 // double build's for the same tree, queries and seed.
 //
 // These are the precision forms, not the throughput path: one lane per
-// query, node records read straight from memory.  The Barnes-Hut walk keeps
-// the union-walk structure of the fp32 kernel (one rope walk per wavefront,
-// node index wave-uniform); the estimator walks each lane's paths on their
-// own.
+// query.  The Barnes-Hut walk keeps the union-walk structure of the fp32
+// kernel (one rope walk per wavefront over 64-byte depth-first records, node
+// index wave-uniform); the estimator walks each lane's paths on their own,
+// with a per-lane memo of its level-1 nodes' diffs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
