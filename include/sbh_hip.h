@@ -99,6 +99,16 @@ typedef struct sbh_tree sbh_tree;
                                       reference GPU sampler shares them per block
                                       (include/tree_sampler_gpu.h:41-71).  Per-query
                                       results do not depend on the query order */
+#define SBH_EVAL_NO_SPLIT 256      /* Barnes-Hut (vote_group 1) / stochastic (shared
+                                      paths), fast terms: run the whole-walk kernels
+                                      at any launch size.  By default a launch of
+                                      fewer than 2^18 queries splits each walk over
+                                      the root's children (one task per subtree or
+                                      subdomain, partial sums added in child order),
+                                      which visits the same terms but rounds the sum
+                                      differently; with this flag the values are bit-
+                                      identical to those of a large launch and of the
+                                      serving engine */
 
 typedef struct sbh_eval_params {
   float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */
@@ -259,12 +269,20 @@ int sbh_tree_load(const char* path, sbh_tree** out);
  * queries, up to max_inflight launches overlapping on separate streams.
  * Barnes-Hut runs per query (vote_group is ignored), the stochastic estimator
  * keys paths on (seed, 64-query block of the request's sorted order): a
- * request's values are identical alone and in any mix, and equal those of
- * sbh_eval with SBH_EVAL_SORT_QUERIES.  Terms use the fast kernels.
+ * request's values are identical alone and in any mix, and bit-identical to
+ * those of sbh_eval with SBH_EVAL_SORT_QUERIES | SBH_EVAL_NO_SPLIT (without
+ * SBH_EVAL_NO_SPLIT, a one-shot launch of fewer than 2^18 queries sums the
+ * same terms per root child, equal up to fp32 rounding).  Terms use the fast
+ * kernels; params.flags may hold only SBH_EVAL_SORT_QUERIES and
+ * SBH_EVAL_NO_SPLIT (what the engine always does), anything else is refused
+ * with SBH_ERR_INVALID.
  *
  * Trees must outlive the server.  `queries` (column-major dim x q) and
  * `results` (q floats) are host memory that must stay valid until the request
- * is done, failed or cancelled.
+ * is done, failed or cancelled.  sbh_server_destroy cancels queued requests,
+ * waits for running ones, and wakes threads blocked in sbh_server_wait (which
+ * then return the request's state); no call on the server may start once
+ * sbh_server_destroy has been entered.
  */
 typedef struct sbh_server sbh_server;
 

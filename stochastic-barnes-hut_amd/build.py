@@ -40,9 +40,23 @@ def _run(cmd):
 
 
 def build_library() -> str:
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Each source compiles to its own object (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, *compile_flags, "-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, HIP_SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

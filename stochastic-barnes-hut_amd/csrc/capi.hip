@@ -481,7 +481,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
                              SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
                              SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16 |
-                             SBH_EVAL_SMOOTH_DISTANCE)) == 0,
+                             SBH_EVAL_SMOOTH_DISTANCE | SBH_EVAL_NO_SPLIT)) == 0,
                 "unknown eval flags");
     SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
                 "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
@@ -553,7 +553,10 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
         ord = tree->order.get();
       }
       float* bh_partial = nullptr;
-      if (method == SBH_BARNES_HUT) {
+      const bool no_split = (p.flags & SBH_EVAL_NO_SPLIT) != 0;
+      if (no_split) {
+        // whole-walk kernels at any size: no partial sums
+      } else if (method == SBH_BARNES_HUT) {
         const uint32_t ns = sbh::bh_split_count(t, nq, p.vote_group, exact, dcnt != nullptr);
         if (ns > 1) {
           // ns * nq floats (rounded up to an even count), then ns * nq int2 counts

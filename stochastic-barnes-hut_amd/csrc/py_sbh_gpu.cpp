@@ -296,25 +296,28 @@ void bind_kernel(py::module_& m, const char* suffix, int kernel) {
         "tree"_a, "queries"_a, "mfma"_a = false, "smooth_distance"_a = false);
   m.def(("eval_barnes_hut_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, float beta, int vote_group, bool sort_queries,
-                 bool return_counts, bool exact_terms, bool smooth_distance) {
+                 bool return_counts, bool exact_terms, bool smooth_distance, bool no_split) {
           return run_eval(t, SBH_BARNES_HUT, kernel, q, beta, 1, 0, vote_group, sort_queries,
                           return_counts, exact_terms,
-                          smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0);
+                          (smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0) |
+                              (no_split ? SBH_EVAL_NO_SPLIT : 0));
         },
         "tree"_a, "queries"_a, "beta"_a, "vote_group"_a = 32, "sort_queries"_a = false,
-        "return_counts"_a = false, "exact_terms"_a = false, "smooth_distance"_a = false);
+        "return_counts"_a = false, "exact_terms"_a = false, "smooth_distance"_a = false,
+        "no_split"_a = false);
   m.def(("multi_level_prefix_control_variate_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array q, int spl, uint64_t seed, bool sort_queries,
                  bool return_counts, bool exact_terms, bool per_query_paths,
-                 bool smooth_distance) {
+                 bool smooth_distance, bool no_split) {
           return run_eval(t, SBH_STOCHASTIC, kernel, q, 0.f, spl, seed, 1, sort_queries,
                           return_counts, exact_terms,
                           (per_query_paths ? SBH_EVAL_PER_QUERY_PATHS : 0) |
-                              (smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0));
+                              (smooth_distance ? SBH_EVAL_SMOOTH_DISTANCE : 0) |
+                              (no_split ? SBH_EVAL_NO_SPLIT : 0));
         },
         "tree"_a, "queries"_a, "samples_per_subdomain"_a, "seed"_a = 0, "sort_queries"_a = false,
         "return_counts"_a = false, "exact_terms"_a = false, "per_query_paths"_a = false,
-        "smooth_distance"_a = false);
+        "smooth_distance"_a = false, "no_split"_a = false);
   m.def(("eval_brute_force_f64_" + s).c_str(),
         [kernel](GPUTreeBase& t, F32Array queries) {
           std::vector<float> q = t.soa(queries);
@@ -538,8 +541,9 @@ PYBIND11_MODULE(sbh_gpu, m) {
   py::class_<PyServer>(m, "Server")
       .def(py::init<int64_t, int>(), "budget_bytes"_a, "max_inflight"_a = 4)
       .def("add_tree", &PyServer::add_tree, "tree"_a)
+      // a Request keeps its Server alive: wait / poll never run on a destroyed engine
       .def("submit", &PyServer::submit, "tree_id"_a, "method"_a, "kernel"_a, "queries"_a,
-           "beta"_a = 2.0f, "samples_per_subdomain"_a = 1, "seed"_a = 0)
+           "beta"_a = 2.0f, "samples_per_subdomain"_a = 1, "seed"_a = 0, py::keep_alive<0, 1>())
       .def("cancel", &PyServer::cancel, "request"_a)
       .def("stats", &PyServer::stats);
   m.attr("REQ_QUEUED") = SBH_REQ_QUEUED;

@@ -46,6 +46,7 @@ class Server {
   std::condition_variable cv_;
   std::thread worker_;
   bool stop_ = false;
+  int waiters_ = 0;  // threads inside wait(); the destructor lets them leave first
   std::string error_;
   std::unique_ptr<Arena> arena_;
   std::vector<DeviceTree*> trees_;

@@ -252,8 +252,14 @@ Server::Server(size_t budget_bytes, int max_inflight) {
 
 Server::~Server() {
   {
-    std::lock_guard<std::mutex> g(mu_);
+    std::unique_lock<std::mutex> g(mu_);
     stop_ = true;
+    // requests that never started are cancelled; threads blocked in wait()
+    // wake up, see the state, and leave before the members go away
+    for (Request* r : queue_) r->state = SBH_REQ_CANCELLED;
+    queue_.clear();
+    cv_.notify_all();
+    cv_.wait(g, [&] { return waiters_ == 0; });
   }
   cv_.notify_all();
   if (worker_.joinable()) worker_.join();
@@ -282,6 +288,12 @@ int64_t Server::submit(int32_t tree_id, int32_t method, int32_t kernel, const sb
               "serving evaluates the gravity, winding-number and smooth-distance kernels");
   SBH_REQUIRE(q >= 1 && q < (int64_t)0x7FFFFFFF, "query count out of range");
   SBH_REQUIRE(queries && results, "NULL queries or results");
+  SBH_REQUIRE(!stop_, "the server is shutting down");
+  // the engine always evaluates in Hilbert order with the whole-walk kernels
+  // and fast terms; any other mode is refused rather than ignored
+  SBH_REQUIRE((p.flags & ~(SBH_EVAL_SORT_QUERIES | SBH_EVAL_NO_SPLIT)) == 0,
+              "serving supports no eval flags besides SBH_EVAL_SORT_QUERIES and "
+              "SBH_EVAL_NO_SPLIT (what it always does)");
   if (method == SBH_BARNES_HUT) SBH_REQUIRE(p.beta == p.beta, "beta is NaN");
   if (method == SBH_STOCHASTIC)
     SBH_REQUIRE(p.samples_per_subdomain >= 1, "samples_per_subdomain must be >= 1");
@@ -324,11 +336,13 @@ int Server::wait(int64_t id, double timeout_s, double* latency_ms) {
   auto it = requests_.find(id);
   SBH_REQUIRE(it != requests_.end(), "unknown request id");
   Request* r = it->second.get();
-  auto pred = [&] { return r->state >= SBH_REQ_DONE || !error_.empty(); };
+  auto pred = [&] { return r->state >= SBH_REQ_DONE || !error_.empty() || stop_; };
+  ++waiters_;
   if (timeout_s < 0)
     cv_.wait(g, pred);
   else
     cv_.wait_for(g, std::chrono::duration<double>(timeout_s), pred);
+  if (--waiters_ == 0 && stop_) cv_.notify_all();
   if (!error_.empty() && r->state < SBH_REQ_DONE) throw Error(kErrHip, error_);
   if (latency_ms) *latency_ms = r->state == SBH_REQ_DONE ? (r->t_done - r->t_submit) * 1e3 : 0.0;
   return r->state;

@@ -26,7 +26,7 @@ BRUTE_FORCE, BARNES_HUT, STOCHASTIC = 0, 1, 2
 TREE_MOMENTS_F64 = 1
 EVAL_DEVICE_POINTERS, EVAL_SORT_QUERIES, EVAL_SPLIT_COUNTS = 1, 2, 4
 EVAL_EXACT_TERMS, EVAL_MFMA, EVAL_PER_QUERY_PATHS, EVAL_MFMA_BF16 = 8, 16, 32, 64
-EVAL_SMOOTH_DISTANCE = 128
+EVAL_SMOOTH_DISTANCE, EVAL_NO_SPLIT = 128, 256
 
 
 class EvalParams(ctypes.Structure):
