@@ -215,6 +215,12 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
 
     head, head_ok = at_target(res["stochastic"])
     bh, bh_ok = at_target(res["barnes_hut"])
+    if kern == "smoothdist":
+        # the distance form -log(w)/alpha the reference's driver reports
+        ok = [r for r in res["stochastic"] if r["distance_rel_err"] <= TARGET_ERR]
+        res["stochastic_at_target_distance"] = (
+            dict(min(ok, key=lambda r: r["kernel_ms"]), target_met=True) if ok else
+            dict(min(res["stochastic"], key=lambda r: r["distance_rel_err"]), target_met=False))
     res["stochastic_at_target"] = dict(head, target_met=head_ok)
     res["barnes_hut_at_target"] = dict(bh, target_met=bh_ok)
     res["stochastic_over_brute_force"] = head["queries_per_s"] / res["brute_force"]["queries_per_s"]
@@ -240,6 +246,14 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
                              samples_per_subdomain=hkw["samples_per_subdomain"], seed=1)
               for _ in range(max(1, headline_steps))]
         res["kernel_only_ms"] = float(np.mean(kk))
+        res["kernel_only_ms_all"] = [float(v) for v in kk]
+        # the work that launch performs (same seed, same 64-query groups):
+        # kernel terms evaluated after the node cache's reuse, path levels stepped
+        th.eval_device(C.STOCHASTIC, kern, qs.data_ptr(), nq, out.data_ptr(),
+                       counts_ptr=cnt.data_ptr(), samples_per_subdomain=hkw["samples_per_subdomain"],
+                       seed=1, flags=C.EVAL_WORK_COUNTS)
+        c = cnt.cpu().numpy().astype(np.int64)
+        res["work"] = {"terms": int(c[:nq].sum()), "path_levels": int(c[nq:].sum())}
         del qs, order
     elif want_counts and headline_steps:
         res["kernel_only_ms"] = float(np.mean(res["headline"]["kernel_ms"]))
@@ -300,6 +314,8 @@ def main():
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
     ap.add_argument("--theta", type=float, default=1.0)
+    ap.add_argument("--detail", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
+                    help="where the full record (sweeps, variants, serving, CPU legs) is written")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="with --profile-kernel: also run a plain read+write kernel over this "
                          "many MiB (a counter calibration kernel of known traffic)")
@@ -326,41 +342,7 @@ def main():
     wall, kms = head["headline"]["wall_ms"], head["headline"]["kernel_ms"]
     ms_per_step = float(np.mean(wall))
     value = head["n_queries"] / (ms_per_step / 1e3)
-    # the dominant kernel's own launch time (the query sort excluded)
-    kernel_ms = head["kernel_only_ms"]
-    nt, pt = head["terms"]["node_terms"], head["terms"]["point_terms"]
-    terms = nt + pt
-    alg_bytes = nt * NODE_RECORD_BYTES + pt * POINT_RECORD_BYTES
-    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
-    tflops = terms * head["terms"]["flops_per_term"] / (kernel_ms / 1e3) / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        try:
-            # measured for this exact configuration by the two counter runs
-            # (FETCH_SIZE, WRITE_SIZE) of the profiling mode; absent otherwise
-            key = "k_stochastic_plummer_1m_k%d_spl%d_%s" % (
-                hs["tree_k"], hs["samples_per_subdomain"],
-                "sorted" if hs["sorted"] else "unsorted")
-            traffic = json.load(open(tpath)).get(key)
-        except Exception:
-            traffic = None
-    roofline = {
-        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-        "kernel": "k_stochastic_sorted<3, GRAVITY>", "kernel_ms": kernel_ms,
-        "eval_ms_with_query_sort": float(np.mean(kms)),
-        "algorithmic_bytes_per_launch": alg_bytes,
-        "bytes_per_node_term": NODE_RECORD_BYTES, "bytes_per_point_term": POINT_RECORD_BYTES,
-        "node_terms": nt, "point_terms": pt,
-        "note": "algorithmic bytes count every record read per query; a wavefront shares "
-                "its sample paths, so records are served from LDS and L2 and frac exceeds 1. "
-                "traffic is the measured HBM bytes per launch",
-        "measured_hbm_frac": (traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS
-                              if traffic else None),
-        "compute": {"unit": "TFLOP/s", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
-                    "frac": tflops / FP32_PEAK_TFLOPS, "flops_per_term": head["terms"]["flops_per_term"],
-                    "kind": "FP32 VALU"}}
+    roofline = estimator_roofline(head)
 
     variants = None
     if not args.no_variants:
@@ -368,7 +350,8 @@ def main():
         specs = [("plummer3d", 1 << 20, 1 << 20, 10.0, (0.5, 1.0), (1, 4, 8, 16)),
                  ("gmm2d", 1 << 20, 1 << 20, None, (0.5, 1.0), (1, 4, 16, 64)),
                  ("icosphere_wn", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16, 64)),
-                 ("icosphere_smoothdist", 1310720, 1 << 21, None, (0.5, 1.0), (1, 4, 16))]
+                 ("icosphere_smoothdist", 1310720, 1 << 21, None, (0.5, 1.0),
+                  (1, 4, 16, 64, 256, 1024, 1536))]
         vargs = argparse.Namespace(**vars(args))
         vargs.sweep_steps, vargs.brute_steps = 2, 1
         for name, n, q, rmax, thetas, spls in specs:
@@ -376,9 +359,9 @@ def main():
             v = measure(make_workload(name, n, q, rmax=rmax), vargs, thetas, spls)
             variants.append({k: v.get(k) for k in (
                 "workload", "n_sources", "n_queries", "subdomains", "build_ms", "brute_force",
-                "brute_force_mfma", "stochastic_at_target", "barnes_hut_at_target",
-                "stochastic_over_brute_force", "barnes_hut_over_brute_force", "barnes_hut",
-                "stochastic")})
+                "brute_force_mfma", "stochastic_at_target", "stochastic_at_target_distance",
+                "barnes_hut_at_target", "stochastic_over_brute_force",
+                "barnes_hut_over_brute_force", "barnes_hut", "stochastic")})
 
     precision = None if args.no_variants else precision_variant(args)
     double_build = None if args.no_variants else double_build_variant(args)
@@ -386,48 +369,116 @@ def main():
     cpu = cpu_c1 = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, hs["samples_per_subdomain"], hs["tree_k"], args.cpu_seconds, value)
-        cpu_c1 = cpu_baseline_c1()
+        cpu_c1 = cpu_baseline_c1(cpu["cores"])
 
+    bf = head["brute_force"]
+    bh = head["barnes_hut_at_target"]
     line = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "data": "synthetic (seeded numpy)",
         "config": {
-            "workload": head["workload"] + ", stochastic Barnes-Hut at <=1e-2 mean rel err",
+            "workload": "3D gravity, untruncated Plummer sphere (scale radius 1, equal masses "
+                        "summing to 1), 2^20 sources and 2^20 queries, stochastic Barnes-Hut at "
+                        "<=1e-2 mean rel err vs float64 brute force",
             "n_sources": head["n_sources"], "n_queries": head["n_queries"],
             "plummer_rmax": args.rmax, "method": "stochastic",
-            "samples_per_subdomain": hs["samples_per_subdomain"],
-            "subdomains": hs["samples_per_query"] // hs["samples_per_subdomain"],
-            "samples_per_query": hs["samples_per_query"],
-            "sorted_queries": hs["sorted"], "mean_rel_err": head["headline"]["mean_rel_err"],
-            "target_err_met": hs["target_met"], "err_queries": head["err_queries"],
             "tree": f"octree k={hs['tree_k']}, leaf size 1",
+            "samples_per_subdomain": hs["samples_per_subdomain"],
+            "samples_per_query": hs["samples_per_query"], "sorted_queries": hs["sorted"],
+            "mean_rel_err": head["headline"]["mean_rel_err"], "target_err_met": hs["target_met"],
+            "err_queries": head["err_queries"],
         },
         "latency_ms": ({"p50": serving["poisson"]["p50_ms"], "p99": serving["poisson"]["p99_ms"],
-                        "kind": "serving request latency, submit to results in host memory, "
+                        "kind": "serving request latency (submit to results in host memory), "
                                 "Poisson arrivals at {:.0%} of burst capacity".format(
-                                    args.serve_load)}
-                       if serving else None),
+                                    args.serve_load)} if serving else None),
         "step_latency_ms": {"p50": percentile(wall, 50), "p99": percentile(wall, 99),
-                            "kind": "per 2^20-query evaluation, wall clock"},
+                            "kind": "one 2^20-query evaluation incl. its query sort"},
         "roofline": roofline,
-        "cpu_baseline": cpu,
-        "cpu_baseline_c1": cpu_c1,
-        "speedup_vs_brute_force": value / head["brute_force"]["queries_per_s"],
-        "brute_force": head["brute_force"],
-        "brute_force_mfma": head.get("brute_force_mfma"),
-        "stochastic_per_query_paths": head.get("stochastic_per_query_paths"),
-        "barnes_hut_at_target": head["barnes_hut_at_target"],
-        "barnes_hut_over_brute_force": head["barnes_hut_over_brute_force"],
-        "methods": {"barnes_hut": head["barnes_hut"], "stochastic": head["stochastic"]},
-        "build_ms": head["build_ms"], "trees": head["trees"],
-        "variants": variants,
-        "precision_variant": precision,
-        "double_build": double_build,
-        "serving": serving,
+        "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                              "host", "gpu_over_cpu")} if cpu else None),
+        "speedup_vs_brute_force": value / bf["queries_per_s"],
+        "brute_force": {"kernel_ms": bf["kernel_ms"], "queries_per_s": bf["queries_per_s"],
+                        "mean_rel_err": bf["mean_rel_err"]},
+        "barnes_hut": {"theta": bh["theta"], "kernel_ms": bh["kernel_ms"],
+                       "queries_per_s": bh["queries_per_s"], "mean_rel_err": bh["mean_rel_err"],
+                       "target_err_met": bh["target_met"],
+                       "over_brute_force": head["barnes_hut_over_brute_force"]},
+        "serving": ({"burst_queries_per_s": serving["burst"]["queries_per_s"],
+                     "poisson_queries_per_s": serving["poisson"]["queries_per_s"],
+                     "p50_ms": serving["poisson"]["p50_ms"], "p99_ms": serving["poisson"]["p99_ms"],
+                     "requests": serving["requests"]} if serving else None),
+        "detail": os.path.relpath(args.detail, ROOT),
     }
-    print(json.dumps(line), flush=True)
+    detail = dict(line, headline=head, variants=variants, precision_variant=precision,
+                  double_build=double_build, serving_detail=serving, cpu_baseline_full=cpu,
+                  cpu_baseline_c1=cpu_c1)
+    os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+    with open(args.detail, "w") as f:
+        json.dump(detail, f, indent=1, default=float)
+    log(f"detail written to {args.detail}")
+    log("summary: " + json.dumps(summary_of(detail), default=float))
+    text = json.dumps(line, default=float)
+    assert len(text) < 4096, len(text)
+    print(text, flush=True)
+
+
+def summary_of(d):
+    """A few figures of the detail record, for the log."""
+    out = {}
+    for v in d.get("variants") or []:
+        sa = v["stochastic_at_target"]
+        out[v["workload"][:40]] = {
+            "brute_ms": v["brute_force"]["kernel_ms"],
+            "bh_x": v["barnes_hut_over_brute_force"], "stoch_x": v["stochastic_over_brute_force"],
+            "stoch_err": sa["mean_rel_err"], "stoch_ok": sa["target_met"]}
+    return out
+
+
+# algorithmic flops per path level of the estimator: the opening ratio (3 sub,
+# 3 products summed = 5, sqrt, times the diameter's reciprocal = 10), the
+# roulette probability and draw (reciprocal, product, min, compare = 4), the
+# path probability (1 - p, product = 2), the level's weight and its sum
+# (product, reciprocal, 2 products, add = 5) and the running beta (max = 1)
+FLOPS_PER_PATH_LEVEL = 22
+
+
+def estimator_roofline(head):
+    """Roofline of the dominant kernel (the sorted-sample estimator launch,
+    timed alone with hipEvents on its stream): algorithmic flops = the kernel
+    terms it evaluates x flops per term + the path levels it steps x flops per
+    level, counted by its work-count mode; peak = MI355X FP32 (157.3 TFLOP/s).
+    The counters put it on VALU issue; HBM traffic is a few percent of peak."""
+    kms = head["kernel_only_ms"]
+    F = head["terms"]["flops_per_term"]
+    w = head["work"]
+    flops = w["terms"] * F + w["path_levels"] * FLOPS_PER_PATH_LEVEL
+    achieved = flops / (kms / 1e3) / 1e12
+    ref_bytes = (head["terms"]["node_terms"] * NODE_RECORD_BYTES
+                 + head["terms"]["point_terms"] * POINT_RECORD_BYTES)
+    traffic = valu = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    hs = head["stochastic_at_target"]
+    key = "k_stochastic_plummer_1m_k%d_spl%d_%s" % (
+        hs["tree_k"], hs["samples_per_subdomain"], "sorted" if hs["sorted"] else "unsorted")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath))
+        traffic = rec.get(key)
+        valu = rec.get("valu_issue", {}).get(key)
+    return {
+        "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+        "kernel": "k_stochastic_sorted<3, GRAVITY>", "kernel_ms": kms,
+        "units": {"terms": w["terms"], "flops_per_term": F, "path_levels": w["path_levels"],
+                  "flops_per_level": FLOPS_PER_PATH_LEVEL},
+        "frac_non_packed_fp32": achieved / (FP32_PEAK_TFLOPS / 2),
+        "valu_issue_frac": valu,
+        "hbm_frac": (traffic / (kms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None),
+        "reference_terms": head["terms"]["node_terms"] + head["terms"]["point_terms"],
+        "reference_term_bytes_over_hbm_peak": ref_bytes / (kms / 1e3) / 1e9 / HBM_PEAK_GBS,
+    }
 
 
 def precision_variant(args):
@@ -668,8 +719,15 @@ def serving_bench(args):
     log(f"serve poisson: rate {rate:.1f} req/s, p50 {np.percentile(lat_p, 50):.1f} ms, "
         f"p99 {np.percentile(lat_p, 99):.1f} ms")
     st = srv.stats()
+    # shut the engine down before the next leg: its scheduler thread and
+    # streams end here, not at interpreter exit
+    import gc
+
+    tree_info = [{k: t[k] for k in ("name", "dim", "kernel", "n", "subdomains")} for t in trees]
+    del srv, trees, reqs
+    gc.collect()
     return {
-        "trees": [{k: t[k] for k in ("name", "dim", "kernel", "n", "subdomains")} for t in trees],
+        "trees": tree_info,
         "requests": nreq, "queries": int(total_q),
         "request_sizes": "log-uniform 1e3..1e6 queries",
         "methods": "barnes_hut theta=0.5 / stochastic 128 samples per query, 50/50",
@@ -685,49 +743,103 @@ def serving_bench(args):
     }
 
 
+def host_cpus():
+    """The host's CPUs as this process sees them: os.cpu_count() (what the
+    reference's std::thread::hardware_concurrency() returns, src/sbh.cpp:32-36),
+    the CPUs this process may run on (affinity) and the cgroup CPU quota."""
+    info = {"os_cpu_count": os.cpu_count() or 1}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = info["os_cpu_count"]
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["model"] = model
+    allowed = info["affinity"]
+    if quota:
+        allowed = min(allowed, max(1, int(round(quota))))
+    info["allowed"] = allowed
+    return info
+
+
+def _best_of(fn, runs):
+    best = float("inf")
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
 def cpu_baseline(wl, spl, k, budget_s, gpu_qps):
     """CPU baseline leg: the oracle (a C++ restatement of the reference's CPU
     estimator with the same threading model and its per-query sample paths)
-    timed on this machine's host cores on a bounded sample of the headline
-    workload."""
+    on this machine's host cores, on a bounded sample of the headline
+    workload.  Timed with the reference's thread count (all CPUs the OS
+    reports) and with the CPUs this process is allowed (affinity and cgroup
+    quota) where that differs; best of 3 after a warm-up each; the faster
+    setting is the baseline."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpus()
+    settings = sorted({host["os_cpu_count"], host["allowed"]})
     t0 = time.perf_counter()
     ot = oracle.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"], split_log=k)
     build_s = time.perf_counter() - t0
     q32 = wl["queries"].astype(np.float32)
-    probe = q32[:2048]
+    probe = q32[:1024]
     t0 = time.perf_counter()
-    ot.stochastic("gravity", probe, spl, seed=1, group=1, nthreads=threads)
+    ot.stochastic("gravity", probe, spl, seed=1, group=1, nthreads=max(settings))
     per_q = (time.perf_counter() - t0) / len(probe)
-    nsample = int(min(len(q32), max(2048, budget_s / max(per_q, 1e-9))))
-    t0 = time.perf_counter()
-    ot.stochastic("gravity", q32[:nsample], spl, seed=1, group=1, nthreads=threads)
-    el = time.perf_counter() - t0
-    qps = nsample / el
-    log(f"cpu baseline: {qps:.0f} q/s on {threads} threads ({nsample} queries, {el:.1f} s); "
-        f"oracle tree build {build_s:.1f} s")
-    return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"stochastic estimator, samples_per_subdomain={spl}, first {nsample} of the "
-                      f"{len(q32)} Plummer queries against the {len(wl['pts'])}-source octree "
-                      f"(k={k}), one sample path per query; tree build excluded "
-                      f"({build_s:.1f} s single-threaded)",
+    # about budget_s of wall time over all timed runs (3 per setting)
+    nsample = int(min(len(q32), max(1024, budget_s / (3 * len(settings)) / max(per_q, 1e-9))))
+    sample = q32[:nsample]
+    timings = {}
+    for th in settings:
+        ot.stochastic("gravity", sample[:256], spl, seed=1, group=1, nthreads=th)  # warm-up
+        timings[th] = _best_of(
+            lambda: ot.stochastic("gravity", sample, spl, seed=1, group=1, nthreads=th), 3)
+        log(f"cpu baseline: {nsample / timings[th]:.0f} q/s on {th} threads "
+            f"({nsample} queries, best of 3: {timings[th]:.2f} s)")
+    best_th = min(timings, key=timings.get)
+    qps = nsample / timings[best_th]
+    log(f"cpu baseline host: {host}; oracle tree build {build_s:.1f} s")
+    return {"value": qps, "unit": "queries/s", "cores": best_th, "kind": "port",
+            "sample": f"stochastic estimator, {spl} samples per subdomain, octree k={k}, one "
+                      f"sample path per query (the reference CPU sampler's draws); first "
+                      f"{nsample} of the {len(q32)} Plummer queries against all "
+                      f"{len(wl['pts'])} sources; best of 3; tree build excluded "
+                      f"({build_s:.1f} s, one thread)",
+            "host": host,
+            "queries_per_s_by_threads": {str(t): nsample / v for t, v in timings.items()},
             "gpu_over_cpu": gpu_qps / qps}
 
 
-def cpu_baseline_c1():
+def cpu_baseline_c1(threads):
     """CPU baseline leg on the reference's CPU-runnable case (3D gravity, 16384
     uniform sources with masses U[0.5,1.5], 16384 queries in [-0.1,1.1]^3):
     brute force, Barnes-Hut (beta 2, leaf sizes 8 and 1, per query) and the
     stochastic estimator (256 samples per query: octree k=2, 4 per subdomain),
-    the oracle on the host cores (best of 3) beside the GPU, each against the
+    the oracle on the host cores (best of 5 after a warm-up) beside the GPU, each against the
     float64 all-pairs sum."""
     import oracle
     import sbh_gpu
     import workloads as W
 
-    threads = min(16, os.cpu_count() or 1)
     pts, _, m, q = W.uniform_cube(16384, 16384, seed=0)
     lo, hi = W.bounds(pts)
     nrm = np.zeros_like(pts)
@@ -740,12 +852,8 @@ def cpu_baseline_c1():
         return float(np.mean(np.abs(np.asarray(v, np.float64) - ref) / np.abs(ref)))
 
     def cpu(fn):
-        best, v = float("inf"), None
-        for _ in range(3):
-            t0 = time.perf_counter()
-            v = fn()
-            best = min(best, time.perf_counter() - t0)
-        return v, best
+        v = fn()  # warm-up
+        return v, _best_of(fn, 5)
 
     def gpu(fn, *a, **kw):
         fn(*a, **kw)

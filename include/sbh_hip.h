@@ -109,6 +109,14 @@ typedef struct sbh_tree sbh_tree;
                                       differently; with this flag the values are bit-
                                       identical to those of a large launch and of the
                                       serving engine */
+#define SBH_EVAL_WORK_COUNTS 512   /* stochastic, fast shared-path estimator on trees of
+                                      at most 64 children per node, with counts: counts
+                                      holds 2q values describing the work the launch
+                                      performed for each query after its reuse of
+                                      node differences across sorted samples: kernel
+                                      terms evaluated in [0, q), sample-path levels
+                                      stepped in [q, 2q).  Values are unchanged; the
+                                      launch is never split */
 
 typedef struct sbh_eval_params {
   float beta;                    /* Barnes-Hut opening ratio (1/theta); +inf = exact */

@@ -481,8 +481,14 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
                              SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
                              SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16 |
-                             SBH_EVAL_SMOOTH_DISTANCE | SBH_EVAL_NO_SPLIT)) == 0,
+                             SBH_EVAL_SMOOTH_DISTANCE | SBH_EVAL_NO_SPLIT |
+                             SBH_EVAL_WORK_COUNTS)) == 0,
                 "unknown eval flags");
+    const bool work = (p.flags & SBH_EVAL_WORK_COUNTS) != 0;
+    if (work)
+      SBH_REQUIRE(method == SBH_STOCHASTIC && counts &&
+                      !(p.flags & (SBH_EVAL_EXACT_TERMS | SBH_EVAL_PER_QUERY_PATHS)),
+                  "SBH_EVAL_WORK_COUNTS applies to the fast shared-path estimator, with counts");
     SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
                 "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
     if (method == SBH_BARNES_HUT) {
@@ -495,7 +501,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
     sbh::DeviceTree& t = tree->t;
     hipStream_t s = t.stream;
     const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
-    const int split = (p.flags & SBH_EVAL_SPLIT_COUNTS) ? 1 : 0;
+    const int split = (p.flags & (SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_WORK_COUNTS)) ? 1 : 0;
     const bool exact = (p.flags & SBH_EVAL_EXACT_TERMS) != 0;
     const uint32_t nq = (uint32_t)q;
     const size_t ncnt = split ? 2ull * nq : nq;
@@ -553,7 +559,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
         ord = tree->order.get();
       }
       float* bh_partial = nullptr;
-      const bool no_split = (p.flags & SBH_EVAL_NO_SPLIT) != 0;
+      const bool no_split = (p.flags & (SBH_EVAL_NO_SPLIT | SBH_EVAL_WORK_COUNTS)) != 0;
       if (no_split) {
         // whole-walk kernels at any size: no partial sums
       } else if (method == SBH_BARNES_HUT) {
@@ -580,7 +586,7 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
       else
         sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed,
                             (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout, dcnt,
-                            split, exact, bh_partial, s);
+                            split, exact, bh_partial, s, work);
       SBH_HIP(hipEventRecord(tree->ev1, s));
     }
     if (p.flags & SBH_EVAL_SMOOTH_DISTANCE) sbh::run_smooth_distance(dout, nq, t.alpha, s);

@@ -279,8 +279,9 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
 }
 
 // Fast-mode estimator with sorted samples and the per-level node cache, for
-// trees whose nodes have at most 64 children.
-template <int D, int K, bool COUNT>
+// trees whose nodes have at most 64 children.  WORK: the counts are the work
+// counts of stochastic_estimate_sorted (kernel terms, path levels).
+template <int D, int K, bool COUNT, bool WORK = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_stochastic_sorted(
     TreeView t, int spl, uint32_t key0, uint32_t key1, const float* __restrict__ qx,
     const float* __restrict__ qy, const float* __restrict__ qz, uint32_t nq,
@@ -304,11 +305,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
   const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
                       sKeys + wave * chunk_for<K>(), sDiff + wave * kCacheLevels * 64,
                       sNext + wave * (kCacheLevels + 1) * 3};
-  const LaneResult r =
-      stochastic_estimate_sorted<D, K, COUNT>(t, spl, key0, key1, g, live, x, y, z, lds);
+  const LaneResult r = stochastic_estimate_sorted<D, K, COUNT, false, WORK>(t, spl, key0, key1,
+                                                                          g, live, x, y, z, lds);
   if (live) {
     out[qi] = r.value;
-    if (COUNT && counts) {
+    if (WORK) {
+      counts[qi] = r.points;
+      counts[nq + qi] = r.nodes;
+    } else if (COUNT && counts) {
       if (split_counts) {
         counts[qi] = r.nodes;
         counts[nq + qi] = r.points;
@@ -633,9 +637,23 @@ template <int D, int K, bool EXACT>
 void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, bool per_query, const float* qx,
                   const float* qy, const float* qz, uint32_t nq, const uint32_t* order,
                   float* out, int32_t* counts, int split, float* partial, uint32_t nsplit,
-                  hipStream_t s) {
+                  hipStream_t s, bool work = false) {
   // nodes have at most 2^(dim * split_log) children
   const bool small_fanout = t.dim * t.split_log <= 6;
+  if (work) {
+    if constexpr (!EXACT && K <= SMOOTH_DIST) {
+      if (small_fanout && !per_query && counts) {
+        k_stochastic_sorted<D, K, true, true><<<nblk(nq, kThreads), kThreads, 0, s>>>(
+            view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, out,
+            counts, 1);
+        SBH_HIP(hipGetLastError());
+        return;
+      }
+    }
+    throw Error(kErrInvalid,
+                "work counts cover the fast sorted-sample estimator (shared paths, at most 64 "
+                "children per node, gravity / winding number / smooth distance) with counts");
+  }
   if (!EXACT && small_fanout && !per_query && nsplit > 1) {
     // partial: nsplit * nq float2, then (with counts) nsplit * nq int2
     float2* part = reinterpret_cast<float2*>(partial);
@@ -789,18 +807,19 @@ uint32_t stochastic_split_count(const DeviceTree& t, uint32_t nq, bool per_query
 
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,
                     const float* q, uint32_t nq, const uint32_t* order, float* out,
-                    int32_t* counts, int split, bool exact, float* partial, hipStream_t s) {
+                    int32_t* counts, int split, bool exact, float* partial, hipStream_t s,
+                    bool work) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
-  const uint32_t ns = partial ? stochastic_split_count(t, nq, per_query, exact) : 1;
+  const uint32_t ns = partial && !work ? stochastic_split_count(t, nq, per_query, exact) : 1;
   dispatch(t.dim, kernel, [&]<int D, int K>() {
     if (exact)
       launch_stoch<D, K, true>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
-                               split, partial, 1, s);
+                               split, partial, 1, s, work);
     else
       launch_stoch<D, K, false>(t, spl, seed, per_query, qx, qy, qz, nq, order, out, counts,
-                                split, partial, ns, s);
+                                split, partial, ns, s, work);
   });
 }
 

@@ -46,9 +46,12 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
 // counts as many int32 again); launches too small to fill the GPU then run
 // one task per root subdomain.
 uint32_t stochastic_split_count(const DeviceTree& t, uint32_t nq, bool per_query, bool exact);
+// work: counts (2 nq values) receive the work counts of the sorted-sample
+// estimator (kernel terms evaluated, then path levels stepped); never split.
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,
                     const float* q, uint32_t nq, const uint32_t* order, float* out,
-                    int32_t* counts, int split, bool exact, float* partial, hipStream_t s);
+                    int32_t* counts, int split, bool exact, float* partial, hipStream_t s,
+                    bool work = false);
 
 // Smooth distance from its sum, in place: w <- -log(w) / alpha.
 void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s);

@@ -598,7 +598,11 @@ __device__ __forceinline__ void wave_sort_keys(uint2* keys, uint32_t n) {
 // RANGE: only subdomains [si0, si1) of the root; the Barnes-Hut part of the
 // sum goes to *bh_part and the control-variate part is returned (the caller
 // adds the parts of all ranges in subdomain order).
-template <int D, int K, bool COUNT, bool RANGE = false>
+// WORK (with COUNT): count instead the work this kernel performs for the
+// query after its reuse: .points = kernel terms evaluated (a cached node's
+// children and the subdomain's level 0 count once, not once per sample),
+// .nodes = sample-path levels stepped (opening ratio, roulette and weight).
+template <int D, int K, bool COUNT, bool RANGE = false, bool WORK = false>
 __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView& t, int spl,
                                                                  uint32_t key0, uint32_t key1,
                                                                  uint32_t g, bool live, float x,
@@ -610,6 +614,9 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   constexpr bool EXACT = false;
   constexpr bool kWN = (K == WINDING_NUMBER);
   constexpr uint32_t kCh = chunk_for<K>();
+  static_assert(!WORK || COUNT, "work counts are a counting mode");
+  constexpr bool REF = COUNT && !WORK;  // the reference's term counts
+  int32_t wterms = 0, wlevels = 0;      // work counts
   const uint32_t lane = threadIdx.x & 63;
   const float4* __restrict__ nodeC = t.nodeC;
   const uint4* __restrict__ nodeL = t.nodeL;
@@ -776,6 +783,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     const float r0sum = children_sum(f0, nch0, &c0_start, &pt0, &nd0);
     const float wd0 = level_weight<EXACT>((float)nS, 1.0f, (float)nS,
                                           r0sum - term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z));
+    if constexpr (WORK) wterms += live ? (int32_t)(pt0 + nd0 + 1) : 0;
     float sub_total = 0.0f;
     for (int s0 = 0; s0 < spl; s0 += (int)kCh) {
       const uint32_t ncur = (uint32_t)min((int)kCh, spl - s0);
@@ -818,7 +826,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         float wd_all = live ? 0.0f + wd0 : 0.0f;
         float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
         bool alive = live;
-        if constexpr (COUNT) {
+        if constexpr (REF) {
           cnt += live ? (int32_t)pt0 : 0;
           cnt_nodes += live ? (int32_t)nd0 : 0;
         }
@@ -846,6 +854,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           // roulette, as selects (no per-lane branches)
           const bool stop = p > tp;
           event_prob = alive ? event_prob * (stop ? 1.0f - tp : tp) : event_prob;
+          if constexpr (WORK) wlevels += alive ? 1 : 0;
           alive = alive && !stop;
           if (__ballot(alive) == 0) break;
           const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
@@ -854,7 +863,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           bool known = false;  // the next node is the one recorded for this cached node
           if (hit) {
             diff = lds.cdiff[cslot];
-            if constexpr (COUNT) {
+            if constexpr (REF) {
               pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(8 + slot));
               nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(16 + slot));
             } else {
@@ -869,11 +878,12 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             uint32_t cs;
             const float r = children_sum(f, nch, &cs, &pt, &nd);
             diff = r - term<D, K, EXACT>(curC, curN, alpha, alpha_k, x, y, z);
+            if constexpr (WORK) wterms += alive ? (int32_t)(pt + nd + 1) : 0;
             below = (uint32_t)__popcll(__ballot(cs <= idx));
             if (slot < kCacheLevels) {
               // each lane reads back only its own diff: no wavefront sync
               lds.cdiff[cslot] = diff;
-              if constexpr (COUNT)
+              if constexpr (REF)
                 cinfo = lane == slot ? cur : lane == 8 + slot ? pt : lane == 16 + slot ? nd : cinfo;
               else
                 cinfo = lane == slot ? cur : cinfo;
@@ -881,7 +891,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           }
           const float wd = level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
           wd_all = alive ? wd_all + wd : wd_all;
-          if constexpr (COUNT) {
+          if constexpr (REF) {
             cnt += alive ? (int32_t)pt : 0;
             cnt_nodes += alive ? (int32_t)nd : 0;
           }
@@ -897,6 +907,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       wave_lds_sync();
     }
     mlcv += sub_total / (float)spl;
+  }
+  if constexpr (WORK) {
+    // the subdomains' own terms (node_accumulate) were counted in cnt / cnt_nodes
+    cnt = wterms + cnt + cnt_nodes;
+    cnt_nodes = wlevels;
   }
   if constexpr (RANGE) {
     *bh_part = bh;

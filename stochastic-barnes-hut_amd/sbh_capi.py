@@ -26,7 +26,7 @@ BRUTE_FORCE, BARNES_HUT, STOCHASTIC = 0, 1, 2
 TREE_MOMENTS_F64 = 1
 EVAL_DEVICE_POINTERS, EVAL_SORT_QUERIES, EVAL_SPLIT_COUNTS = 1, 2, 4
 EVAL_EXACT_TERMS, EVAL_MFMA, EVAL_PER_QUERY_PATHS, EVAL_MFMA_BF16 = 8, 16, 32, 64
-EVAL_SMOOTH_DISTANCE, EVAL_NO_SPLIT = 128, 256
+EVAL_SMOOTH_DISTANCE, EVAL_NO_SPLIT, EVAL_WORK_COUNTS = 128, 256, 512
 
 
 class EvalParams(ctypes.Structure):
@@ -199,7 +199,8 @@ class Tree:
         soa = np.ascontiguousarray(q.T)
         nq = len(q)
         out = np.empty(nq, np.float32)
-        cnt = np.empty(nq * (2 if flags & EVAL_SPLIT_COUNTS else 1), np.int32) if counts else None
+        cnt = (np.empty(nq * (2 if flags & (EVAL_SPLIT_COUNTS | EVAL_WORK_COUNTS) else 1), np.int32)
+               if counts else None)
         p = EvalParams(beta, samples_per_subdomain, seed, vote_group, flags)
         ms = ctypes.c_float()
         k = KERNELS[kernel] if isinstance(kernel, str) else kernel

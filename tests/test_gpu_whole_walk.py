@@ -200,3 +200,59 @@ def test_plummer_headline_shape_small(sbh_gpu):
     check_bh(sbh_gpu, g, o, q, "gravity", 1.0, "plummer", no_split=True, sort_queries=True)
     check_stochastic(sbh_gpu, g, o, q, "gravity", 64, 1, "plummer spl64", order=order,
                      no_split=True, sort_queries=True)
+
+
+def test_work_counts(sbh_gpu):
+    """SBH_EVAL_WORK_COUNTS (the instrumentation behind the benchmark's
+    roofline): values unchanged; with one sample per subdomain nothing is
+    reused, so the kernel terms are the reference counts plus one own term per
+    internal subdomain and per evaluated path level; with many sorted samples
+    the node cache evaluates far fewer terms than the reference sums; modes it
+    does not cover are refused."""
+    import sbh_capi as C
+
+    pts, nrm, m, q, lo, hi = _inputs(6000, 700, 3, 191, "gravity")
+    t = C.Tree(pts, None, m, lo, hi, split_log=2)
+    nq = len(q)
+    nsub_internal = int(np.count_nonzero(
+        t_nodes_internal_children(t)))
+    for spl in (1, 64):
+        ref, _, rc = t.eval_host(C.STOCHASTIC, "gravity", q, samples_per_subdomain=spl, seed=4,
+                                 flags=C.EVAL_SPLIT_COUNTS | C.EVAL_NO_SPLIT, counts=True)
+        got, _, wc = t.eval_host(C.STOCHASTIC, "gravity", q, samples_per_subdomain=spl, seed=4,
+                                 flags=C.EVAL_WORK_COUNTS, counts=True)
+        np.testing.assert_array_equal(got, ref)
+        ref_terms = rc[:nq] + rc[nq:]
+        terms, levels = wc[:nq].astype(np.int64), wc[nq:].astype(np.int64)
+        print(f"[work] spl {spl}: reference {ref_terms.sum()} terms, evaluated {terms.sum()}, "
+              f"levels {levels.sum()}", flush=True)
+        assert (levels >= 0).all() and (terms > 0).all()
+        if spl == 1:
+            assert (terms >= ref_terms).all()
+            assert (terms <= ref_terms + nsub_internal + levels).all()
+        else:
+            assert terms.sum() * 4 < ref_terms.sum()
+    for bad in (C.EVAL_EXACT_TERMS, C.EVAL_PER_QUERY_PATHS):
+        with pytest.raises(C.SbhError):
+            t.eval_host(C.STOCHASTIC, "gravity", q, samples_per_subdomain=2,
+                        flags=C.EVAL_WORK_COUNTS | bad, counts=True)
+    with pytest.raises(C.SbhError):
+        t.eval_host(C.BARNES_HUT, "gravity", q, beta=2.0, flags=C.EVAL_WORK_COUNTS, counts=True)
+    with pytest.raises(C.SbhError):
+        t.eval_host(C.STOCHASTIC, "gravity", q, samples_per_subdomain=2, flags=C.EVAL_WORK_COUNTS)
+    t.close()
+
+
+def t_nodes_internal_children(t):
+    """Per root child of a sbh_capi tree: 1 if it is an internal node."""
+    import ctypes
+
+    import sbh_capi as C
+
+    nn = t.info()["num_nodes"]
+    first = np.empty(nn, np.uint32)
+    nch = np.empty(nn, np.uint32)
+    C.check(C.lib().sbh_tree_nodes(t.h, None, None, first.ctypes.data, nch.ctypes.data,
+                                   *([None] * 10)))
+    kids = np.arange(first[0], first[0] + nch[0])
+    return (nch[kids] > 0).astype(np.int64)
