@@ -189,6 +189,26 @@ int sbh_tree_nodes(const sbh_tree* tree, uint32_t* start, uint32_t* end, uint32_
 int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_params* params,
              const float* queries, int64_t q, float* results, int32_t* counts, float* kernel_ms);
 
+/*
+ * sbh_eval for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, kernels and results as
+ * sbh_eval (bit-identical), but params->flags must include
+ * SBH_EVAL_DEVICE_POINTERS and the call makes no device allocation: the
+ * scratch of a sorted launch (launch order and sort work space, 26 bytes a
+ * query) and of split launches (partial sums) comes from one static array of
+ * SBH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  An
+ * evaluation that needs more fails with SBH_ERR_INVALID and a message giving
+ * the size (e.g. sorted launches above about 2.5M queries; SBH_EVAL_NO_SPLIT
+ * removes the partial sums).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two trees on two threads or streams) race
+ * and must be serialised by the caller.
+ */
+#define SBH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int sbh_eval_noalloc(sbh_tree* tree, int32_t method, int32_t kernel,
+                     const sbh_eval_params* params, const float* queries, int64_t q,
+                     float* results, int32_t* counts, float* kernel_ms);
+
 /* float64 all-pairs sums of the fp32 kernel formulas (error reference). */
 int sbh_eval_f64(sbh_tree* tree, int32_t kernel, const float* queries, int64_t q,
                  double* results, int32_t flags, float* kernel_ms);

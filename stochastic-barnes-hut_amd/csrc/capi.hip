@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -35,6 +36,7 @@ struct sbh_tree {
   sbh::DevBuf<uint32_t> order;
   sbh::DevBuf<uint32_t> sort_scratch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int device = 0;  // the GPU the tree lives on
 };
 
 namespace {
@@ -111,6 +113,212 @@ void check_tree_inputs(int32_t dim, int32_t split_log, int32_t leaf_size, int32_
   SBH_REQUIRE(std::isfinite(alpha), "alpha must be finite");
 }
 
+// Scratch of sbh_eval_noalloc: one array per GPU (each device holds its own
+// copy of a __device__ variable), kept until the process ends.  Host code
+// never uses the name itself (on the host it is only a stand-in for the
+// symbol): the device address comes from hipGetSymbolAddress, once per GPU.
+static __device__ __align__(256) unsigned char scratch[SBH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on `device`, looked up on the first call for that
+// device with the device current.
+unsigned char* scratch_on(int device) {
+  SBH_REQUIRE(device >= 0 && device < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    SBH_HIP(hipGetDevice(&cur));
+    if (cur != device) SBH_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) SBH_HIP(hipSetDevice(cur));
+    SBH_HIP(e);
+    SBH_REQUIRE(ptr != nullptr, "no device address for the evaluation scratch");
+    g_scratch_ptr[device] = ptr;
+  }
+  return static_cast<unsigned char*>(g_scratch_ptr[device]);
+}
+
+// Device scratch of one float-build evaluation: the launch order of a sorted
+// launch, the sort's work space, the split launches' partial sums.  Either
+// grown on demand in the tree's own buffers, or carved from a fixed region
+// (sbh_eval_noalloc), which is never grown: a request beyond it fails.
+class EvalScratch {
+ public:
+  explicit EvalScratch(sbh_tree* tree) : tree_(tree) {}
+  EvalScratch(unsigned char* base, size_t bytes) : base_(base), cap_(bytes) {}
+  bool fixed() const { return base_ != nullptr; }
+  uint32_t* order(size_t n) {
+    if (!fixed()) {
+      ensure(tree_->order, n);
+      return tree_->order.get();
+    }
+    return static_cast<uint32_t*>(carve(n * 4, "launch order"));
+  }
+  uint32_t* sort(size_t n) {
+    if (!fixed()) {
+      ensure(tree_->sort_scratch, n);
+      return tree_->sort_scratch.get();
+    }
+    return static_cast<uint32_t*>(carve(n * 4, "query sort"));
+  }
+  double* partial(size_t n) {
+    if (!fixed()) {
+      ensure(tree_->partial, n);
+      return tree_->partial.get();
+    }
+    return static_cast<double*>(carve(n * 8, "partial sums"));
+  }
+
+ private:
+  void* carve(size_t bytes, const char* what) {
+    const size_t off = (used_ + 255) & ~(size_t)255;
+    if (off + bytes > cap_)
+      throw sbh::Error(SBH_ERR_INVALID,
+                       std::string("sbh_eval_noalloc: this evaluation needs more than the ") +
+                           std::to_string(cap_) + " bytes of its scratch (at the " + what +
+                           ": " + std::to_string(off + bytes) +
+                           " bytes); fewer queries, SBH_EVAL_NO_SPLIT or sbh_eval fit");
+    used_ = off + bytes;
+    return base_ + off;
+  }
+  sbh_tree* tree_ = nullptr;
+  unsigned char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+};
+
+// sbh_eval's body; with fixed scratch (sbh_eval_noalloc) it allocates no
+// device memory: device pointers only, every scratch region carved.
+void eval_float(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_params* params,
+                const float* queries, int64_t q, float* results, int32_t* counts,
+                float* kernel_ms, EvalScratch& sc) {
+  SBH_REQUIRE(tree && params, "NULL argument");
+  require_float_build(tree->t, sc.fixed() ? "sbh_eval_noalloc" : "sbh_eval");
+  SBH_REQUIRE(method == SBH_BRUTE_FORCE || method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
+              "unknown method");
+  SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
+  SBH_REQUIRE(kernel != SBH_GRAVITY_GRAD_Z || tree->t.dim == 3,
+              "the gradient's z component needs a 3D tree");
+  SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
+  if (kernel_ms) *kernel_ms = 0.0f;
+  if (q == 0) return;
+  SBH_REQUIRE(queries && results, "NULL queries or results");
+  const sbh_eval_params& p = *params;
+  SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
+                           SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
+                           SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16 |
+                           SBH_EVAL_SMOOTH_DISTANCE | SBH_EVAL_NO_SPLIT |
+                           SBH_EVAL_WORK_COUNTS)) == 0,
+              "unknown eval flags");
+  const bool work = (p.flags & SBH_EVAL_WORK_COUNTS) != 0;
+  if (work)
+    SBH_REQUIRE(method == SBH_STOCHASTIC && counts &&
+                    !(p.flags & (SBH_EVAL_EXACT_TERMS | SBH_EVAL_PER_QUERY_PATHS)),
+                "SBH_EVAL_WORK_COUNTS applies to the fast shared-path estimator, with counts");
+  SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
+              "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
+  if (method == SBH_BARNES_HUT) {
+    SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
+    SBH_REQUIRE(p.vote_group == 1 || p.vote_group == 32 || p.vote_group == 64,
+                "vote_group must be 1, 32 or 64");
+  }
+  if (method == SBH_STOCHASTIC)
+    SBH_REQUIRE(p.samples_per_subdomain >= 1, "samples_per_subdomain must be >= 1");
+  sbh::DeviceTree& t = tree->t;
+  hipStream_t s = t.stream;
+  const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
+  const int split = (p.flags & (SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_WORK_COUNTS)) ? 1 : 0;
+  const bool exact = (p.flags & SBH_EVAL_EXACT_TERMS) != 0;
+  const uint32_t nq = (uint32_t)q;
+  const size_t ncnt = split ? 2ull * nq : nq;
+  const size_t qfloats = (size_t)nq * t.dim;
+  const float* dq = queries;
+  float* dout = results;
+  int32_t* dcnt = counts;
+  if (!dev) {
+    SBH_REQUIRE(!sc.fixed(), "sbh_eval_noalloc takes device pointers (SBH_EVAL_DEVICE_POINTERS)");
+    SBH_REQUIRE(all_finite(queries, qfloats), "queries must be finite");
+    ensure(tree->q, qfloats);
+    ensure(tree->out, nq);
+    SBH_HIP(hipMemcpyAsync(tree->q.get(), queries, qfloats * 4, hipMemcpyHostToDevice, s));
+    dq = tree->q.get();
+    dout = tree->out.get();
+    if (counts) {
+      ensure(tree->counts, ncnt);
+      dcnt = tree->counts.get();
+    }
+  }
+  // Counting only exists for the tree walks.
+  if (method == SBH_BRUTE_FORCE) {
+    const bool mfma = (p.flags & (SBH_EVAL_MFMA | SBH_EVAL_MFMA_BF16)) != 0;
+    if (mfma)
+      SBH_REQUIRE(t.dim == 3 && (kernel == SBH_GRAVITY || kernel == SBH_SMOOTH_DIST),
+                  "SBH_EVAL_MFMA covers 3D gravity and smooth distance");
+    uint32_t ns = mfma ? sbh::brute_mfma_split_count(t.n, nq) : sbh::brute_split_count(t.n, nq);
+    double* partial = ns > 1 ? sc.partial((size_t)ns * nq) : nullptr;
+    uint32_t* ord = mfma ? sc.order(nq) : nullptr;
+    uint32_t* sort_ws = mfma ? sc.sort(sbh::sort_scratch_u32(nq)) : nullptr;
+    SBH_HIP(hipEventRecord(tree->ev0, s));
+    if (mfma) {
+      sbh::query_order(t, dq, nq, ord, sort_ws, s);
+      sbh::run_brute_mfma(t, kernel, (p.flags & SBH_EVAL_MFMA_BF16) != 0, dq, nq, ord, dout,
+                          partial, ns, s);
+    } else {
+      sbh::run_brute(t, kernel, dq, nq, dout, partial, ns, s);
+    }
+    SBH_HIP(hipEventRecord(tree->ev1, s));
+    if (dcnt) {
+      // every query evaluates every point; split counts have no node terms
+      if (split) sbh::fill_i32(dcnt, nq, 0, s);
+      sbh::fill_i32(dcnt + (split ? nq : 0), nq, (int32_t)t.n, s);
+    }
+  } else {
+    const bool sort = (p.flags & SBH_EVAL_SORT_QUERIES) != 0;
+    uint32_t* ord = sort ? sc.order(nq) : nullptr;
+    uint32_t* sort_ws = sort ? sc.sort(sbh::sort_scratch_u32(nq)) : nullptr;
+    float* bh_partial = nullptr;
+    const bool no_split = (p.flags & (SBH_EVAL_NO_SPLIT | SBH_EVAL_WORK_COUNTS)) != 0;
+    if (no_split) {
+      // whole-walk kernels at any size: no partial sums
+    } else if (method == SBH_BARNES_HUT) {
+      const uint32_t ns = sbh::bh_split_count(t, nq, p.vote_group, exact, dcnt != nullptr);
+      if (ns > 1) {
+        // ns * nq floats (rounded up to an even count), then ns * nq int2 counts
+        bh_partial = reinterpret_cast<float*>(
+            sc.partial(((size_t)ns * nq + 1) / 2 + (dcnt ? (size_t)ns * nq : 0)));
+      }
+    } else {
+      const uint32_t ns = sbh::stochastic_split_count(
+          t, nq, (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, exact);
+      if (ns > 1) {
+        // ns * nq float2 parts, then ns * nq int2 counts
+        bh_partial = reinterpret_cast<float*>(sc.partial((size_t)ns * nq * (dcnt ? 2 : 1)));
+      }
+    }
+    SBH_HIP(hipEventRecord(tree->ev0, s));
+    if (sort) sbh::query_order(t, dq, nq, ord, sort_ws, s);
+    if (method == SBH_BARNES_HUT)
+      sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, exact,
+                  bh_partial, s);
+    else
+      sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed,
+                          (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout, dcnt,
+                          split, exact, bh_partial, s, work);
+    SBH_HIP(hipEventRecord(tree->ev1, s));
+  }
+  if (p.flags & SBH_EVAL_SMOOTH_DISTANCE) sbh::run_smooth_distance(dout, nq, t.alpha, s);
+  if (!dev) {
+    SBH_HIP(hipMemcpyAsync(results, dout, nq * 4, hipMemcpyDeviceToHost, s));
+    if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, ncnt * 4, hipMemcpyDeviceToHost, s));
+  }
+  SBH_HIP(hipEventSynchronize(tree->ev1));
+  if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));
+  SBH_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace
 
 extern "C" {
@@ -155,6 +363,7 @@ int sbh_tree_create(int32_t dim, int32_t split_log, int32_t leaf_size, int32_t m
       SBH_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
       SBH_HIP(hipEventCreate(&h->ev0));
       SBH_HIP(hipEventCreate(&h->ev1));
+      SBH_HIP(hipGetDevice(&h->device));
       sbh::build_device_tree(t, points, normals, masses, build_ms);
     } catch (...) {
       sbh_tree_destroy(h);
@@ -195,6 +404,7 @@ int sbh_tree_create_double(int32_t dim, int32_t split_log, int32_t leaf_size,
       SBH_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
       SBH_HIP(hipEventCreate(&h->ev0));
       SBH_HIP(hipEventCreate(&h->ev1));
+      SBH_HIP(hipGetDevice(&h->device));
       sbh::build_device_tree_double(t, points, normals, masses, build_ms);
     } catch (...) {
       sbh_tree_destroy(h);
@@ -261,6 +471,7 @@ sbh_tree* tree_from_image(const uint8_t* data, size_t size) {
     sbh::deserialize_tree(h->t, data, size);  // validates, then creates the stream
     SBH_HIP(hipEventCreate(&h->ev0));
     SBH_HIP(hipEventCreate(&h->ev1));
+    SBH_HIP(hipGetDevice(&h->device));
   } catch (...) {
     sbh_tree_destroy(h);
     throw;
@@ -466,137 +677,21 @@ int sbh_query_order(sbh_tree* tree, const float* queries, int64_t q, uint32_t* o
 int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_params* params,
              const float* queries, int64_t q, float* results, int32_t* counts, float* kernel_ms) {
   return guarded([&] {
+    SBH_REQUIRE(tree, "NULL argument");
+    EvalScratch sc(tree);
+    eval_float(tree, method, kernel, params, queries, q, results, counts, kernel_ms, sc);
+  });
+}
+
+int sbh_eval_noalloc(sbh_tree* tree, int32_t method, int32_t kernel,
+                     const sbh_eval_params* params, const float* queries, int64_t q,
+                     float* results, int32_t* counts, float* kernel_ms) {
+  return guarded([&] {
     SBH_REQUIRE(tree && params, "NULL argument");
-    require_float_build(tree->t, "sbh_eval");
-    SBH_REQUIRE(method == SBH_BRUTE_FORCE || method == SBH_BARNES_HUT || method == SBH_STOCHASTIC,
-                "unknown method");
-    SBH_REQUIRE(kernel >= SBH_GRAVITY && kernel <= SBH_GRAVITY_GRAD_Z, "unknown kernel");
-    SBH_REQUIRE(kernel != SBH_GRAVITY_GRAD_Z || tree->t.dim == 3,
-                "the gradient's z component needs a 3D tree");
-    SBH_REQUIRE(q >= 0 && q < (int64_t)0x7FFFFFFF, "query count out of range");
-    if (kernel_ms) *kernel_ms = 0.0f;
-    if (q == 0) return;
-    SBH_REQUIRE(queries && results, "NULL queries or results");
-    const sbh_eval_params& p = *params;
-    SBH_REQUIRE((p.flags & ~(SBH_EVAL_DEVICE_POINTERS | SBH_EVAL_SORT_QUERIES |
-                             SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_EXACT_TERMS | SBH_EVAL_MFMA |
-                             SBH_EVAL_PER_QUERY_PATHS | SBH_EVAL_MFMA_BF16 |
-                             SBH_EVAL_SMOOTH_DISTANCE | SBH_EVAL_NO_SPLIT |
-                             SBH_EVAL_WORK_COUNTS)) == 0,
-                "unknown eval flags");
-    const bool work = (p.flags & SBH_EVAL_WORK_COUNTS) != 0;
-    if (work)
-      SBH_REQUIRE(method == SBH_STOCHASTIC && counts &&
-                      !(p.flags & (SBH_EVAL_EXACT_TERMS | SBH_EVAL_PER_QUERY_PATHS)),
-                  "SBH_EVAL_WORK_COUNTS applies to the fast shared-path estimator, with counts");
-    SBH_REQUIRE(!(p.flags & SBH_EVAL_SMOOTH_DISTANCE) || kernel == SBH_SMOOTH_DIST,
-                "SBH_EVAL_SMOOTH_DISTANCE applies to the smooth-distance kernel");
-    if (method == SBH_BARNES_HUT) {
-      SBH_REQUIRE(!std::isnan(p.beta), "beta is NaN");
-      SBH_REQUIRE(p.vote_group == 1 || p.vote_group == 32 || p.vote_group == 64,
-                  "vote_group must be 1, 32 or 64");
-    }
-    if (method == SBH_STOCHASTIC)
-      SBH_REQUIRE(p.samples_per_subdomain >= 1, "samples_per_subdomain must be >= 1");
-    sbh::DeviceTree& t = tree->t;
-    hipStream_t s = t.stream;
-    const bool dev = p.flags & SBH_EVAL_DEVICE_POINTERS;
-    const int split = (p.flags & (SBH_EVAL_SPLIT_COUNTS | SBH_EVAL_WORK_COUNTS)) ? 1 : 0;
-    const bool exact = (p.flags & SBH_EVAL_EXACT_TERMS) != 0;
-    const uint32_t nq = (uint32_t)q;
-    const size_t ncnt = split ? 2ull * nq : nq;
-    const size_t qfloats = (size_t)nq * t.dim;
-    const float* dq = queries;
-    float* dout = results;
-    int32_t* dcnt = counts;
-    if (!dev) {
-      SBH_REQUIRE(all_finite(queries, qfloats), "queries must be finite");
-      ensure(tree->q, qfloats);
-      ensure(tree->out, nq);
-      SBH_HIP(hipMemcpyAsync(tree->q.get(), queries, qfloats * 4, hipMemcpyHostToDevice, s));
-      dq = tree->q.get();
-      dout = tree->out.get();
-      if (counts) {
-        ensure(tree->counts, ncnt);
-        dcnt = tree->counts.get();
-      }
-    }
-    // Counting only exists for the tree walks.
-    if (method == SBH_BRUTE_FORCE) {
-      const bool mfma = (p.flags & (SBH_EVAL_MFMA | SBH_EVAL_MFMA_BF16)) != 0;
-      if (mfma)
-        SBH_REQUIRE(t.dim == 3 && (kernel == SBH_GRAVITY || kernel == SBH_SMOOTH_DIST),
-                    "SBH_EVAL_MFMA covers 3D gravity and smooth distance");
-      uint32_t ns = mfma ? sbh::brute_mfma_split_count(t.n, nq) : sbh::brute_split_count(t.n, nq);
-      if (ns > 1) ensure(tree->partial, (size_t)ns * nq);
-      if (mfma) {
-        ensure(tree->order, nq);
-        ensure(tree->sort_scratch, sbh::sort_scratch_u32(nq));
-      }
-      SBH_HIP(hipEventRecord(tree->ev0, s));
-      if (mfma) {
-        sbh::query_order(t, dq, nq, tree->order.get(), tree->sort_scratch.get(), s);
-        sbh::run_brute_mfma(t, kernel, (p.flags & SBH_EVAL_MFMA_BF16) != 0, dq, nq,
-                            tree->order.get(), dout, tree->partial.get(), ns,
-                            s);
-      } else {
-        sbh::run_brute(t, kernel, dq, nq, dout, tree->partial.get(), ns, s);
-      }
-      SBH_HIP(hipEventRecord(tree->ev1, s));
-      if (dcnt) {
-        // every query evaluates every point; split counts have no node terms
-        std::vector<int32_t> c(ncnt, (int32_t)t.n);
-        if (split) std::fill(c.begin(), c.begin() + nq, 0);
-        SBH_HIP(hipMemcpyAsync(dcnt, c.data(), ncnt * 4, hipMemcpyHostToDevice, s));
-        SBH_HIP(hipStreamSynchronize(s));
-      }
-    } else {
-      const bool sort = (p.flags & SBH_EVAL_SORT_QUERIES) != 0;
-      uint32_t* ord = nullptr;
-      if (sort) {
-        ensure(tree->order, nq);
-        ensure(tree->sort_scratch, sbh::sort_scratch_u32(nq));
-        ord = tree->order.get();
-      }
-      float* bh_partial = nullptr;
-      const bool no_split = (p.flags & (SBH_EVAL_NO_SPLIT | SBH_EVAL_WORK_COUNTS)) != 0;
-      if (no_split) {
-        // whole-walk kernels at any size: no partial sums
-      } else if (method == SBH_BARNES_HUT) {
-        const uint32_t ns = sbh::bh_split_count(t, nq, p.vote_group, exact, dcnt != nullptr);
-        if (ns > 1) {
-          // ns * nq floats (rounded up to an even count), then ns * nq int2 counts
-          ensure(tree->partial, ((size_t)ns * nq + 1) / 2 + (dcnt ? (size_t)ns * nq : 0));
-          bh_partial = reinterpret_cast<float*>(tree->partial.get());
-        }
-      } else {
-        const uint32_t ns = sbh::stochastic_split_count(
-            t, nq, (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, exact);
-        if (ns > 1) {
-          // ns * nq float2 parts, then ns * nq int2 counts
-          ensure(tree->partial, (size_t)ns * nq * (dcnt ? 2 : 1));
-          bh_partial = reinterpret_cast<float*>(tree->partial.get());
-        }
-      }
-      SBH_HIP(hipEventRecord(tree->ev0, s));
-      if (sort) sbh::query_order(t, dq, nq, ord, tree->sort_scratch.get(), s);
-      if (method == SBH_BARNES_HUT)
-        sbh::run_bh(t, kernel, p.beta, p.vote_group, dq, nq, ord, dout, dcnt, split, exact,
-                    bh_partial, s);
-      else
-        sbh::run_stochastic(t, kernel, p.samples_per_subdomain, p.seed,
-                            (p.flags & SBH_EVAL_PER_QUERY_PATHS) != 0, dq, nq, ord, dout, dcnt,
-                            split, exact, bh_partial, s, work);
-      SBH_HIP(hipEventRecord(tree->ev1, s));
-    }
-    if (p.flags & SBH_EVAL_SMOOTH_DISTANCE) sbh::run_smooth_distance(dout, nq, t.alpha, s);
-    if (!dev) {
-      SBH_HIP(hipMemcpyAsync(results, dout, nq * 4, hipMemcpyDeviceToHost, s));
-      if (counts) SBH_HIP(hipMemcpyAsync(counts, dcnt, ncnt * 4, hipMemcpyDeviceToHost, s));
-    }
-    SBH_HIP(hipEventSynchronize(tree->ev1));
-    if (kernel_ms) SBH_HIP(hipEventElapsedTime(kernel_ms, tree->ev0, tree->ev1));
-    SBH_HIP(hipStreamSynchronize(s));
+    SBH_REQUIRE(params->flags & SBH_EVAL_DEVICE_POINTERS,
+                "sbh_eval_noalloc takes device pointers (SBH_EVAL_DEVICE_POINTERS)");
+    EvalScratch sc(scratch_on(tree->device), SBH_NOALLOC_SCRATCH_BYTES);
+    eval_float(tree, method, kernel, params, queries, q, results, counts, kernel_ms, sc);
   });
 }
 

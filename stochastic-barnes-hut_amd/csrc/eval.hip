@@ -824,12 +824,23 @@ void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, boo
 }
 
 namespace {
+__global__ void k_fill_i32(int32_t* p, uint32_t n, int32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // -log(w) / alpha in place (python/example.py:79).
 __global__ void k_smooth_distance(float* w, uint32_t n, float alpha) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) w[i] = -logf(w[i]) / alpha;
 }
 }  // namespace
+
+void fill_i32(int32_t* p, uint32_t n, int32_t v, hipStream_t s) {
+  if (n == 0) return;
+  k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(p, n, v);
+  SBH_HIP(hipGetLastError());
+}
 
 void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s) {
   k_smooth_distance<<<nblk(n, 256), 256, 0, s>>>(w, n, alpha);

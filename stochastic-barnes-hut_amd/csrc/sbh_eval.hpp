@@ -53,6 +53,9 @@ void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, boo
                     int32_t* counts, int split, bool exact, float* partial, hipStream_t s,
                     bool work = false);
 
+// p[0, n) <- v, on the stream (a kernel: no host copy, no memset node).
+void fill_i32(int32_t* p, uint32_t n, int32_t v, hipStream_t s);
+
 // Smooth distance from its sum, in place: w <- -log(w) / alpha.
 void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s);
 

@@ -71,6 +71,7 @@ def lib():
         L.sbh_tree_nodes.argtypes = [vp] + [vp] * 14
         L.sbh_eval.argtypes = [vp, i32, i32, ctypes.POINTER(EvalParams), vp, i64, vp, vp,
                                ctypes.POINTER(f32)]
+        L.sbh_eval_noalloc.argtypes = L.sbh_eval.argtypes
         L.sbh_eval_f64.argtypes = [vp, i32, vp, i64, vp, i32, ctypes.POINTER(f32)]
         L.sbh_query_order.argtypes = [vp, vp, i64, vp, i32]
         L.sbh_tree_stream.argtypes = [vp, ctypes.POINTER(vp)]
@@ -191,6 +192,19 @@ class Tree:
         check(lib().sbh_eval(self.h, method, k, ctypes.byref(p), ctypes.c_void_p(q_ptr), nq,
                              ctypes.c_void_p(out_ptr), ctypes.c_void_p(counts_ptr or None),
                              ctypes.byref(ms)))
+        return ms.value
+
+    def eval_device_noalloc(self, method, kernel, q_ptr, nq, out_ptr, counts_ptr=0, beta=0.0,
+                            samples_per_subdomain=1, seed=0, vote_group=1, flags=0) -> float:
+        """eval_device through sbh_eval_noalloc: no device allocation, scratch
+        from the library's fixed per-GPU array."""
+        p = EvalParams(beta, samples_per_subdomain, seed, vote_group,
+                       flags | EVAL_DEVICE_POINTERS)
+        ms = ctypes.c_float()
+        k = KERNELS[kernel] if isinstance(kernel, str) else kernel
+        check(lib().sbh_eval_noalloc(self.h, method, k, ctypes.byref(p), ctypes.c_void_p(q_ptr),
+                                     nq, ctypes.c_void_p(out_ptr),
+                                     ctypes.c_void_p(counts_ptr or None), ctypes.byref(ms)))
         return ms.value
 
     def eval_host(self, method, kernel, queries, beta=0.0, samples_per_subdomain=1, seed=0,
