@@ -301,7 +301,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-variants", action="store_true", help="skip the variant workloads")
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="Poisson offered load as a fraction of the measured burst capacity")
     ap.add_argument("--tree-k", type=int, default=2,
@@ -591,6 +591,9 @@ def profile(wl, args):
         r = serving_bench(args)
         log(f"serve: burst {r['burst']['queries_per_s'] / 1e6:.1f} M q/s, poisson p50 "
             f"{r['poisson']['p50_ms']:.1f} ms, p99 {r['poisson']['p99_ms']:.1f} ms")
+        os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+        with open(args.detail, "w") as f:
+            json.dump(r, f, indent=1, default=float)
         return
     if args.profile_kernel == "double":
         # the double build: Barnes-Hut at --theta and the estimator at --spl[0],
@@ -719,6 +722,20 @@ def serving_bench(args):
     log(f"serve poisson: rate {rate:.1f} req/s, p50 {np.percentile(lat_p, 50):.1f} ms, "
         f"p99 {np.percentile(lat_p, 99):.1f} ms")
     st = srv.stats()
+    # the tail: each of the slowest requests once more, alone on the server
+    tail = []
+    for i in np.argsort(lat_p)[::-1][:6]:
+        t, method, pts, spl, seed = reqs[i]
+        h = srv.submit(t["id"], method, t["kernel"], pts, beta=2.0, samples_per_subdomain=spl,
+                       seed=seed)
+        alone = h.result(600.0)[1]
+        tail.append({"latency_ms": float(lat_p[i]), "alone_ms": alone, "queries": len(pts),
+                     "method": method, "tree": t["name"]})
+        log(f"serve tail: {len(pts)} queries {method} on {t['name']}: {lat_p[i]:.1f} ms in the "
+            f"stream, {alone:.1f} ms alone")
+    per_request = [{"queries": len(r[2]), "method": r[1], "tree": r[0]["name"],
+                    "arrival_s": float(a), "latency_ms": float(l)}
+                   for r, a, l in zip(reqs, arrivals, lat_p)]
     # shut the engine down before the next leg: its scheduler thread and
     # streams end here, not at interpreter exit
     import gc
@@ -738,6 +755,7 @@ def serving_bench(args):
                     "queries_per_s": total_q / el_p, "elapsed_s": el_p,
                     "p50_ms": float(np.percentile(lat_p, 50)),
                     "p99_ms": float(np.percentile(lat_p, 99))},
+        "tail": tail, "per_request": per_request,
         "launches": st["launches"], "peak_request_bytes": st["peak_bytes"],
         "budget_bytes": st["budget_bytes"], "resident_tree_bytes": st["tree_bytes"],
     }

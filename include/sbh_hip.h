@@ -298,12 +298,12 @@ int sbh_tree_load(const char* path, sbh_tree** out);
  * Barnes-Hut runs per query (vote_group is ignored), the stochastic estimator
  * keys paths on (seed, 64-query block of the request's sorted order): a
  * request's values are identical alone and in any mix, and bit-identical to
- * those of sbh_eval with SBH_EVAL_SORT_QUERIES | SBH_EVAL_NO_SPLIT (without
- * SBH_EVAL_NO_SPLIT, a one-shot launch of fewer than 2^18 queries sums the
- * same terms per root child, equal up to fp32 rounding).  Terms use the fast
- * kernels; params.flags may hold only SBH_EVAL_SORT_QUERIES and
- * SBH_EVAL_NO_SPLIT (what the engine always does), anything else is refused
- * with SBH_ERR_INVALID.
+ * those of sbh_eval with SBH_EVAL_SORT_QUERIES plus the request's flags: a
+ * request of fewer than 2^18 queries has its walks split over the root's
+ * children as a one-shot launch of that size does, unless params.flags holds
+ * SBH_EVAL_NO_SPLIT.  Terms use the fast kernels; params.flags may hold only
+ * SBH_EVAL_SORT_QUERIES and SBH_EVAL_NO_SPLIT, anything else is refused with
+ * SBH_ERR_INVALID.
  *
  * Trees must outlive the server.  `queries` (column-major dim x q) and
  * `results` (q floats) are host memory that must stay valid until the request

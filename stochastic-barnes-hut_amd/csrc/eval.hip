@@ -190,26 +190,11 @@ __global__ __launch_bounds__(kThreads) void k_bh_split(TreeView t, float beta,
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const WalkRecord root = t.dfsR[0];
-  // the root's opening test, decided exactly as the whole walk decides it
-  const float diam = __uint_as_float(root.l.x);
-  int far = opens_far_fast<D>(root.c, __builtin_amdgcn_rcpf(diam), beta, x, y, z);
-  if (far < 0) far = (int)opens_far_exact<D>(root.c, diam, beta, x, y, z);
-  const bool take_root = far != 0;
-  // depth-first position of the root's child blockIdx.y: follow the sibling ropes
-  uint32_t v0 = 1;
-  for (uint32_t j = 0; j < blockIdx.y; ++j) v0 = uniform_u32(t.dfsR[v0].l.z);
-  const uint32_t vstop = uniform_u32(t.dfsR[v0].l.z);
-  float acc = 0.0f;
-  if (blockIdx.y == 0 && take_root)
-    acc = accum<D, K, false>(acc, root.c, t.dfsN[0], t.alpha, t.alpha_k, x, y, z);
-  const LaneResult r =
-      bh_walk<D, K, 1, false, COUNT>(t, beta, live && !take_root, x, y, z, v0, vstop);
+  const LaneResult r = bh_walk_subtree<D, K, COUNT>(t, beta, live, x, y, z, blockIdx.y);
   if (live) {
     const size_t e = (size_t)blockIdx.y * nq + slot;
-    partial[e] = take_root ? acc : r.value;
-    if constexpr (COUNT)
-      pcnt[e] = take_root ? make_int2(blockIdx.y == 0 ? 1 : 0, 0) : make_int2(r.nodes, r.points);
+    partial[e] = r.value;
+    if constexpr (COUNT) pcnt[e] = make_int2(r.nodes, r.points);
   }
 }
 
@@ -219,10 +204,8 @@ __global__ void k_reduce_bh_split(const float* __restrict__ partial,
                                   int32_t* __restrict__ counts, int split_counts) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nq) return;
-  float s = partial[slot];
-  for (uint32_t k = 1; k < nsplit; ++k) s += partial[(size_t)k * nq + slot];
   const uint32_t qi = order ? order[slot] : slot;
-  out[qi] = s;
+  out[qi] = sum_bh_parts(partial, nsplit, nq, slot);
   if (counts) {
     int32_t nodes = 0, points = 0;
     for (uint32_t k = 0; k < nsplit; ++k) {
@@ -372,14 +355,8 @@ __global__ void k_reduce_stoch_split(const float2* __restrict__ partial,
                                      int split_counts) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nq) return;
-  float bh = 0.0f, mlcv = 0.0f;
-  for (uint32_t k = 0; k < nsplit; ++k) {
-    const float2 p = partial[(size_t)k * nq + slot];
-    bh += p.x;
-    mlcv += p.y;
-  }
   const uint32_t qi = order ? order[slot] : slot;
-  out[qi] = bh + mlcv;
+  out[qi] = sum_stochastic_parts(partial, nsplit, nq, slot);
   if (counts) {
     int32_t nodes = 0, points = 0;
     for (uint32_t k = 0; k < nsplit; ++k) {

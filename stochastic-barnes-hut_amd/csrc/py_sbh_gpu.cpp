@@ -454,7 +454,7 @@ struct PyServer {
     }
   }
   PyRequest submit(int tree_id, const std::string& method, const std::string& kernel,
-                   F32Array queries, float beta, int spl, uint64_t seed) {
+                   F32Array queries, float beta, int spl, uint64_t seed, bool no_split) {
     sweep();
     if (tree_id < 0 || tree_id >= (int)trees.size()) throw py::value_error("unknown tree id");
     GPUTreeBase& t = trees[tree_id].cast<GPUTreeBase&>();
@@ -475,6 +475,7 @@ struct PyServer {
     r.params.samples_per_subdomain = spl;
     r.params.seed = seed;
     r.params.vote_group = 1;
+    r.params.flags = no_split ? SBH_EVAL_NO_SPLIT : 0;
     r.queries = b->queries.data();
     r.q = nq;
     r.results = b->results.mutable_data();
@@ -543,7 +544,8 @@ PYBIND11_MODULE(sbh_gpu, m) {
       .def("add_tree", &PyServer::add_tree, "tree"_a)
       // a Request keeps its Server alive: wait / poll never run on a destroyed engine
       .def("submit", &PyServer::submit, "tree_id"_a, "method"_a, "kernel"_a, "queries"_a,
-           "beta"_a = 2.0f, "samples_per_subdomain"_a = 1, "seed"_a = 0, py::keep_alive<0, 1>())
+           "beta"_a = 2.0f, "samples_per_subdomain"_a = 1, "seed"_a = 0, "no_split"_a = false,
+           py::keep_alive<0, 1>())
       .def("cancel", &PyServer::cancel, "request"_a)
       .def("stats", &PyServer::stats);
   m.attr("REQ_QUEUED") = SBH_REQ_QUEUED;

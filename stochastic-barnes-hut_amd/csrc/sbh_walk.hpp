@@ -187,6 +187,57 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   return {acc, cnt, cnt_nodes};
 }
 
+// One task of the per-query Barnes-Hut walk split over the root's children
+// (for launches too small to fill the GPU): the subtree of the root's child c
+// for this lane's query.  A query whose opening test accepts the root takes
+// the root's moment in task 0 and nothing else, as the whole walk would.
+// .value is the task's part of the sum; the parts are added in child order
+// (sum_bh_parts).  Fast terms only.
+template <int D, int K, bool COUNT>
+__device__ __forceinline__ LaneResult bh_walk_subtree(const TreeView& t, float beta, bool live,
+                                                      float x, float y, float z, uint32_t c) {
+  const WalkRecord root = t.dfsR[0];
+  // the root's opening test, decided exactly as the whole walk decides it
+  const float diam = __uint_as_float(root.l.x);
+  int far = opens_far_fast<D>(root.c, __builtin_amdgcn_rcpf(diam), beta, x, y, z);
+  if (far < 0) far = (int)opens_far_exact<D>(root.c, diam, beta, x, y, z);
+  const bool take_root = far != 0;
+  // depth-first position of child c: follow the sibling ropes
+  uint32_t v0 = 1;
+  for (uint32_t j = 0; j < c; ++j) v0 = uniform_u32(t.dfsR[v0].l.z);
+  const uint32_t vstop = uniform_u32(t.dfsR[v0].l.z);
+  float acc = 0.0f;
+  if (c == 0 && take_root)
+    acc = accum<D, K, false>(acc, root.c, t.dfsN[0], t.alpha, t.alpha_k, x, y, z);
+  const LaneResult r =
+      bh_walk<D, K, 1, false, COUNT>(t, beta, live && !take_root, x, y, z, v0, vstop);
+  if (take_root) return {acc, 0, c == 0 ? 1 : 0};
+  return r;
+}
+
+// The parts of a split walk for launch slot `slot`, added in child order:
+// partial[k * nq + slot] is child k's part.
+__device__ __forceinline__ float sum_bh_parts(const float* __restrict__ partial, uint32_t nsplit,
+                                              uint32_t nq, uint32_t slot) {
+  float s = partial[slot];
+  for (uint32_t k = 1; k < nsplit; ++k) s += partial[(size_t)k * nq + slot];
+  return s;
+}
+
+// The parts of a split estimate ({Barnes-Hut part, control-variate part} per
+// subdomain), each summed in subdomain order, then added.
+__device__ __forceinline__ float sum_stochastic_parts(const float2* __restrict__ partial,
+                                                      uint32_t nsplit, uint32_t nq,
+                                                      uint32_t slot) {
+  float bh = 0.0f, mlcv = 0.0f;
+  for (uint32_t k = 0; k < nsplit; ++k) {
+    const float2 p = partial[(size_t)k * nq + slot];
+    bh += p.x;
+    mlcv += p.y;
+  }
+  return bh + mlcv;
+}
+
 // ---------------------------------------------------------------------------
 // Stochastic control-variate estimator (include/tree_sampler.h:21-113,
 // include/tree_sampler_gpu.h:25-140).  The 64 queries of a wavefront share

@@ -16,7 +16,12 @@
 // A request's values depend only on its own queries and parameters: the walks
 // are per query (Barnes-Hut without votes) or keyed on (seed, block of the
 // request's sorted order) (stochastic), so a result is identical alone and in
-// any mix, and equal to a one-shot sorted evaluation of the request.
+// any mix, and equal to a one-shot sorted evaluation of the request.  Like a
+// one-shot launch, a request too small to fill the GPU on its own has its
+// walks split over the root's children (one task per 64-query block and
+// child, the parts added in child order by a reduce pass after the
+// work-queue kernel): a small request on a large tree then spreads over the
+// whole GPU instead of a few wavefronts walking serially.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +46,14 @@ namespace sbh {
 namespace {
 
 constexpr int kServeThreads = 256;
+// Tasks (64-query blocks) a wavefront of the work-queue kernel takes at most.
+constexpr uint32_t kTasksPerWave = 1;
+// A request of at least this many queries is large: it runs alone in its
+// launch, and large launches leave one launch slot to the other requests.
+// (Classing by measured time per query of earlier launches on the same tree
+// and method instead admitted fewer requests per launch: lower throughput at
+// the same latency.)
+constexpr uint32_t kLargeQueries = 65536;
 
 // Device descriptor of one admitted request.
 struct ReqDesc {
@@ -55,35 +68,60 @@ struct ReqDesc {
   float beta;
   int32_t spl;
   uint32_t key0, key1;
-  int32_t sorted;  // nodes have at most 64 children: the sorted-sample estimator
+  int32_t sorted;   // nodes have at most 64 children: the sorted-sample estimator
+  uint32_t nsplit;  // > 1: the walks are split over the root's nsplit children
+  void* partial;    // split requests: nsplit x nq parts (float, or float2 for the estimator)
 };
+
+constexpr uint32_t kWhole = 0xFFFFFFFFu;
 
 struct Task {
   uint32_t req;    // index into the launch's descriptor array
   uint32_t block;  // 64-query block within the request
+  uint32_t sub;    // root child of a split request, or kWhole
 };
 
+// One task for one lane: the whole walk (result to out[qi]) or one root
+// child's part of a split walk (to the request's partial sums), with the
+// kernels and the split rule a one-shot sorted evaluation of the request
+// uses, so a served result equals that evaluation's bit for bit.
 template <int D, int K>
-__device__ __forceinline__ float serve_one(const ReqDesc& r, bool stochastic, uint32_t block,
-                                           bool live, float x, float y, float z,
-                                           SamplerLds lds, SortedLds sorted_lds) {
-  // the same estimator as a one-shot sorted evaluation of the request picks,
-  // so a served result equals that one-shot result
+__device__ __forceinline__ void serve_one(const ReqDesc& r, bool stochastic, uint32_t block,
+                                          uint32_t sub, bool live, uint32_t slot, uint32_t qi,
+                                          float x, float y, float z, SamplerLds lds,
+                                          SortedLds sorted_lds) {
+  if (sub != kWhole) {
+    const size_t e = (size_t)sub * r.nq + slot;
+    if (stochastic) {
+      float bh = 0.0f;
+      const LaneResult p = stochastic_estimate_sorted<D, K, false, true>(
+          r.tree, r.spl, r.key0, r.key1, block, live, x, y, z, sorted_lds, sub, sub + 1, &bh);
+      if (live) static_cast<float2*>(r.partial)[e] = make_float2(bh, p.value);
+    } else {
+      const LaneResult p = bh_walk_subtree<D, K, false>(r.tree, r.beta, live, x, y, z, sub);
+      if (live) static_cast<float*>(r.partial)[e] = p.value;
+    }
+    return;
+  }
+  float v;
   if (stochastic && uniform_u32((uint32_t)r.sorted))
-    return stochastic_estimate_sorted<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x,
-                                                   y, z, sorted_lds)
-        .value;
-  if (stochastic)
-    return stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
-                                            lds, false)
-        .value;
-  return bh_walk<D, K, 1, false, false>(r.tree, r.beta, live, x, y, z).value;
+    v = stochastic_estimate_sorted<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y,
+                                                z, sorted_lds)
+            .value;
+  else if (stochastic)
+    v = stochastic_estimate<D, K, false>(r.tree, r.spl, r.key0, r.key1, block, live, x, y, z,
+                                         lds, false)
+            .value;
+  else
+    v = bh_walk<D, K, 1, false, false>(r.tree, r.beta, live, x, y, z).value;
+  if (live) r.out[qi] = v;
 }
 
-__global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_serve(const ReqDesc* __restrict__ reqs,
+__global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_serve(const ReqDesc* __restrict__ reqs,
                                                           const Task* __restrict__ tasks,
                                                           uint32_t ntasks,
-                                                          uint32_t* __restrict__ next_task) {
+                                                          uint32_t* __restrict__ next_task,
+                                                          uint32_t per_wave) {
   __shared__ float4 sSrc[kServeThreads];
   __shared__ float4 sNrm[kServeThreads];
   __shared__ uint4 sRec[kServeThreads];
@@ -95,7 +133,9 @@ __global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(5
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
-  while (true) {
+  // at most per_wave tasks a wavefront: blocks retire as they finish, so a
+  // launch on another stream gets CUs without waiting for this one to drain
+  for (uint32_t taken = 0; taken < per_wave; ++taken) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(next_task, 1u);
     t = uniform_u32(__shfl(t, 0, 64));
@@ -114,28 +154,42 @@ __global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(5
     const uint32_t wave = threadIdx.x >> 6;
     const SortedLds sc{sSrc + wbase, sNrm + wbase, sRec2 + wbase, sKeys + wave * kChunk,
                        sDiff + wave * kCacheLevels * 64, sNext + wave * (kCacheLevels + 1) * 3};
-    float v;
+    const uint32_t sub = uniform_u32(tk.sub);
     switch (variant >> 1) {
       case 0:
-        v = serve_one<2, GRAVITY>(r, stoch, tk.block, live, x, y, z, c, sc);
+        serve_one<2, GRAVITY>(r, stoch, tk.block, sub, live, slot, qi, x, y, z, c, sc);
         break;
       case 1:
-        v = serve_one<2, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c, sc);
+        serve_one<2, WINDING_NUMBER>(r, stoch, tk.block, sub, live, slot, qi, x, y, z, c, sc);
         break;
       case 2:
-        v = serve_one<2, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c, sc);
+        serve_one<2, SMOOTH_DIST>(r, stoch, tk.block, sub, live, slot, qi, x, y, z, c, sc);
         break;
       case 3:
-        v = serve_one<3, GRAVITY>(r, stoch, tk.block, live, x, y, z, c, sc);
+        serve_one<3, GRAVITY>(r, stoch, tk.block, sub, live, slot, qi, x, y, z, c, sc);
         break;
       case 4:
-        v = serve_one<3, WINDING_NUMBER>(r, stoch, tk.block, live, x, y, z, c, sc);
+        serve_one<3, WINDING_NUMBER>(r, stoch, tk.block, sub, live, slot, qi, x, y, z, c, sc);
         break;
       default:
-        v = serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, live, x, y, z, c, sc);
+        serve_one<3, SMOOTH_DIST>(r, stoch, tk.block, sub, live, slot, qi, x, y, z, c, sc);
         break;
     }
-    if (live) r.out[qi] = v;
+  }
+}
+
+// The split requests of a launch (blockIdx.y = index into split_reqs): their
+// parts added in child order, as a one-shot split launch adds them.
+__global__ void k_serve_reduce(const ReqDesc* __restrict__ reqs,
+                               const uint32_t* __restrict__ split_reqs) {
+  const ReqDesc& r = reqs[split_reqs[blockIdx.y]];
+  const bool stoch = r.variant & 1;
+  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < r.nq;
+       slot += gridDim.x * blockDim.x) {
+    const float v =
+        stoch ? sum_stochastic_parts(static_cast<const float2*>(r.partial), r.nsplit, r.nq, slot)
+              : sum_bh_parts(static_cast<const float*>(r.partial), r.nsplit, r.nq, slot);
+    r.out[r.order[slot]] = v;
   }
 }
 
@@ -144,9 +198,12 @@ __global__ void k_reset_counter(uint32_t* p) { *p = 0; }
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Device bytes of a request in the arena: queries and results (floats), the
-// launch order (u32), and the scratch of the sort that makes it.
-size_t request_bytes(uint32_t nq, int dim) {
-  return align256((size_t)nq * (dim + 2) * 4) + align256(sort_scratch_u32(nq) * 4);
+// launch order (u32), the scratch of the sort that makes it, and a split
+// request's parts (nsplit x nq floats, float2 for the estimator).
+size_t request_bytes(uint32_t nq, int dim, uint32_t nsplit, bool stochastic) {
+  const size_t parts = nsplit > 1 ? (size_t)nsplit * nq * (stochastic ? 8 : 4) : 0;
+  return align256((size_t)nq * (dim + 2) * 4) + align256(sort_scratch_u32(nq) * 4) +
+         align256(parts);
 }
 
 double now_s() {
@@ -215,6 +272,7 @@ struct Server::Request {
   float* results = nullptr;        // host
   uint32_t nq = 0;
   int dim = 3;
+  uint32_t nsplit = 1;  // > 1: walks split over the root's children
   size_t bytes = 0, off = 0;
   int state = SBH_REQ_QUEUED;
   double t_submit = 0, t_done = 0;
@@ -224,6 +282,7 @@ struct Server::Launch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool large = false;  // runs one large request alone
   std::vector<Request*> reqs;
   std::vector<size_t> res_off;  // offset of each request's results in the staging buffer
   float* stage = nullptr;       // pinned host staging: queries then results
@@ -231,8 +290,10 @@ struct Server::Launch {
   DevBuf<ReqDesc> descs;
   DevBuf<Task> tasks;
   DevBuf<uint32_t> counter;
+  DevBuf<uint32_t> split_reqs;
   std::vector<ReqDesc> h_descs;
   std::vector<Task> h_tasks;
+  std::vector<uint32_t> h_split;
 };
 
 Server::Server(size_t budget_bytes, int max_inflight) {
@@ -289,11 +350,11 @@ int64_t Server::submit(int32_t tree_id, int32_t method, int32_t kernel, const sb
   SBH_REQUIRE(q >= 1 && q < (int64_t)0x7FFFFFFF, "query count out of range");
   SBH_REQUIRE(queries && results, "NULL queries or results");
   SBH_REQUIRE(!stop_, "the server is shutting down");
-  // the engine always evaluates in Hilbert order with the whole-walk kernels
-  // and fast terms; any other mode is refused rather than ignored
+  // the engine always evaluates in Hilbert order with fast terms; it honours
+  // SBH_EVAL_NO_SPLIT, and refuses any other mode rather than ignoring it
   SBH_REQUIRE((p.flags & ~(SBH_EVAL_SORT_QUERIES | SBH_EVAL_NO_SPLIT)) == 0,
-              "serving supports no eval flags besides SBH_EVAL_SORT_QUERIES and "
-              "SBH_EVAL_NO_SPLIT (what it always does)");
+              "serving supports no eval flags besides SBH_EVAL_SORT_QUERIES (what it always "
+              "does) and SBH_EVAL_NO_SPLIT");
   if (method == SBH_BARNES_HUT) SBH_REQUIRE(p.beta == p.beta, "beta is NaN");
   if (method == SBH_STOCHASTIC)
     SBH_REQUIRE(p.samples_per_subdomain >= 1, "samples_per_subdomain must be >= 1");
@@ -306,8 +367,14 @@ int64_t Server::submit(int32_t tree_id, int32_t method, int32_t kernel, const sb
   r->queries = queries;
   r->results = results;
   r->nq = (uint32_t)q;
-  r->dim = trees_[tree_id]->dim;
-  r->bytes = request_bytes(r->nq, r->dim);
+  const DeviceTree& t = *trees_[tree_id];
+  r->dim = t.dim;
+  // the split rule of a one-shot launch of the same size (fast terms, per
+  // query, shared paths)
+  r->nsplit = (p.flags & SBH_EVAL_NO_SPLIT) ? 1u
+              : method == SBH_STOCHASTIC   ? stochastic_split_count(t, r->nq, false, false)
+                                           : bh_split_count(t, r->nq, 1, false, false);
+  r->bytes = request_bytes(r->nq, r->dim, r->nsplit, method == SBH_STOCHASTIC);
   if (r->bytes > arena_->size())
     throw Error(kErrBudget, "request working set (" + std::to_string(r->bytes) +
                                 " bytes) exceeds the server's HBM budget");
@@ -406,17 +473,35 @@ void Server::loop() {
           break;
         }
       if (free_slot) {
+        // In arrival order: small requests join the batch while memory
+        // allows; a large request runs alone, and large launches take at most
+        // all but one slot, so small requests never queue behind a launch
+        // that holds the GPU for long.
+        int large_busy = 0;
+        for (auto& L : launches_) large_busy += (L->busy && L->large) ? 1 : 0;
+        const int large_cap = std::max(1, (int)launches_.size() - 1);
         std::vector<Request*> batch;
-        while (!queue_.empty()) {
-          Request* r = queue_.front();
+        bool large = false;
+        for (auto it = queue_.begin(); it != queue_.end();) {
+          Request* r = *it;
+          const bool is_large = r->nq >= kLargeQueries;
+          if (is_large && (!batch.empty() || large_busy >= large_cap)) {
+            ++it;  // runs later, alone
+            continue;
+          }
           size_t off;
           if (!arena_->alloc(r->bytes, &off)) break;  // head of line waits for memory
           r->off = off;
           r->state = SBH_REQ_RUNNING;
-          queue_.pop_front();
+          it = queue_.erase(it);
           batch.push_back(r);
+          if (is_large) {
+            large = true;
+            break;
+          }
         }
         if (!batch.empty()) {
+          free_slot->large = large;
           try {
             launch(*free_slot, batch);
             ++nlaunches_;
@@ -470,6 +555,7 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
   hipStream_t s = L.stream;
   L.h_descs.clear();
   L.h_tasks.clear();
+  L.h_split.clear();
   L.res_off.clear();
   size_t need = 0;
   for (Request* r : batch) need += (size_t)r->nq * (r->dim + 1);
@@ -491,6 +577,8 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     uint32_t* dorder = reinterpret_cast<uint32_t*>(dout + r->nq);
     uint32_t* dscratch =
         reinterpret_cast<uint32_t*>(base + r->off + align256((size_t)r->nq * (r->dim + 2) * 4));
+    void* dparts = base + r->off + align256((size_t)r->nq * (r->dim + 2) * 4) +
+                   align256(sort_scratch_u32(r->nq) * 4);
     const size_t qf = (size_t)r->nq * r->dim;
     std::memcpy(L.stage + so, r->queries, qf * sizeof(float));
     SBH_HIP(hipMemcpyAsync(dq, L.stage + so, qf * sizeof(float), hipMemcpyHostToDevice, s));
@@ -512,9 +600,17 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     d.key0 = (uint32_t)r->params.seed;
     d.key1 = (uint32_t)(r->params.seed >> 32);
     d.sorted = t.dim * t.split_log <= 6 ? 1 : 0;  // at most 2^6 children per node
+    d.nsplit = r->nsplit;
+    d.partial = r->nsplit > 1 ? dparts : nullptr;
     L.h_descs.push_back(d);
     const uint32_t nb = (r->nq + 63) / 64;
-    for (uint32_t b = 0; b < nb; ++b) L.h_tasks.push_back(Task{(uint32_t)i, b});
+    if (r->nsplit > 1) {
+      L.h_split.push_back((uint32_t)i);
+      for (uint32_t b = 0; b < nb; ++b)
+        for (uint32_t c = 0; c < r->nsplit; ++c) L.h_tasks.push_back(Task{(uint32_t)i, b, c});
+    } else {
+      for (uint32_t b = 0; b < nb; ++b) L.h_tasks.push_back(Task{(uint32_t)i, b, kWhole});
+    }
   }
   // largest requests first: their blocks start early and the tail is short
   std::stable_sort(L.h_tasks.begin(), L.h_tasks.end(), [&](const Task& a, const Task& b) {
@@ -529,9 +625,22 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
   k_reset_counter<<<1, 1, 0, s>>>(L.counter.get());
   const uint32_t ntasks = (uint32_t)L.h_tasks.size();
   const uint32_t waves_per_block = kServeThreads / 64;
-  const uint32_t grid = std::min<uint32_t>((ntasks + waves_per_block - 1) / waves_per_block, 2048);
-  k_serve<<<grid, kServeThreads, 0, s>>>(L.descs.get(), L.tasks.get(), ntasks, L.counter.get());
+  const uint32_t per_block = waves_per_block * kTasksPerWave;
+  const uint32_t grid = (ntasks + per_block - 1) / per_block;
+  k_serve<<<grid, kServeThreads, 0, s>>>(L.descs.get(), L.tasks.get(), ntasks, L.counter.get(),
+                                         kTasksPerWave);
   SBH_HIP(hipGetLastError());
+  if (!L.h_split.empty()) {
+    if (L.split_reqs.size() < L.h_split.size()) L.split_reqs.alloc(L.h_split.size() * 2);
+    SBH_HIP(hipMemcpyAsync(L.split_reqs.get(), L.h_split.data(),
+                           L.h_split.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    uint32_t max_nq = 0;
+    for (uint32_t i : L.h_split) max_nq = std::max(max_nq, L.h_descs[i].nq);
+    const dim3 rgrid(std::min<uint32_t>((max_nq + 255) / 256, 1024),
+                     (uint32_t)L.h_split.size());
+    k_serve_reduce<<<rgrid, 256, 0, s>>>(L.descs.get(), L.split_reqs.get());
+    SBH_HIP(hipGetLastError());
+  }
   for (size_t i = 0; i < batch.size(); ++i) {
     Request* r = batch[i];
     SBH_HIP(hipMemcpyAsync(L.stage + L.res_off[i], L.h_descs[i].out,

@@ -50,23 +50,24 @@ def make_requests(trees, count, seed, max_q=4000):
         method = "stochastic" if rng.random() < 0.5 else "barnes_hut"
         reqs.append(dict(ti=ti, kernel=kern, method=method, queries=pts,
                          beta=float(rng.choice([1.5, 2.0, 3.0])),
-                         spl=int(rng.integers(1, 4)), seed=int(rng.integers(1 << 40))))
+                         spl=int(rng.integers(1, 4)), seed=int(rng.integers(1 << 40)),
+                         no_split=bool(rng.random() < 0.25)))
     return reqs
 
 
-def one_shot(sbh_gpu, trees, r):
-    """The request as one sorted launch of the whole-walk kernels, with term
-    counts: (values, counts)."""
+def one_shot(sbh_gpu, trees, r, no_split=False):
+    """The request as one sorted launch (split over the root's children below
+    2^18 queries unless no_split), with term counts: (values, counts)."""
     kern, tree, *_ = trees[r["ti"]]
     if r["method"] == "barnes_hut":
         fn = getattr(sbh_gpu, "eval_barnes_hut_" + kern)
         v, _, c = fn(tree, r["queries"], r["beta"], vote_group=1, sort_queries=True,
-                     no_split=True, return_counts=True)
+                     no_split=no_split, return_counts=True)
         return v, c
     fn = getattr(sbh_gpu, "multi_level_prefix_control_variate_" + kern)
     # the engine evaluates each request in its queries' spatial order
     v, _, c = fn(tree, r["queries"], r["spl"], seed=r["seed"], sort_queries=True,
-                 no_split=True, return_counts=True)
+                 no_split=no_split, return_counts=True)
     return v, c
 
 
@@ -96,7 +97,8 @@ def test_results_identical_alone_and_in_mix(sbh_gpu, trees):
     handles = []
     for r in reqs:
         handles.append(srv.submit(ids[r["ti"]], r["method"], r["kernel"], r["queries"],
-                                  beta=r["beta"], samples_per_subdomain=r["spl"], seed=r["seed"]))
+                                  beta=r["beta"], samples_per_subdomain=r["spl"], seed=r["seed"],
+                                  no_split=r["no_split"]))
     solo = sbh_gpu.Server(64 << 20, 1)
     solo_ids = [solo.add_tree(t[1]) for t in trees]
     for i, (r, h) in enumerate(zip(reqs, handles)):
@@ -104,10 +106,11 @@ def test_results_identical_alone_and_in_mix(sbh_gpu, trees):
         assert lat > 0
         # the same request served alone: identical bits
         a = solo.submit(solo_ids[r["ti"]], r["method"], r["kernel"], r["queries"],
-                        beta=r["beta"], samples_per_subdomain=r["spl"], seed=r["seed"])
+                        beta=r["beta"], samples_per_subdomain=r["spl"], seed=r["seed"],
+                        no_split=r["no_split"])
         np.testing.assert_array_equal(got, a.result(60.0)[0])
-        # the one-shot sorted launch of the whole-walk kernels: identical bits
-        ref, cnt = one_shot(sbh_gpu, trees, r)
+        # the one-shot sorted launch (split the same way): identical bits
+        ref, cnt = one_shot(sbh_gpu, trees, r, no_split=r["no_split"])
         np.testing.assert_array_equal(got, ref)
         # and the oracle
         check_against_oracle(trees, r, got, cnt, f"request {i} {r['method']} "
@@ -127,9 +130,18 @@ def test_admission_by_budget_and_cancel(sbh_gpu, trees):
     # one request's working set, as the engine accounts it
     probe = sbh_gpu.Server(64 << 20, 1)
     probe.add_tree(tree)
+    probe.submit(0, "stochastic", kern, pts, samples_per_subdomain=1, no_split=True).wait(60.0)
+    whole = probe.stats()["peak_bytes"]
+    assert q * (dim + 1) * 4 < whole < q * (dim + 12) * 4
+    del probe
+    # split over the root's children, a request also holds its parts: a
+    # {Barnes-Hut, control-variate} float pair per query and child
+    probe = sbh_gpu.Server(64 << 20, 1)
+    probe.add_tree(tree)
     probe.submit(0, "stochastic", kern, pts, samples_per_subdomain=1).wait(60.0)
     per_request = probe.stats()["peak_bytes"]
-    assert q * (dim + 1) * 4 < per_request < q * (dim + 12) * 4
+    nsub = int(tree.nodes()["nchild"][0])
+    assert 0 <= per_request - whole - nsub * q * 8 < 512
     del probe
     srv = sbh_gpu.Server(2 * per_request, 1)
     tid = srv.add_tree(tree)
@@ -142,7 +154,7 @@ def test_admission_by_budget_and_cancel(sbh_gpu, trees):
     st = srv.stats()
     assert st["peak_bytes"] <= 2 * per_request
     with pytest.raises(RuntimeError, match="budget"):
-        srv.submit(tid, "barnes_hut", kern, lo + (hi - lo) * rng.random((3 * q, dim)))
+        srv.submit(tid, "stochastic", kern, lo + (hi - lo) * rng.random((3 * q, dim)))
 
 
 def test_invalid_requests(sbh_gpu, trees):
@@ -171,7 +183,7 @@ def test_request_keeps_server_alive(sbh_gpu, trees):
     gc.collect()
     vals, lat = r.result(60.0)
     ref, _ = getattr(sbh_gpu, "eval_barnes_hut_" + kern)(tree, pts, 2.0, vote_group=1,
-                                                          sort_queries=True, no_split=True)
+                                                          sort_queries=True)
     np.testing.assert_array_equal(vals, ref)
 
 
