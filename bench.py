@@ -112,6 +112,23 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
         return int(nch[0])
 
     nsub, nsub1 = root_children(t2, i2), root_children(t1, i1)
+
+    def build_roofline(t, info):
+        """Algorithmic bytes of a build: at every level a point's record is read
+        to find its cell (16 B) and its index and cell digit written (4 + 2 B),
+        for each internal node it lies in; each node's records are written
+        once (~96 B)."""
+        nn = info["num_nodes"]
+        st, en, nch = (np.empty(nn, np.uint32) for _ in range(3))
+        C.check(C.lib().sbh_tree_nodes(t.h, st.ctypes.data, en.ctypes.data, None,
+                                       nch.ctypes.data, *([None] * 10)))
+        inner = nch > 0
+        point_levels = int((en[inner].astype(np.int64) - st[inner]).sum())
+        nbytes = 22 * point_levels + 96 * nn
+        return {"build_ms": t.build_ms, "point_levels": point_levels, "bytes": nbytes,
+                "gb_per_s": nbytes / (t.build_ms / 1e3) / 1e9,
+                "hbm_frac": nbytes / (t.build_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                "note": "wall time of sbh_tree_create incl. host input packing and copies"}
     log(f"[{wl['name']}] octree1 {i1['num_nodes']} nodes depth {i1['depth']} "
         f"{t1.build_ms:.1f} ms; octree2 {i2['num_nodes']} nodes depth {i2['depth']} "
         f"{t2.build_ms:.1f} ms; {nsub} subdomains (k=2), {nsub1} (k=1)")
@@ -160,6 +177,7 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
     res = {"workload": wl["desc"], "n_sources": len(wl["pts"]), "n_queries": nq,
            "subdomains": nsub, "err_queries": ne, "fp64_reference_ms": ref_ms,
            "build_ms": {"octree1": t1.build_ms, "octree2": t2.build_ms},
+           "build": {"octree1": build_roofline(t1, i1), "octree2": build_roofline(t2, i2)},
            "trees": {"octree1": i1, "octree2": i2}, "barnes_hut": [], "stochastic": []}
 
     _, kms = run(t1, C.BRUTE_FORCE, args.brute_steps, 1)
@@ -309,7 +327,7 @@ def main():
     ap.add_argument("--no-sort", action="store_true",
                     help="profiling mode: evaluate the stochastic estimator in input order")
     ap.add_argument("--profile-kernel",
-                    choices=["stochastic", "bh", "brute", "mfma", "double", "serve"],
+                    choices=["stochastic", "bh", "brute", "mfma", "double", "serve", "build"],
                     default=None,
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
@@ -594,6 +612,13 @@ def profile(wl, args):
         os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
         with open(args.detail, "w") as f:
             json.dump(r, f, indent=1, default=float)
+        return
+    if args.profile_kernel == "build":
+        # the headline tree (octree k = --tree-k) built --steps times
+        for _ in range(args.steps):
+            t = C.Tree(wl["pts"], None, wl["m"], wl["lo"], wl["hi"], split_log=args.tree_k)
+            log(f"build k={args.tree_k}: {t.build_ms:.2f} ms, {t.info()['num_nodes']} nodes")
+            t.close()
         return
     if args.profile_kernel == "double":
         # the double build: Barnes-Hut at --theta and the estimator at --spl[0],

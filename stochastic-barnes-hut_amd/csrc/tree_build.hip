@@ -276,7 +276,6 @@ struct LargeCtx {
   uint32_t* cellstart;   // [nlarge * B]
   uint32_t* mcount;      // [nlarge]
   uint2* ccnt;           // [nchunks]
-  uint2* coff;           // [nchunks]
   const uint32_t* l_chunk0;  // first chunk of each large node
   const uint32_t* l_nchunk;
   uint16_t* digit;       // [N] position-indexed
@@ -303,6 +302,17 @@ __global__ __launch_bounds__(kChunkThreads) void k_large_digits(LargeCtx cx, con
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < B; i += kChunkThreads)
     if (lh[i]) atomicAdd(&cx.hist[ch.x * B + i], lh[i]);
+}
+
+// The level's large nodes with their point ranges: out = [nodes | starts | ends].
+__global__ void k_gather_large(const uint32_t* large_list, uint32_t nlarge,
+                               const uint32_t* n_start, const uint32_t* n_end, uint32_t* out) {
+  const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nlarge) return;
+  const uint32_t v = large_list[li];
+  out[li] = v;
+  out[nlarge + li] = n_start[v];
+  out[2 * nlarge + li] = n_end[v];
 }
 
 // Per large node: child ranges from the histogram (children of a partitioned
@@ -350,34 +360,27 @@ __global__ __launch_bounds__(kChunkThreads) void k_large_count(LargeCtx cx, uint
   if (threadIdx.x == 0) cx.ccnt[blockIdx.x] = make_uint2(tot & 0xFFFFu, tot >> 16);
 }
 
-// One workgroup per large node: exclusive scans of its chunks' left-false and
-// right-true counts, in chunk (position) order.
-__global__ __launch_bounds__(kChunkThreads) void k_large_offsets(LargeCtx cx) {
-  __shared__ uint32_t wsum[kChunkThreads / 64];
-  const uint32_t li = blockIdx.x;
-  const uint32_t c0 = cx.l_chunk0[li], nc = cx.l_nchunk[li];
-  uint32_t carry_l = 0, carry_r = 0;
-  for (uint32_t b = 0; b < nc; b += kChunkThreads) {
-    const uint32_t j = b + threadIdx.x;
-    const uint2 cc = j < nc ? cx.ccnt[c0 + j] : make_uint2(0u, 0u);
-    uint32_t tl, tr;
-    const uint32_t pl = block_exclusive_scan<kChunkThreads>(cc.x, wsum, tl);
-    const uint32_t pr = block_exclusive_scan<kChunkThreads>(cc.y, wsum, tr);
-    if (j < nc) cx.coff[c0 + j] = make_uint2(carry_l + pl, carry_r + pr);
-    carry_l += tl;
-    carry_r += tr;
-  }
-  if (threadIdx.x == 0) cx.mcount[li] = carry_l;
-}
-
-// Chunk of kChunk positions, 4 consecutive positions per thread.
+// Chunk of kChunk positions, 4 consecutive positions per thread.  The
+// chunk's offsets (left-false and right-true counts of the node's earlier
+// chunks) and the node's left-false total m come from the chunk counts: each
+// block adds them up itself (a node has at most a few thousand chunks).
 __global__ __launch_bounds__(kChunkThreads) void k_large_rank(LargeCtx cx, uint32_t B, uint32_t c) {
   __shared__ uint32_t wsum[kChunkThreads / 64];
   const uint4 ch = cx.chunks[blockIdx.x];
   if (cx.hist[ch.x * B + c] == 0) return;  // uniform per block
   const uint32_t s = cx.cellstart[ch.x * B + c];
-  const uint32_t m = cx.mcount[ch.x];
-  const uint2 off = cx.coff[blockIdx.x];
+  const uint32_t c0 = cx.l_chunk0[ch.x], nc = cx.l_nchunk[ch.x], me = blockIdx.x - c0;
+  uint32_t bl = 0, br = 0, ml = 0;
+  for (uint32_t j = threadIdx.x; j < nc; j += kChunkThreads) {
+    const uint2 cc = cx.ccnt[c0 + j];
+    bl += j < me ? cc.x : 0u;
+    br += j < me ? cc.y : 0u;
+    ml += cc.x;
+  }
+  const uint2 off = make_uint2(block_sum<kChunkThreads>(bl, wsum),
+                               block_sum<kChunkThreads>(br, wsum));
+  const uint32_t m = block_sum<kChunkThreads>(ml, wsum);
+  if (me == 0 && threadIdx.x == 0) cx.mcount[ch.x] = m;  // for the swap pass
   uint32_t p0 = ch.y + threadIdx.x * 4;
   bool lf[4], rt[4];
   uint32_t nl = 0, nr = 0;
@@ -915,7 +918,7 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
 
   t.level_begin.clear();
   uint32_t lb = 0, lcount = 1, num_nodes = 1;
-  std::vector<uint32_t> h_large, h_lstart, h_lend, h_chunk0, h_nchunk;
+  std::vector<uint32_t> h_large, h_lstart, h_lend, h_chunk0, h_nchunk, h_hist;
   std::vector<uint4> h_chunks;
   while (lcount > 0) {
     t.level_begin.push_back(lb);
@@ -929,18 +932,17 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
     const uint32_t nsmall = hc[0], nlarge = hc[1];
 
     if (nlarge > 0) {
-      h_large.resize(nlarge);
-      d2h(h_large.data(), large_list.get(), nlarge, s);
-      h_lstart.resize(nlarge);
-      h_lend.resize(nlarge);
-      // Few large nodes per level: fetch their ranges one by one.
-      for (uint32_t li = 0; li < nlarge; ++li) {
-        SBH_HIP(hipMemcpyAsync(&h_lstart[li], t.n_start.get() + h_large[li], 4,
-                               hipMemcpyDeviceToHost, s));
-        SBH_HIP(hipMemcpyAsync(&h_lend[li], t.n_end.get() + h_large[li], 4,
-                               hipMemcpyDeviceToHost, s));
-      }
-      SBH_HIP(hipStreamSynchronize(s));
+      // the large nodes and their point ranges, gathered into one copy
+      DevBuf<uint32_t> d_lrange((size_t)nlarge * 3);
+      k_gather_large<<<blocks_for(nlarge, 64), 64, 0, s>>>(large_list.get(), nlarge,
+                                                          t.n_start.get(), t.n_end.get(),
+                                                          d_lrange.get());
+      SBH_HIP(hipGetLastError());
+      std::vector<uint32_t> h_lrange((size_t)nlarge * 3);
+      d2h(h_lrange.data(), d_lrange.get(), (size_t)nlarge * 3, s);
+      h_large.assign(h_lrange.begin(), h_lrange.begin() + nlarge);
+      h_lstart.assign(h_lrange.begin() + nlarge, h_lrange.begin() + 2 * nlarge);
+      h_lend.assign(h_lrange.begin() + 2 * nlarge, h_lrange.end());
       h_chunks.clear();
       h_chunk0.resize(nlarge);
       h_nchunk.resize(nlarge);
@@ -955,7 +957,7 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
       DevBuf<uint32_t> d_lstart(nlarge), d_lend(nlarge), d_chunk0(nlarge), d_nchunk(nlarge);
       DevBuf<uint32_t> d_hist((size_t)nlarge * B), d_cellstart((size_t)nlarge * B),
           d_m(nlarge);
-      DevBuf<uint2> d_ccnt(nchunks), d_coff(nchunks);
+      DevBuf<uint2> d_ccnt(nchunks);
       h2d(d_chunks.get(), h_chunks.data(), nchunks, s);
       h2d(d_lstart.get(), h_lstart.data(), nlarge, s);
       h2d(d_lend.get(), h_lend.data(), nlarge, s);
@@ -970,7 +972,6 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
       cx.cellstart = d_cellstart.get();
       cx.mcount = d_m.get();
       cx.ccnt = d_ccnt.get();
-      cx.coff = d_coff.get();
       cx.l_chunk0 = d_chunk0.get();
       cx.l_nchunk = d_nchunk.get();
       cx.digit = digit.get();
@@ -982,9 +983,22 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
       k_large_cells<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, large_list.get(), nlarge, B,
                                                          t.n_nchild.get(), temp_child.get());
       SBH_HIP(hipGetLastError());
+      // The pass of cell c moves a node's points of cell c to the front of
+      // its unsorted remainder.  It changes nothing in a node whose cell c is
+      // empty or is its last non-empty cell (the remainder then holds only
+      // cell c), so only passes some large node needs are launched.
+      h_hist.resize((size_t)nlarge * B);
+      d2h(h_hist.data(), d_hist.get(), (size_t)nlarge * B, s);
+      std::vector<uint8_t> need(B, 0);
+      for (uint32_t li = 0; li < nlarge; ++li) {
+        const uint32_t* h = h_hist.data() + (size_t)li * B;
+        int last = (int)B - 1;
+        while (last > 0 && h[last] == 0) --last;
+        for (int c = 0; c < last; ++c) need[c] |= h[c] != 0;
+      }
       for (uint32_t c = 0; c < B; ++c) {
+        if (!need[c]) continue;
         k_large_count<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c);
-        k_large_offsets<<<nlarge, kChunkThreads, 0, s>>>(cx);
         k_large_rank<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c);
         k_large_swap<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c, t.perm.get());
       }

@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# PyTorch ships its own HIP / HSA runtime.  Imported before libsbh_hip.so is
+# loaded, it makes the library bind to that same HSA runtime; in the other
+# order the process holds two runtimes and torch finds no GPU.  Tests that
+# hand torch device buffers to the library rely on this order.
+try:
+    import torch  # noqa: F401
+except Exception:  # torch is optional for the CPU tests
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "stochastic-barnes-hut_amd")
 for p in (ROOT, PKG):
