@@ -327,11 +327,14 @@ def main():
     ap.add_argument("--no-sort", action="store_true",
                     help="profiling mode: evaluate the stochastic estimator in input order")
     ap.add_argument("--profile-kernel",
-                    choices=["stochastic", "bh", "brute", "mfma", "double", "serve", "build"],
+                    choices=["stochastic", "bh", "brute", "mfma", "double", "serve", "build",
+                             "precision"],
                     default=None,
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
     ap.add_argument("--theta", type=float, default=1.0)
+    ap.add_argument("--kernel", choices=["gravity", "smoothdist"], default="gravity",
+                    help="profiling mode precision: the kernel of the 2^18 uniform-cube case")
     ap.add_argument("--detail", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
                     help="where the full record (sweeps, variants, serving, CPU legs) is written")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -612,6 +615,21 @@ def profile(wl, args):
         os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
         with open(args.detail, "w") as f:
             json.dump(r, f, indent=1, default=float)
+        return
+    if args.profile_kernel == "precision":
+        # the precision case: 2^18 x 2^18 uniform cube, brute force direct,
+        # with fp32 and with bf16 matrix-core operands, --steps times each
+        pw = make_workload("cube3d_" + args.kernel, 1 << 18, 1 << 18)
+        t = C.Tree(pw["pts"], None, pw["m"], pw["lo"], pw["hi"], alpha=pw["alpha"])
+        q32 = pw["queries"].astype(np.float32)
+        qd = torch.from_numpy(np.ascontiguousarray(q32.T)).to(dev)
+        out = torch.empty(len(q32), dtype=torch.float32, device=dev)
+        for label, flags in (("direct", 0), ("mfma", C.EVAL_MFMA), ("mfma_bf16", C.EVAL_MFMA_BF16)):
+            for _ in range(args.steps):
+                ms = t.eval_device(C.BRUTE_FORCE, args.kernel, qd.data_ptr(), len(q32),
+                                   out.data_ptr(), flags=flags)
+                log(f"precision {args.kernel} {label}: {ms:.3f} ms")
+        torch.cuda.synchronize()
         return
     if args.profile_kernel == "build":
         # the headline tree (octree k = --tree-k) built --steps times
