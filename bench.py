@@ -335,8 +335,10 @@ def main():
     ap.add_argument("--theta", type=float, default=1.0)
     ap.add_argument("--kernel", choices=["gravity", "smoothdist"], default="gravity",
                     help="profiling mode precision: the kernel of the 2^18 uniform-cube case")
-    ap.add_argument("--detail", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
-                    help="where the full record (sweeps, variants, serving, CPU legs) is written")
+    ap.add_argument("--detail", default=None,
+                    help="where the full record (sweeps, variants, serving, CPU legs) is written "
+                         "(default gpurun_out/bench_detail.json; profiling modes write none "
+                         "unless given)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="with --profile-kernel: also run a plain read+write kernel over this "
                          "many MiB (a counter calibration kernel of known traffic)")
@@ -356,6 +358,8 @@ def main():
     if args.profile_kernel:
         profile(wl, args)
         return
+    if args.detail is None:
+        args.detail = os.path.join(ROOT, "gpurun_out", "bench_detail.json")
 
     head = measure(wl, args, (0.3, 0.4, 0.5, 0.6, 0.7, 1.0), args.spl,
                    headline_steps=args.steps, warmup=args.warmup, want_counts=True)
@@ -612,9 +616,10 @@ def profile(wl, args):
         r = serving_bench(args)
         log(f"serve: burst {r['burst']['queries_per_s'] / 1e6:.1f} M q/s, poisson p50 "
             f"{r['poisson']['p50_ms']:.1f} ms, p99 {r['poisson']['p99_ms']:.1f} ms")
-        os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
-        with open(args.detail, "w") as f:
-            json.dump(r, f, indent=1, default=float)
+        if args.detail:
+            os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+            with open(args.detail, "w") as f:
+                json.dump(r, f, indent=1, default=float)
         return
     if args.profile_kernel == "precision":
         # the precision case: 2^18 x 2^18 uniform cube, brute force direct,
