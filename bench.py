@@ -320,6 +320,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-variants", action="store_true", help="skip the variant workloads")
     ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-walk-k", type=int, choices=[1, 2], default=1,
+                    help="split_log of the trees serving Barnes-Hut requests (the estimator's "
+                         "trees are k=2)")
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="Poisson offered load as a fraction of the measured burst capacity")
     ap.add_argument("--tree-k", type=int, default=2,
@@ -728,11 +731,21 @@ def serving_bench(args):
     for name, dim, kern, n in SERVE_TREES:
         pts, nrm, m = _serve_sources(name, n)
         lo, hi = pts.min(0) - 0.01, pts.max(0) + 0.01
+        alpha = 10.0 if kern == "smoothdist" else 1.0
         cls = sbh_gpu.OctreeGPU2 if dim == 3 else sbh_gpu.QuadtreeGPU2
-        tr = cls(pts, nrm, m, lo, hi, alpha=10.0 if kern == "smoothdist" else 1.0)
+        tr = cls(pts, nrm, m, lo, hi, alpha=alpha)
         nsub = int(tr.nodes()["nchild"][0])
-        trees.append(dict(name=name, dim=dim, kernel=kern, n=len(pts), lo=lo, hi=hi,
-                          id=srv.add_tree(tr), tree=tr, subdomains=nsub))
+        rec = dict(name=name, dim=dim, kernel=kern, n=len(pts), lo=lo, hi=hi,
+                   id=srv.add_tree(tr), tree=tr, subdomains=nsub)
+        # Barnes-Hut walks a 2^d-ary tree of the same sources (as the 1M-source
+        # measurements do), the estimator samples the 4^d-ary one
+        rec["walk_tree"] = tr
+        rec["walk_id"] = rec["id"]
+        if args.serve_walk_k == 1:
+            cls1 = sbh_gpu.OctreeGPU1 if dim == 3 else sbh_gpu.QuadtreeGPU1
+            rec["walk_tree"] = cls1(pts, nrm, m, lo, hi, alpha=alpha)
+            rec["walk_id"] = srv.add_tree(rec["walk_tree"])
+        trees.append(rec)
         log(f"serve tree {name}: {len(pts)} points, {nsub} subdomains, build {tr.build_ms:.1f} ms")
     nreq = args.serve_requests
     reqs = []
@@ -752,14 +765,16 @@ def serving_bench(args):
             delay = t0 + at - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            hs.append(srv.submit(t["id"], method, t["kernel"], pts, beta=2.0,
-                                 samples_per_subdomain=spl, seed=seed))
+            hs.append(srv.submit(t["walk_id"] if method == "barnes_hut" else t["id"], method,
+                                 t["kernel"], pts, beta=2.0, samples_per_subdomain=spl,
+                                 seed=seed))
         lats = [h.result(600.0)[1] for h in hs]
         el = time.perf_counter() - t0
         return np.asarray(lats), el
 
-    for h in [srv.submit(t["id"], m_, t["kernel"], p[:2000], beta=2.0, samples_per_subdomain=s_,
-                         seed=1) for (t, m_, p, s_, _) in reqs[:16]]:
+    for h in [srv.submit(t["walk_id"] if m_ == "barnes_hut" else t["id"], m_, t["kernel"],
+                         p[:2000], beta=2.0, samples_per_subdomain=s_, seed=1)
+              for (t, m_, p, s_, _) in reqs[:16]]:
         h.result(600.0)
     lat_b, el_b = run(np.zeros(nreq))
     cap = total_q / el_b
@@ -774,8 +789,8 @@ def serving_bench(args):
     tail = []
     for i in np.argsort(lat_p)[::-1][:6]:
         t, method, pts, spl, seed = reqs[i]
-        h = srv.submit(t["id"], method, t["kernel"], pts, beta=2.0, samples_per_subdomain=spl,
-                       seed=seed)
+        h = srv.submit(t["walk_id"] if method == "barnes_hut" else t["id"], method, t["kernel"],
+                       pts, beta=2.0, samples_per_subdomain=spl, seed=seed)
         alone = h.result(600.0)[1]
         tail.append({"latency_ms": float(lat_p[i]), "alone_ms": alone, "queries": len(pts),
                      "method": method, "tree": t["name"]})
@@ -789,6 +804,7 @@ def serving_bench(args):
     import gc
 
     tree_info = [{k: t[k] for k in ("name", "dim", "kernel", "n", "subdomains")} for t in trees]
+    resident = len({id(t["tree"]) for t in trees} | {id(t["walk_tree"]) for t in trees})
     del srv, trees, reqs
     gc.collect()
     return {
@@ -796,6 +812,8 @@ def serving_bench(args):
         "requests": nreq, "queries": int(total_q),
         "request_sizes": "log-uniform 1e3..1e6 queries",
         "methods": "barnes_hut theta=0.5 / stochastic 128 samples per query, 50/50",
+        "walk_tree_k": args.serve_walk_k,
+        "resident_trees": resident,
         "burst": {"queries_per_s": cap, "elapsed_s": el_b,
                   "p50_ms": float(np.percentile(lat_b, 50)),
                   "p99_ms": float(np.percentile(lat_b, 99))},
