@@ -170,15 +170,18 @@ __global__ __launch_bounds__(kThreads) void k_bh(TreeView t, float beta,
   }
 }
 
-// Barnes-Hut per query for launches too small to fill the GPU: the walk below
-// the root is split over the root's children (blockIdx.y = child), each task
-// walking one subtree for 64 queries, and the partial sums are added in child
-// order afterwards.  A query whose opening test accepts the root takes the
-// root's moment (in the first child's task) and nothing else, as the whole
-// walk would.  Fast terms only: the association of the sum changes.  With
-// COUNT the tasks also count their terms (pcnt: {nodes, points} per task).
+// Barnes-Hut per query for launches too small to fill the GPU: the walk is
+// split over a frontier of disjoint subtrees (blockIdx.y = task, tasks in
+// depth-first order), each task walking one subtree for 64 queries, and the
+// partial sums are added in task order afterwards.  A query whose opening
+// test accepts an ancestor of a subtree takes that ancestor's moment (in the
+// first task below it) and nothing else, as the whole walk would.  Fast terms
+// only: the association of the sum changes.  With COUNT the tasks also count
+// their terms (pcnt: {nodes, points} per task).
 template <int D, int K, bool COUNT>
-__global__ __launch_bounds__(kThreads) void k_bh_split(TreeView t, float beta,
+__global__ __launch_bounds__(kThreads) void k_bh_split(TreeView t,
+                                                        const WalkTask* __restrict__ tasks,
+                                                        float beta,
                                                         const float* __restrict__ qx,
                                                         const float* __restrict__ qy,
                                                         const float* __restrict__ qz, uint32_t nq,
@@ -190,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_bh_split(TreeView t, float beta,
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const LaneResult r = bh_walk_subtree<D, K, COUNT>(t, beta, live, x, y, z, blockIdx.y);
+  const LaneResult r = bh_walk_task<D, K, COUNT>(t, tasks, blockIdx.y, beta, live, x, y, z);
   if (live) {
     const size_t e = (size_t)blockIdx.y * nq + slot;
     partial[e] = r.value;
@@ -445,23 +448,174 @@ __device__ __forceinline__ uint32_t grid_coord(float v, float lo, float hi, doub
   return (uint32_t)t;
 }
 
-__global__ void k_curve_keys(const float* qx, const float* qy, const float* qz, uint32_t nq,
-                             int dim, float4 lo, float4 hi, uint64_t* keys, uint32_t* idx) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  uint64_t key;
+__device__ __forceinline__ uint64_t curve_key(const float* qx, const float* qy, const float* qz,
+                                              uint32_t i, int dim, float4 lo, float4 hi) {
   if (dim == 3) {
     const double cells = 2097152.0;
     uint32_t X[3] = {grid_coord(qx[i], lo.x, hi.x, cells), grid_coord(qy[i], lo.y, hi.y, cells),
                      grid_coord(qz[i], lo.z, hi.z, cells)};
-    key = hilbert_key<3>(X, 21);
-  } else {
-    const double cells = 2147483648.0;
-    uint32_t X[2] = {grid_coord(qx[i], lo.x, hi.x, cells), grid_coord(qy[i], lo.y, hi.y, cells)};
-    key = hilbert_key<2>(X, 31);
+    return hilbert_key<3>(X, 21);
   }
-  keys[i] = key;
+  const double cells = 2147483648.0;
+  uint32_t X[2] = {grid_coord(qx[i], lo.x, hi.x, cells), grid_coord(qy[i], lo.y, hi.y, cells)};
+  return hilbert_key<2>(X, 31);
+}
+
+__global__ void k_curve_keys(const float* qx, const float* qy, const float* qz, uint32_t nq,
+                             int dim, float4 lo, float4 hi, uint64_t* keys, uint32_t* idx) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  keys[i] = curve_key(qx, qy, qz, i, dim, lo, hi);
   idx[i] = i;
+}
+
+// The launch order is the queries sorted by (curve key, input index): a
+// total order, so every correct sort yields the same permutation (the LSD
+// radix passes below are stable, and the keys start in input order).
+//
+// Up to kFusedSortMax queries, where launch latency dominates, the sort takes
+// ten launches: the histograms are zeroed, the keys computed (one per thread,
+// over the whole GPU) with their first digit's per-tile histogram, then eight
+// digit passes.  A pass's workgroup derives its tile's output offsets from
+// the pass's per-tile histograms (every digit's total before it plus the same
+// digit in earlier tiles), ranks its elements stably (per wavefront, chunk by
+// chunk, with running digit counts in LDS), scatters them, and adds each
+// element's next digit to the next pass's histogram of the tile it lands in.
+constexpr int kFrThreads = 1024;
+constexpr int kFrItems = 4;
+constexpr uint32_t kFrTile = kFrThreads * kFrItems;
+constexpr uint32_t kFusedSortMax = 1u << 18;
+constexpr int kFrChunks = kFrItems;  // 64-element chunks per wavefront
+
+__global__ void k_zero_u32n(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// Keys and input indices, and the first pass's histograms hist[0][digit][tile].
+__global__ __launch_bounds__(256) void k_fr_keys(const float* qx, const float* qy,
+                                                 const float* qz, uint32_t nq, int dim,
+                                                 float4 lo, float4 hi, uint64_t* keys,
+                                                 uint32_t* idx, uint32_t* hist0, uint32_t nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nq) {
+    const uint64_t key = curve_key(qx, qy, qz, i, dim, lo, hi);
+    keys[i] = key;
+    idx[i] = i;
+    atomicAdd(&h[(uint32_t)key & 0xFFu], 1u);
+  }
+  __syncthreads();
+  // a 256-key block lies in one tile
+  const uint32_t c = h[threadIdx.x];
+  if (c) atomicAdd(&hist0[threadIdx.x * nb + blockIdx.x * 256 / kFrTile], c);
+}
+
+__global__ __launch_bounds__(kFrThreads) void k_fr_pass(const uint64_t* __restrict__ kin,
+                                                        const uint32_t* __restrict__ vin,
+                                                        uint64_t* __restrict__ kout,
+                                                        uint32_t* __restrict__ vout, uint32_t n,
+                                                        int shift, uint32_t nb,
+                                                        const uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ hist_next) {
+  __shared__ uint32_t wcnt[kFrThreads / 64][256];  // running, then exclusive, per-wave counts
+  __shared__ uint32_t boff[256];
+  __shared__ uint32_t scan[256];
+  __shared__ uint32_t hn[256 * (kFusedSortMax / kFrTile)];  // next pass: [digit][tile]
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t b = blockIdx.x;
+  // the tile: chunks of 64 consecutive elements, wavefront w owning chunks
+  // [w kFrChunks, (w + 1) kFrChunks) in order (loaded first, so that their
+  // latency overlaps the offsets below)
+  uint64_t key[kFrItems];
+  uint32_t val[kFrItems], rank[kFrItems];
+#pragma unroll
+  for (int m = 0; m < kFrItems; ++m) {
+    const uint32_t i = b * kFrTile + (wave * kFrChunks + m) * 64 + lane;
+    const bool ok = i < n;
+    key[m] = ok ? kin[i] : ~0ull;
+    val[m] = ok ? vin[i] : 0u;
+  }
+  // this tile's output start for digit tid, from the pass's histograms
+  // (staged in hn with one round of coalesced loads)
+  for (uint32_t e = tid; e < 256 * nb; e += kFrThreads) hn[e] = hist[e];
+  for (uint32_t e = tid; e < (kFrThreads / 64) * 256; e += kFrThreads) (&wcnt[0][0])[e] = 0;
+  __syncthreads();
+  const bool digit_thread = tid < 256;  // threads 0..255 own a digit each
+  if (digit_thread) {
+    uint32_t tot = 0, pre = 0;
+    for (uint32_t j = 0; j < nb; ++j) {
+      const uint32_t c = hn[tid * nb + j];
+      tot += c;
+      pre += j < b ? c : 0u;
+    }
+    scan[tid] = tot;
+    boff[tid] = pre;
+  }
+  __syncthreads();
+  if (hist_next)
+    for (uint32_t e = tid; e < 256 * nb; e += kFrThreads) hn[e] = 0;
+  // exclusive scan of the digit totals (Hillis-Steele over 256 entries)
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = digit_thread && tid >= off ? scan[tid - off] : 0u;
+    __syncthreads();
+    if (digit_thread) scan[tid] += v;
+    __syncthreads();
+  }
+  if (digit_thread) boff[tid] += tid ? scan[tid - 1] : 0u;  // totals of the smaller digits
+  // stable ranks: a lane's rank among the chunk's lanes holding its digit
+  // (eight ballots), after the wavefront's earlier chunks (running counts)
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int m = 0; m < kFrItems; ++m) {
+    const uint32_t i = b * kFrTile + (wave * kFrChunks + m) * 64 + lane;
+    const bool ok = i < n;
+    const uint32_t d = (uint32_t)(key[m] >> shift) & 0xFFu;
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t set = __ballot(on);
+      peers &= on ? set : ~set;
+    }
+    const uint32_t below = (uint32_t)__popcll(peers & lt);
+    const uint32_t before = wcnt[wave][d];
+    rank[m] = before + below;
+    if (ok && below == 0) wcnt[wave][d] = before + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  if (digit_thread) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kFrThreads / 64; ++w) {
+      const uint32_t c = wcnt[w][tid];
+      wcnt[w][tid] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < kFrItems; ++m) {
+    const uint32_t i = b * kFrTile + (wave * kFrChunks + m) * 64 + lane;
+    if (i >= n) continue;
+    const uint32_t d = (uint32_t)(key[m] >> shift) & 0xFFu;
+    const uint32_t pos = boff[d] + wcnt[wave][d] + rank[m];
+    kout[pos] = key[m];
+    vout[pos] = val[m];
+    if (hist_next) {
+      const uint32_t dn = (uint32_t)(key[m] >> (shift + 8)) & 0xFFu;
+      atomicAdd(&hn[dn * nb + pos / kFrTile], 1u);
+    }
+  }
+  if (hist_next) {
+    __syncthreads();
+    for (uint32_t e = tid; e < 256 * nb; e += kFrThreads) {
+      const uint32_t c = hn[e];
+      if (c) atomicAdd(&hist_next[e], c);
+    }
+  }
 }
 
 // LSD radix sort of (key, index) pairs, 8 bits per pass: per-block digit
@@ -577,18 +731,19 @@ void launch_brute_f64(const DeviceTree& t, const float* qx, const float* qy, con
 template <int D, int K, bool EXACT>
 void launch_bh(const DeviceTree& t, float beta, int group, const float* qx, const float* qy,
                const float* qz, uint32_t nq, const uint32_t* order, float* out, int32_t* counts,
-               int split, float* partial, uint32_t nsplit, hipStream_t s) {
+               int split, float* partial, WalkSplit ws, hipStream_t s) {
   uint32_t grid = nblk(nq, kThreads);
   const TreeView tv = view_of(t);
+  const uint32_t nsplit = ws.n;
   if (!EXACT && nsplit > 1) {
     // partial: nsplit * nq floats, then (with counts) nsplit * nq int2
     int2* pcnt = reinterpret_cast<int2*>(partial + (((size_t)nsplit * nq + 1) & ~(size_t)1));
     if (counts)
-      k_bh_split<D, K, true><<<dim3(grid, nsplit), kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq,
-                                                                      order, partial, pcnt);
+      k_bh_split<D, K, true><<<dim3(grid, nsplit), kThreads, 0, s>>>(
+          tv, ws.tasks, beta, qx, qy, qz, nq, order, partial, pcnt);
     else
-      k_bh_split<D, K, false><<<dim3(grid, nsplit), kThreads, 0, s>>>(tv, beta, qx, qy, qz, nq,
-                                                                       order, partial, pcnt);
+      k_bh_split<D, K, false><<<dim3(grid, nsplit), kThreads, 0, s>>>(
+          tv, ws.tasks, beta, qx, qy, qz, nq, order, partial, pcnt);
     SBH_HIP(hipGetLastError());
     k_reduce_bh_split<<<nblk(nq, 256), 256, 0, s>>>(partial, pcnt, nsplit, nq, order, out,
                                                      counts, split);
@@ -745,14 +900,27 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
   });
 }
 
-uint32_t bh_split_count(const DeviceTree& t, uint32_t nq, int group, bool exact, bool counts) {
+WalkSplit bh_split(const DeviceTree& t, uint32_t nq, int group, bool exact) {
   // fewer than four wavefronts per SIMD (1024 SIMDs): split the walk over the
-  // root's children, if there are between 2 and 64 of them
-  (void)counts;  // counted launches split the same way (the values must not depend on it)
-  if (exact || group != 1 || t.is_double || t.level_begin.size() < 3) return 1;
-  const uint32_t nsub = t.level_begin[2] - t.level_begin[1];
-  if (nsub < 2 || nsub > 64 || nq >= 4u * 1024u * 64u) return 1;
-  return nsub;
+  // shallowest frontier that makes kSplitWaves wavefronts, else the deepest
+  // one (at most kMaxWalkTasks subtrees).  Counted launches split the same way
+  // (the values must not depend on it).
+  constexpr uint32_t kSplitWaves = 16u * 1024u;
+  WalkSplit ws;
+  if (exact || group != 1 || t.is_double || t.walk_frontiers.empty() || nq >= 4u * 1024u * 64u)
+    return ws;
+  const uint32_t waves = nblk(nq, kWave);
+  for (const auto& f : t.walk_frontiers) {
+    ws.n = f.second;
+    ws.tasks = t.walk_tasks.get() + f.first;
+    if ((size_t)f.second * waves >= kSplitWaves) break;
+  }
+  return ws;
+}
+
+uint32_t bh_split_count(const DeviceTree& t, uint32_t nq, int group, bool exact, bool counts) {
+  (void)counts;
+  return bh_split(t, nq, group, exact).n;
 }
 
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
@@ -761,14 +929,14 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
-  const uint32_t ns = partial ? bh_split_count(t, nq, group, exact, counts != nullptr) : 1;
+  const WalkSplit ws = partial ? bh_split(t, nq, group, exact) : WalkSplit{};
   dispatch(t.dim, kernel, [&]<int D, int K>() {
     if (exact)
       launch_bh<D, K, true>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, partial,
-                            1, s);
+                            WalkSplit{}, s);
     else
       launch_bh<D, K, false>(t, beta, group, qx, qy, qz, nq, order, out, counts, split, partial,
-                             ns, s);
+                             ws, s);
   });
 }
 
@@ -825,9 +993,13 @@ void run_smooth_distance(float* w, uint32_t n, float alpha, hipStream_t s) {
 }
 
 size_t sort_scratch_u32(uint32_t nq) {
-  uint32_t nb = nblk(nq, kSortTile);
-  // two 64-bit key buffers, one value buffer, histograms, offsets, scan scratch
-  return 4ull * nq + (size_t)nq + 2ull * 256 * nb + exclusive_scan_scratch(256u * nb) + 8;
+  // two 64-bit key buffers, one value buffer, then histograms and offsets
+  // and scan scratch (radix passes) or the per-tile histograms of the eight
+  // passes (fused passes)
+  const uint32_t nb = nblk(nq, kSortTile);
+  const size_t radix = 2ull * 256 * nb + exclusive_scan_scratch(256u * nb);
+  const size_t fused = nq <= kFusedSortMax ? 8ull * 256 * nblk(nq, kFrTile) : 0;
+  return 4ull * nq + (size_t)nq + std::max(radix, fused) + 8;
 }
 
 void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* order,
@@ -844,6 +1016,28 @@ void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* ord
   uint32_t* scan_scratch = offs + 256ull * nb;
   float4 lo = make_float4(t.lo[0], t.lo[1], t.lo[2], 0.f);
   float4 hi = make_float4(t.hi[0], t.hi[1], t.dim == 3 ? t.hi[2] : 1.f, 0.f);
+  if (nq <= kFusedSortMax) {
+    const uint32_t fb = nblk(nq, kFrTile);
+    uint32_t* fh = v1 + nq;  // hist[pass][digit][tile]
+    const uint32_t nh = 8u * 256u * fb;
+    k_zero_u32n<<<std::min(nblk(nh, 256), 256u), 256, 0, s>>>(fh, nh);
+    SBH_HIP(hipGetLastError());
+    k_fr_keys<<<nblk(nq, 256), 256, 0, s>>>(qx, qy, qz, nq, t.dim, lo, hi, k0, order, fh, fb);
+    SBH_HIP(hipGetLastError());
+    uint64_t* kin = k0;
+    uint32_t* vin = order;
+    uint64_t* kout = k1;
+    uint32_t* vout = v1;
+    for (int pss = 0; pss < 8; ++pss) {
+      const uint32_t* h = fh + (size_t)pss * 256 * fb;
+      uint32_t* hn = pss < 7 ? fh + (size_t)(pss + 1) * 256 * fb : nullptr;
+      k_fr_pass<<<fb, kFrThreads, 0, s>>>(kin, vin, kout, vout, nq, 8 * pss, fb, h, hn);
+      SBH_HIP(hipGetLastError());
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+    return;  // eight passes: the sorted indices are in `order`
+  }
   k_curve_keys<<<nblk(nq, 256), 256, 0, s>>>(qx, qy, qz, nq, t.dim, lo, hi, k0, order);
   SBH_HIP(hipGetLastError());
   const int passes = 8;  // 64-bit keys, 8 bits per pass

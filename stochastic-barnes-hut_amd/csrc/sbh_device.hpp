@@ -21,6 +21,18 @@ struct alignas(32) WalkRecord {
   uint4 l;
 };
 
+// One task of a Barnes-Hut walk split over a frontier of disjoint subtrees
+// that covers the tree: the subtree at depth-first positions [v0, vstop) and
+// the depth-first positions of its nanc ancestors, root first.  Bit d of
+// `first` is set when this is the first task (in depth-first order) below
+// anc[d]: a query that accepts anc[d] takes that moment there.
+constexpr int kMaxWalkAnc = 4;
+constexpr uint32_t kMaxWalkTasks = 64;
+struct alignas(32) WalkTask {
+  uint32_t v0, vstop, nanc, first;
+  uint32_t anc[kMaxWalkAnc];
+};
+
 // include/kernel.h:24-28 (bound kernels) and :61-83 (unbound: Helmholtz,
 // and the gravity gradient as one scalar kernel per axis)
 enum Kernel : int {

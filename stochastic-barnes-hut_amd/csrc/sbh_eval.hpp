@@ -35,7 +35,13 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
 // rounded fp32 operations in the reference's order (otherwise hardware
 // sqrt / reciprocal / exp2).
 // partial (optional): bh_split_count(...) * nq floats; launches too small to
-// fill the GPU then split the per-query fast walk over the root's subtrees.
+// fill the GPU then split the per-query fast walk over a frontier of subtrees
+// (bh_split: the task count and the frontier's task table).
+struct WalkSplit {
+  uint32_t n = 1;
+  const WalkTask* tasks = nullptr;
+};
+WalkSplit bh_split(const DeviceTree& t, uint32_t nq, int group, bool exact);
 uint32_t bh_split_count(const DeviceTree& t, uint32_t nq, int group, bool exact, bool counts);
 void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float* q, uint32_t nq,
             const uint32_t* order, float* out, int32_t* counts, int split, bool exact,

@@ -150,6 +150,12 @@ struct DeviceTree {
   // and {range start, range end, single-child chain target, 0} (derived).
   DevBuf<WalkRecord> bfsR;
   DevBuf<uint4> stepR;
+  // Split frontiers of the Barnes-Hut walk (derived), shallowest first: the
+  // frontier at depth L holds the nodes at depth L and the leaves above them
+  // (2 to kMaxWalkTasks subtrees), as the tasks
+  // walk_tasks[walk_frontiers[f].first, + .second).
+  DevBuf<WalkTask> walk_tasks;
+  std::vector<std::pair<uint32_t, uint32_t>> walk_frontiers;
 
   // Double build (the reference's USE_DOUBLE configuration): points, cell
   // bounds, diameters and moments in float64 instead of the float arrays
@@ -175,9 +181,10 @@ struct DeviceTree {
            n_depth.bytes() + n_lo.bytes() + n_hi.bytes() + n_diam.bytes() + nodeC.bytes() +
            nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
            n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
-           kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes() + pts64.bytes() +
-           nrm64.bytes() + n_lo64.bytes() + n_hi64.bytes() + n_diam64.bytes() +
-           nodeC64.bytes() + nodeN64.bytes() + dfsR64.bytes() + dfsN64.bytes();
+           kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes() + walk_tasks.bytes() +
+           pts64.bytes() + nrm64.bytes() + n_lo64.bytes() + n_hi64.bytes() +
+           n_diam64.bytes() + nodeC64.bytes() + nodeN64.bytes() + dfsR64.bytes() +
+           dfsN64.bytes();
   }
 };
 
@@ -195,7 +202,10 @@ void build_device_tree_double(DeviceTree& t, const double* points, const double*
 void serialize_tree(const DeviceTree& t, std::vector<uint8_t>& out);
 void deserialize_tree(DeviceTree& t, const uint8_t* data, size_t size);
 // Fills kidC / kidN, dfsR, bfsR and stepR from the node, point and
-// depth-first arrays (after a build or a load).
+// depth-first arrays (after a build or a load), then the walk frontiers.
 void derive_child_sources(DeviceTree& t);
+// The split frontiers of the Barnes-Hut walk (walk_tasks, walk_frontiers),
+// from host copies of the top levels; blocks on t.stream.
+void derive_walk_tasks(DeviceTree& t);
 
 }  // namespace sbh

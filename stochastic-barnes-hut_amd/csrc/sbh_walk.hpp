@@ -187,36 +187,48 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   return {acc, cnt, cnt_nodes};
 }
 
-// One task of the per-query Barnes-Hut walk split over the root's children
-// (for launches too small to fill the GPU): the subtree of the root's child c
-// for this lane's query.  A query whose opening test accepts the root takes
-// the root's moment in task 0 and nothing else, as the whole walk would.
-// .value is the task's part of the sum; the parts are added in child order
-// (sum_bh_parts).  Fast terms only.
+// One task of the per-query Barnes-Hut walk split over a frontier of
+// disjoint subtrees (for launches too small to fill the GPU): task k's
+// subtree for this lane's query.  A query whose opening test accepts an
+// ancestor of the subtree takes the moment of the topmost ancestor it accepts
+// in the first task below that ancestor, and nothing else, as the whole walk
+// would.  .value is the task's part of the sum; the parts are added in task
+// (depth-first) order (sum_bh_parts).  Fast terms only.
 template <int D, int K, bool COUNT>
-__device__ __forceinline__ LaneResult bh_walk_subtree(const TreeView& t, float beta, bool live,
-                                                      float x, float y, float z, uint32_t c) {
-  const WalkRecord root = t.dfsR[0];
-  // the root's opening test, decided exactly as the whole walk decides it
-  const float diam = __uint_as_float(root.l.x);
-  int far = opens_far_fast<D>(root.c, __builtin_amdgcn_rcpf(diam), beta, x, y, z);
-  if (far < 0) far = (int)opens_far_exact<D>(root.c, diam, beta, x, y, z);
-  const bool take_root = far != 0;
-  // depth-first position of child c: follow the sibling ropes
-  uint32_t v0 = 1;
-  for (uint32_t j = 0; j < c; ++j) v0 = uniform_u32(t.dfsR[v0].l.z);
-  const uint32_t vstop = uniform_u32(t.dfsR[v0].l.z);
+__device__ __forceinline__ LaneResult bh_walk_task(const TreeView& t,
+                                                   const WalkTask* __restrict__ tasks,
+                                                   uint32_t k, float beta, bool live, float x,
+                                                   float y, float z) {
+  const WalkTask w = tasks[k];
   float acc = 0.0f;
-  if (c == 0 && take_root)
-    acc = accum<D, K, false>(acc, root.c, t.dfsN[0], t.alpha, t.alpha_k, x, y, z);
+  bool taken = false;
+  int32_t nodes = 0;
+#pragma unroll
+  for (int d = 0; d < kMaxWalkAnc; ++d) {
+    if (d < (int)w.nanc) {
+      const uint32_t a = w.anc[d];
+      const WalkRecord r = t.dfsR[a];
+      // the ancestor's opening test, decided exactly as the whole walk decides it
+      const float diam = __uint_as_float(r.l.x);
+      int far = opens_far_fast<D>(r.c, __builtin_amdgcn_rcpf(diam), beta, x, y, z);
+      if (far < 0) far = (int)opens_far_exact<D>(r.c, diam, beta, x, y, z);
+      const bool take = !taken && far != 0;
+      if (((w.first >> d) & 1u) && __ballot(take)) {
+        const float with = accum<D, K, false>(acc, r.c, t.dfsN[a], t.alpha, t.alpha_k, x, y, z);
+        acc = take ? with : acc;
+        nodes = take ? 1 : nodes;
+      }
+      taken = taken || far != 0;
+    }
+  }
   const LaneResult r =
-      bh_walk<D, K, 1, false, COUNT>(t, beta, live && !take_root, x, y, z, v0, vstop);
-  if (take_root) return {acc, 0, c == 0 ? 1 : 0};
+      bh_walk<D, K, 1, false, COUNT>(t, beta, live && !taken, x, y, z, w.v0, w.vstop);
+  if (taken) return {acc, 0, nodes};
   return r;
 }
 
-// The parts of a split walk for launch slot `slot`, added in child order:
-// partial[k * nq + slot] is child k's part.
+// The parts of a split walk for launch slot `slot`, added in task order:
+// partial[k * nq + slot] is task k's part.
 __device__ __forceinline__ float sum_bh_parts(const float* __restrict__ partial, uint32_t nsplit,
                                               uint32_t nq, uint32_t slot) {
   float s = partial[slot];

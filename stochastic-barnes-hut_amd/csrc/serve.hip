@@ -18,10 +18,11 @@
 // request's sorted order) (stochastic), so a result is identical alone and in
 // any mix, and equal to a one-shot sorted evaluation of the request.  Like a
 // one-shot launch, a request too small to fill the GPU on its own has its
-// walks split over the root's children (one task per 64-query block and
-// child, the parts added in child order by a reduce pass after the
-// work-queue kernel): a small request on a large tree then spreads over the
-// whole GPU instead of a few wavefronts walking serially.
+// walks split (one task per 64-query block and subtree of the walk's split
+// frontier, or root subdomain of the estimator, the parts added in task order
+// by a reduce pass after the work-queue kernel): a small request on a large
+// tree then spreads over the whole GPU instead of a few wavefronts walking
+// serially.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,8 +70,9 @@ struct ReqDesc {
   int32_t spl;
   uint32_t key0, key1;
   int32_t sorted;   // nodes have at most 64 children: the sorted-sample estimator
-  uint32_t nsplit;  // > 1: the walks are split over the root's nsplit children
+  uint32_t nsplit;  // > 1: the walks are split into nsplit tasks
   void* partial;    // split requests: nsplit x nq parts (float, or float2 for the estimator)
+  const WalkTask* wtasks;  // split Barnes-Hut requests: the frontier's task table
 };
 
 constexpr uint32_t kWhole = 0xFFFFFFFFu;
@@ -78,11 +80,12 @@ constexpr uint32_t kWhole = 0xFFFFFFFFu;
 struct Task {
   uint32_t req;    // index into the launch's descriptor array
   uint32_t block;  // 64-query block within the request
-  uint32_t sub;    // root child of a split request, or kWhole
+  uint32_t sub;    // task of a split request (a walk task, or the estimator's
+                   // root subdomain), or kWhole
 };
 
-// One task for one lane: the whole walk (result to out[qi]) or one root
-// child's part of a split walk (to the request's partial sums), with the
+// One task for one lane: the whole walk (result to out[qi]) or one task's
+// part of a split walk (to the request's partial sums), with the
 // kernels and the split rule a one-shot sorted evaluation of the request
 // uses, so a served result equals that evaluation's bit for bit.
 template <int D, int K>
@@ -98,7 +101,8 @@ __device__ __forceinline__ void serve_one(const ReqDesc& r, bool stochastic, uin
           r.tree, r.spl, r.key0, r.key1, block, live, x, y, z, sorted_lds, sub, sub + 1, &bh);
       if (live) static_cast<float2*>(r.partial)[e] = make_float2(bh, p.value);
     } else {
-      const LaneResult p = bh_walk_subtree<D, K, false>(r.tree, r.beta, live, x, y, z, sub);
+      const LaneResult p =
+          bh_walk_task<D, K, false>(r.tree, r.wtasks, sub, r.beta, live, x, y, z);
       if (live) static_cast<float*>(r.partial)[e] = p.value;
     }
     return;
@@ -272,7 +276,7 @@ struct Server::Request {
   float* results = nullptr;        // host
   uint32_t nq = 0;
   int dim = 3;
-  uint32_t nsplit = 1;  // > 1: walks split over the root's children
+  uint32_t nsplit = 1;  // > 1: walks split into nsplit tasks
   size_t bytes = 0, off = 0;
   int state = SBH_REQ_QUEUED;
   double t_submit = 0, t_done = 0;
@@ -602,6 +606,12 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     d.sorted = t.dim * t.split_log <= 6 ? 1 : 0;  // at most 2^6 children per node
     d.nsplit = r->nsplit;
     d.partial = r->nsplit > 1 ? dparts : nullptr;
+    d.wtasks = nullptr;
+    if (r->nsplit > 1 && r->method != SBH_STOCHASTIC) {
+      const WalkSplit ws = bh_split(t, r->nq, 1, false);
+      SBH_REQUIRE(ws.n == r->nsplit, "serving: split plan changed");
+      d.wtasks = ws.tasks;
+    }
     L.h_descs.push_back(d);
     const uint32_t nb = (r->nq + 63) / 64;
     if (r->nsplit > 1) {

@@ -1130,6 +1130,70 @@ void derive_child_sources(DeviceTree& t) {
       t.dfsL.get(), t.n_start.get(), t.n_end.get(), t.n_skip.get(), t.kidC.get(), t.kidN.get(),
       t.dfsR.get(), t.bfsR.get(), t.stepR.get());
   SBH_HIP(hipGetLastError());
+  derive_walk_tasks(t);
+}
+
+void derive_walk_tasks(DeviceTree& t) {
+  // Frontier at depth L: the nodes at depth L and the leaves above it, in
+  // depth-first order; a frontier's size never falls as L grows (every
+  // internal node has a child).  Frontiers of 2 to kMaxWalkTasks subtrees at
+  // depths 1 .. kMaxWalkAnc are kept, shallowest first.
+  t.walk_frontiers.clear();
+  const int levels = (int)t.level_begin.size() - 1;
+  int top = 0;  // deepest level a kept frontier can reach
+  while (top + 1 < levels && top + 1 <= kMaxWalkAnc &&
+         t.level_begin[top + 2] - t.level_begin[top + 1] <= kMaxWalkTasks)
+    ++top;
+  if (top == 0) {
+    t.walk_tasks.alloc(1);
+    return;
+  }
+  const uint32_t m = t.level_begin[top + 1];  // nodes of levels 0 .. top
+  std::vector<uint32_t> nch(m), first(m), dfs(m), size(m);
+  hipStream_t s = t.stream;
+  SBH_HIP(hipMemcpyAsync(nch.data(), t.n_nchild.get(), m * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipMemcpyAsync(first.data(), t.n_first.get(), m * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipMemcpyAsync(dfs.data(), t.n_dfs.get(), m * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipMemcpyAsync(size.data(), t.n_size.get(), m * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipStreamSynchronize(s));
+  std::vector<WalkTask> all;
+  for (int L = 1; L <= top; ++L) {
+    std::vector<WalkTask> f;
+    uint32_t path[kMaxWalkAnc];
+    // depth-first over the levels above L (children in order, so the tasks
+    // come out in depth-first order)
+    auto visit = [&](auto&& self, uint32_t v, int depth) -> void {
+      if (depth == L || nch[v] == 0) {
+        WalkTask w{};
+        w.v0 = dfs[v];
+        const uint32_t rope = dfs[v] + size[v];
+        w.vstop = rope < t.num_nodes ? rope : kEnd;
+        w.nanc = (uint32_t)depth;
+        for (int d = 0; d < depth; ++d) w.anc[d] = dfs[path[d]];
+        f.push_back(w);
+        return;
+      }
+      path[depth] = v;
+      for (uint32_t c = first[v]; c < first[v] + nch[v]; ++c) self(self, c, depth + 1);
+    };
+    visit(visit, 0, 0);
+    if (f.size() > kMaxWalkTasks) break;
+    if (f.size() < 2) continue;
+    for (size_t k = 0; k < f.size(); ++k) {
+      for (uint32_t d = 0; d < f[k].nanc; ++d) {
+        const bool below_prev = k > 0 && f[k - 1].nanc > d && f[k - 1].anc[d] == f[k].anc[d];
+        if (!below_prev) f[k].first |= 1u << d;
+      }
+    }
+    t.walk_frontiers.emplace_back((uint32_t)all.size(), (uint32_t)f.size());
+    all.insert(all.end(), f.begin(), f.end());
+  }
+  t.walk_tasks.alloc(all.size());
+  if (!all.empty()) {
+    SBH_HIP(hipMemcpyAsync(t.walk_tasks.get(), all.data(), all.size() * sizeof(WalkTask),
+                           hipMemcpyHostToDevice, s));
+    SBH_HIP(hipStreamSynchronize(s));
+  }
 }
 
 size_t exclusive_scan_scratch(uint32_t n) { return scan_scratch_size(n); }

@@ -383,34 +383,42 @@ def test_host_tree_constructor(sbh_gpu):
 
 
 @pytest.mark.parametrize("dim", [3, 2])
-def test_query_order_is_a_stable_hilbert_sort(sbh_gpu, dim):
+@pytest.mark.parametrize("n", [1, 2, 2047, 2048, 2049, 6001, 100003, (1 << 20) + 5])
+def test_query_order_is_a_stable_hilbert_sort(sbh_gpu, dim, n):
     """sbh_query_order (the order SBH_EVAL_SORT_QUERIES evaluates in): a
-    permutation, Hilbert keys non-decreasing, equal keys in input order."""
-    import ctypes
-
+    permutation, Hilbert keys non-decreasing, equal keys in input order.
+    Sizes around the sort's 2048-pair tiles and its merge passes, with
+    repeated keys (the same point many times) mixed in."""
     import sbh_capi as C
 
-    rng = np.random.default_rng(40 + dim)
+    rng = np.random.default_rng(40 + dim + n)
     pts = rng.random((5000, dim))
     lo, hi = W.bounds(pts)
-    n = 100003  # not a multiple of the sort tile
-    q = np.concatenate([lo + (hi - lo) * rng.random((n - 3000, dim)),
-                        np.repeat(pts[:30], 100, axis=0)])  # repeated keys
+    rep = min(30, n // 100)
+    q = np.concatenate([lo + (hi - lo) * rng.random((n - 100 * rep, dim)),
+                        np.repeat(pts[:rep], 100, axis=0)])
+    q = q[rng.permutation(n)] if rep else q
     t = C.Tree(pts, None, np.ones(len(pts)), lo, hi)
     soa = np.ascontiguousarray(q.astype(np.float32).T)
     order = np.empty(n, np.uint32)
     C.check(C.lib().sbh_query_order(t.h, soa.ctypes.data, n, order.ctypes.data, 0))
     np.testing.assert_array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
-    keys = hilbert_keys(q, lo, hi)[order]
+    qkeys = hilbert_keys(q, lo, hi)
+    keys = qkeys[order]
     assert (keys[1:] >= keys[:-1]).all()
     same = keys[1:] == keys[:-1]
     assert (order[1:][same] > order[:-1][same]).all()
+    # the order by (key, input index) is unique
+    np.testing.assert_array_equal(order, np.lexsort((np.arange(n), qkeys)))
+    if rep:
+        assert same.sum() >= 99 * rep
 
 
 @pytest.mark.parametrize("dim,k", [(3, 1), (3, 2), (2, 2)])
-def test_small_launches_split_over_root_children(sbh_gpu, dim, k):
-    """Launches too small to fill the GPU split Barnes-Hut and the sorted
-    estimator over the root's children.  Queries far outside the box accept
+def test_small_launches_split(sbh_gpu, dim, k):
+    """Launches too small to fill the GPU split Barnes-Hut over a frontier of
+    subtrees and the sorted estimator over the root's subdomains.  Queries
+    far outside the box accept
     the root itself (its moment, one node term); near queries walk every
     subtree.  Values agree with the oracle to fp32 rounding, counts exactly."""
     pts, nrm, m, q, lo, hi = _cube(3000, 300, dim, 61 + k, "gravity")
@@ -436,3 +444,91 @@ def test_small_launches_split_over_root_children(sbh_gpu, dim, k):
     same = scnt == rcnt
     assert np.count_nonzero(~same) <= max(2, len(same) // 100)
     assert rel_err(st[same], ref[same], "gravity").max() <= TOL
+
+
+def _walk_split_tasks(sbh_gpu, tree, kern, q):
+    """Tasks a Barnes-Hut launch of q queries splits into, from the serving
+    engine's working-set accounting (a split request holds nsplit x q float
+    parts beyond the whole-walk request's working set)."""
+    def peak(no_split):
+        srv = sbh_gpu.Server(256 << 20, 1)
+        srv.add_tree(tree)
+        srv.submit(0, "barnes_hut", kern, q, beta=2.0, no_split=no_split).wait(60.0)
+        b = srv.stats()["peak_bytes"]
+        del srv
+        return b
+
+    extra = peak(False) - peak(True)
+    ns = round(extra / (len(q) * 4))
+    assert abs(extra - ns * len(q) * 4) < 512
+    return max(ns, 1)
+
+
+def _clustered_with_shallow_leaves():
+    """An octree whose depth-1 level holds two leaves (clusters of 5 and 7
+    points at leaf size 8) beside one populated octant: its split frontier
+    has leaves above the frontier's depth."""
+    rng = np.random.default_rng(77)
+    a = 0.05 + 0.1 * rng.random((5, 3))
+    c = np.array([0.85, 0.05, 0.85]) + 0.1 * rng.random((7, 3))
+    b = 0.5 + 0.5 * rng.random((3000, 3))
+    pts = np.concatenate([a, b, c])
+    return pts, np.zeros_like(pts), rng.uniform(0.5, 1.5, len(pts)), np.zeros(3), np.ones(3)
+
+
+def _shell_queries(q, center, size, rng):
+    """Queries at 0.3 to 4 box widths from the box centre in random
+    directions: far ones accept the root, middle ones nodes below it."""
+    d = rng.normal(size=(q, len(center)))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return center + size * rng.uniform(0.3, 4.0, (q, 1)) * d
+
+
+@pytest.mark.parametrize("case", ["octree_k1", "quadtree_k1", "plummer_k1_leaf8",
+                                  "shallow_leaves_leaf8", "octree_k1_150k"])
+def test_barnes_hut_split_frontiers(sbh_gpu, case):
+    """The split Barnes-Hut launch over frontiers below the root's children:
+    the tasks are the nodes at depth L and the leaves above them (up to 64),
+    and a query that accepts an ancestor above the frontier takes that
+    ancestor's moment once.  Counts equal the oracle's per-query walk
+    exactly, values agree to fp32 rounding, with near queries (walks into
+    every task) and shells of queries that accept the root or nodes between
+    the root and the frontier.  The frontier's task count is checked where
+    the tree's shape fixes it."""
+    rng = np.random.default_rng(sum(map(ord, case)))
+    leaf, k, dim, want = 1, 1, 3, None
+    if case == "octree_k1":
+        pts, nrm, m, q, lo, hi = _cube(6000, 700, 3, 5, "gravity")
+        want = 64  # the 64 nodes at depth 2
+    elif case == "quadtree_k1":
+        pts, nrm, m, q, lo, hi = _cube(3000, 700, 2, 6, "gravity")
+        dim, want = 2, 64  # the 64 nodes at depth 3
+    elif case == "plummer_k1_leaf8":
+        pts, nrm, m, q = W.plummer(1 << 13, 700, seed_sources=8, seed_queries=9)
+        lo, hi = W.bounds(pts)
+        leaf = 8
+    elif case == "shallow_leaves_leaf8":
+        pts, nrm, m, lo, hi = _clustered_with_shallow_leaves()
+        q = rng.random((700, 3))
+        leaf, want = 8, 10  # the populated octant's 8 children and the two leaves
+    else:
+        pts, nrm, m, q, lo, hi = _cube(6000, 150000, 3, 7, "gravity")
+        want = 8  # 150000 queries fill the GPU with the root's children already
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    q = np.concatenate([q, _shell_queries(len(q) // 2, (lo + hi) / 2, (hi - lo).max(), rng)])
+    g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, k, leaf_size=leaf)
+    ns = _walk_split_tasks(sbh_gpu, g, "gravity", q)
+    print(f"[frontier] {case}: {ns} tasks for {len(q)} queries", flush=True)
+    assert ns > 1
+    if want is not None:
+        assert ns == want
+    for beta in (1.5, 3.0):
+        got, _, cnt = sbh_gpu.eval_barnes_hut_gravity(g, q, beta, vote_group=1,
+                                                      return_counts=True)
+        ref, rcnt = o.barnes_hut("gravity", q, beta, vote_group=1, counts=True)
+        np.testing.assert_array_equal(cnt, rcnt)
+        assert rel_err(got, ref, "gravity").max() <= TOL, (case, beta)
+        np.testing.assert_array_equal(
+            sbh_gpu.eval_barnes_hut_gravity(g, q, beta, vote_group=1)[0], got)
+        whole, _ = sbh_gpu.eval_barnes_hut_gravity(g, q, beta, vote_group=1, no_split=True)
+        assert rel_err(got, whole, "gravity").max() <= TOL

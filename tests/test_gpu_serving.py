@@ -56,7 +56,7 @@ def make_requests(trees, count, seed, max_q=4000):
 
 
 def one_shot(sbh_gpu, trees, r, no_split=False):
-    """The request as one sorted launch (split over the root's children below
+    """The request as one sorted launch (its walks split into tasks below
     2^18 queries unless no_split), with term counts: (values, counts)."""
     kern, tree, *_ = trees[r["ti"]]
     if r["method"] == "barnes_hut":

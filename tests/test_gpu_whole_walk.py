@@ -2,8 +2,8 @@
 walk and the sorted-sample estimator, with and without term counts) against
 the oracle on the same seeded inputs.
 
-Launches of fewer than 2^18 queries normally split each walk over the root's
-children; `no_split=True` (SBH_EVAL_NO_SPLIT) makes small launches run the
+Launches of fewer than 2^18 queries normally split each walk into tasks;
+`no_split=True` (SBH_EVAL_NO_SPLIT) makes small launches run the
 whole-walk kernels, and one case of 2^18 + 77 queries reaches them by size
 alone.  Bars:
 
