@@ -208,22 +208,45 @@ def measure(wl, args, thetas, spls, headline_steps=0, warmup=0, want_counts=Fals
             + ("" if kern != "smoothdist"
                else f" (distance {res['barnes_hut'][-1]['distance_rel_err']:.2e})"))
     trees = {2: (t2, nsub), 1: (t1, nsub1)}
-    for k, spl in [(2, s) for s in spls] + [(1, s) for s in spls]:
+
+    def stochastic_row(k, spl, sort, refined=False):
         tk, nsk = trees[k]
+        flags = C.EVAL_SORT_QUERIES if sort else 0
+        _, kms = run(tk, C.STOCHASTIC, args.sweep_steps, 1, samples_per_subdomain=spl,
+                     seed=1, flags=flags)
+        ms = float(np.mean(kms))
+        row = {"tree_k": k, "samples_per_subdomain": spl, "samples_per_query": spl * nsk,
+               "sorted": sort, "kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
+               "mean_rel_err": err_of(), "distance_rel_err": dist_err_of()}
+        if refined:
+            row["refined"] = True
+        res["stochastic"].append(row)
+        log(f"  SBH k={k} spl={spl} sort={sort}: {ms:.2f} ms err {row['mean_rel_err']:.2e}"
+            + ("" if kern != "smoothdist" else f" (distance {row['distance_rel_err']:.2e})"))
+        return row
+
+    for k, spl in [(2, s) for s in spls] + [(1, s) for s in spls]:
         for sort in (False, True):
-            flags = C.EVAL_SORT_QUERIES if sort else 0
-            _, kms = run(tk, C.STOCHASTIC, args.sweep_steps, 1, samples_per_subdomain=spl,
-                         seed=1, flags=flags)
-            ms = float(np.mean(kms))
-            res["stochastic"].append({"tree_k": k, "samples_per_subdomain": spl,
-                                      "samples_per_query": spl * nsk, "sorted": sort,
-                                      "kernel_ms": ms, "queries_per_s": nq / (ms / 1e3),
-                                      "mean_rel_err": err_of(),
-                                      "distance_rel_err": dist_err_of()})
-            log(f"  SBH k={k} spl={spl} sort={sort}: {ms:.2f} ms "
-                f"err {res['stochastic'][-1]['mean_rel_err']:.2e}"
-                + ("" if kern != "smoothdist"
-                   else f" (distance {res['stochastic'][-1]['distance_rel_err']:.2e})"))
+            stochastic_row(k, spl, sort)
+    # the grid brackets the target; for the headline workload, bisect the
+    # sample budget of the fastest configuration that meets it (same tree and
+    # query order) between the grid's last budget that misses and its first
+    # that meets the target, to multiples of 32 samples per subdomain
+    crit = "distance_rel_err" if kern == "smoothdist" else "mean_rel_err"
+    ok = [r for r in res["stochastic"] if r[crit] <= TARGET_ERR]
+    if ok and args.refine and headline_steps:
+        best = min(ok, key=lambda r: r["kernel_ms"])
+        k, sort, hi = best["tree_k"], best["sorted"], best["samples_per_subdomain"]
+        below = [r["samples_per_subdomain"] for r in res["stochastic"]
+                 if r["tree_k"] == k and r["sorted"] == sort and r[crit] > TARGET_ERR
+                 and r["samples_per_subdomain"] < hi]
+        lo = max(below) if below else 0
+        while hi - lo > 32:
+            mid = max(lo + 32, (lo + hi) // 64 * 32)
+            if stochastic_row(k, mid, sort, refined=True)[crit] <= TARGET_ERR:
+                hi = mid
+            else:
+                lo = mid
 
     def at_target(rows):
         ok = [r for r in rows if r["mean_rel_err"] <= TARGET_ERR]
@@ -317,6 +340,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU work of the baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-refine", dest="refine", action="store_false",
+                    help="report the best grid configuration without refining its sample "
+                         "budget")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-variants", action="store_true", help="skip the variant workloads")
     ap.add_argument("--serve-requests", type=int, default=600)
