@@ -357,7 +357,7 @@ def main():
                     help="profiling mode: evaluate the stochastic estimator in input order")
     ap.add_argument("--profile-kernel",
                     choices=["stochastic", "bh", "brute", "mfma", "double", "serve", "build",
-                             "precision"],
+                             "precision", "c1"],
                     default=None,
                     help="profiling mode: build the headline trees, run only this method "
                          "--steps times (stochastic at --spl[0], Barnes-Hut at --theta) and exit")
@@ -664,6 +664,30 @@ def profile(wl, args):
                                    out.data_ptr(), flags=flags)
                 log(f"precision {args.kernel} {label}: {ms:.3f} ms")
         torch.cuda.synchronize()
+        return
+    if args.profile_kernel == "c1":
+        # the reference's CPU-runnable case on the GPU: brute force, Barnes-Hut
+        # (beta 2, leaf sizes 8 and 1, per query, sorted) and the estimator
+        # (octree k=2, 4 samples per subdomain, sorted), --steps times each
+        import sbh_gpu
+        import workloads as W
+
+        pts, _, m, q = W.uniform_cube(16384, 16384, seed=0)
+        lo, hi = W.bounds(pts)
+        nrm = np.zeros_like(pts)
+        q32 = q.astype(np.float32)
+        g8 = sbh_gpu.OctreeGPU1(pts, nrm, m, lo, hi, leaf_size=8)
+        g1 = sbh_gpu.OctreeGPU1(pts, nrm, m, lo, hi)
+        g2 = sbh_gpu.OctreeGPU2(pts, nrm, m, lo, hi)
+        for _ in range(args.steps):
+            log(f"c1 brute: {sbh_gpu.eval_brute_force_gravity(g1, q32)[1]:.3f} ms")
+            for name, g in (("leaf 8", g8), ("leaf 1", g1)):
+                _, ms = sbh_gpu.eval_barnes_hut_gravity(g, q32, 2.0, vote_group=1,
+                                                        sort_queries=True)
+                log(f"c1 barnes-hut {name}: {ms:.3f} ms")
+            _, ms = sbh_gpu.multi_level_prefix_control_variate_gravity(g2, q32, 4, seed=1,
+                                                                      sort_queries=True)
+            log(f"c1 stochastic: {ms:.3f} ms")
         return
     if args.profile_kernel == "build":
         # the headline tree (octree k = --tree-k) built --steps times

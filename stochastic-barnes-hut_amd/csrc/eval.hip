@@ -100,6 +100,7 @@ __global__ void k_reduce_splits(const double* partial, uint32_t nsplit, uint32_t
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   double s = 0.0;
+#pragma unroll 8  // loads in flight ahead of the (ordered) additions
   for (uint32_t k = 0; k < nsplit; ++k) s += partial[(size_t)k * nq + i];
   out[i] = (float)s;
 }
@@ -139,6 +140,7 @@ __global__ void k_reduce_splits_f64(const double* partial, uint32_t nsplit, uint
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   double s = 0.0;
+#pragma unroll 8  // loads in flight ahead of the (ordered) additions
   for (uint32_t k = 0; k < nsplit; ++k) s += partial[(size_t)k * nq + i];
   out[i] = s;
 }
@@ -211,6 +213,7 @@ __global__ void k_reduce_bh_split(const float* __restrict__ partial,
   out[qi] = sum_bh_parts(partial, nsplit, nq, slot);
   if (counts) {
     int32_t nodes = 0, points = 0;
+#pragma unroll 8
     for (uint32_t k = 0; k < nsplit; ++k) {
       const int2 c = pcnt[(size_t)k * nq + slot];
       nodes += c.x;
@@ -362,6 +365,7 @@ __global__ void k_reduce_stoch_split(const float2* __restrict__ partial,
   out[qi] = sum_stochastic_parts(partial, nsplit, nq, slot);
   if (counts) {
     int32_t nodes = 0, points = 0;
+#pragma unroll 8
     for (uint32_t k = 0; k < nsplit; ++k) {
       const int2 c = pcnt[(size_t)k * nq + slot];
       nodes += c.x;
@@ -481,11 +485,8 @@ __global__ void k_curve_keys(const float* qx, const float* qy, const float* qz, 
 // digit in earlier tiles), ranks its elements stably (per wavefront, chunk by
 // chunk, with running digit counts in LDS), scatters them, and adds each
 // element's next digit to the next pass's histogram of the tile it lands in.
-constexpr int kFrThreads = 1024;
-constexpr int kFrItems = 4;
-constexpr uint32_t kFrTile = kFrThreads * kFrItems;
+constexpr uint32_t kFrTile = 4096;
 constexpr uint32_t kFusedSortMax = 1u << 18;
-constexpr int kFrChunks = kFrItems;  // 64-element chunks per wavefront
 
 __global__ void k_zero_u32n(uint32_t* p, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -513,64 +514,75 @@ __global__ __launch_bounds__(256) void k_fr_keys(const float* qx, const float* q
   if (c) atomicAdd(&hist0[threadIdx.x * nb + blockIdx.x * 256 / kFrTile], c);
 }
 
-__global__ __launch_bounds__(kFrThreads) void k_fr_pass(const uint64_t* __restrict__ kin,
+// THREADS threads per tile (the order does not depend on it): 1024 for a
+// launch of its own, 256 where the sort shares the GPU with running work
+// (serving), whose workgroups then fit in the CUs' gaps.  Dynamic LDS: the
+// pass's histograms, 1 KB per tile.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_fr_pass(const uint64_t* __restrict__ kin,
                                                         const uint32_t* __restrict__ vin,
                                                         uint64_t* __restrict__ kout,
                                                         uint32_t* __restrict__ vout, uint32_t n,
                                                         int shift, uint32_t nb,
                                                         const uint32_t* __restrict__ hist,
                                                         uint32_t* __restrict__ hist_next) {
-  __shared__ uint32_t wcnt[kFrThreads / 64][256];  // running, then exclusive, per-wave counts
+  constexpr int kItems = (int)kFrTile / THREADS;
+  constexpr int kChunks = kItems;  // 64-element chunks per wavefront
+  static_assert(THREADS >= 256 && kItems * THREADS == (int)kFrTile, "tile shape");
+  __shared__ uint32_t wcnt[THREADS / 64][256];  // running, then exclusive, per-wave counts
   __shared__ uint32_t boff[256];
-  __shared__ uint32_t scan[256];
-  __shared__ uint32_t hn[256 * (kFusedSortMax / kFrTile)];  // next pass: [digit][tile]
+  __shared__ uint32_t scan[4];  // digit totals of wavefronts 0..3
+  extern __shared__ uint32_t hn[];  // the pass's, then the next pass's [digit][tile]
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t b = blockIdx.x;
   // the tile: chunks of 64 consecutive elements, wavefront w owning chunks
-  // [w kFrChunks, (w + 1) kFrChunks) in order (loaded first, so that their
+  // [w kChunks, (w + 1) kChunks) in order (loaded first, so that their
   // latency overlaps the offsets below)
-  uint64_t key[kFrItems];
-  uint32_t val[kFrItems], rank[kFrItems];
+  uint64_t key[kItems];
+  uint32_t val[kItems], rank[kItems];
 #pragma unroll
-  for (int m = 0; m < kFrItems; ++m) {
-    const uint32_t i = b * kFrTile + (wave * kFrChunks + m) * 64 + lane;
+  for (int m = 0; m < kItems; ++m) {
+    const uint32_t i = b * kFrTile + (wave * kChunks + m) * 64 + lane;
     const bool ok = i < n;
     key[m] = ok ? kin[i] : ~0ull;
     val[m] = ok ? vin[i] : 0u;
   }
   // this tile's output start for digit tid, from the pass's histograms
   // (staged in hn with one round of coalesced loads)
-  for (uint32_t e = tid; e < 256 * nb; e += kFrThreads) hn[e] = hist[e];
-  for (uint32_t e = tid; e < (kFrThreads / 64) * 256; e += kFrThreads) (&wcnt[0][0])[e] = 0;
+  for (uint32_t e = tid; e < 256 * nb; e += THREADS) hn[e] = hist[e];
+  for (uint32_t e = tid; e < (THREADS / 64) * 256; e += THREADS) (&wcnt[0][0])[e] = 0;
   __syncthreads();
-  const bool digit_thread = tid < 256;  // threads 0..255 own a digit each
+  const bool digit_thread = tid < 256;  // wavefronts 0..3 own a digit per thread
+  uint32_t tot = 0, pre = 0, incl = 0;
   if (digit_thread) {
-    uint32_t tot = 0, pre = 0;
     for (uint32_t j = 0; j < nb; ++j) {
       const uint32_t c = hn[tid * nb + j];
       tot += c;
       pre += j < b ? c : 0u;
     }
-    scan[tid] = tot;
-    boff[tid] = pre;
+    // inclusive scan of the digit totals within the wavefront
+    incl = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, off, 64);
+      incl += lane >= (uint32_t)off ? v : 0u;
+    }
+    if (lane == 63) scan[wave] = incl;  // the wavefront's total
   }
   __syncthreads();
   if (hist_next)
-    for (uint32_t e = tid; e < 256 * nb; e += kFrThreads) hn[e] = 0;
-  // exclusive scan of the digit totals (Hillis-Steele over 256 entries)
-  for (uint32_t off = 1; off < 256; off <<= 1) {
-    const uint32_t v = digit_thread && tid >= off ? scan[tid - off] : 0u;
-    __syncthreads();
-    if (digit_thread) scan[tid] += v;
-    __syncthreads();
+    for (uint32_t e = tid; e < 256 * nb; e += THREADS) hn[e] = 0;
+  if (digit_thread) {
+    uint32_t before = 0;  // totals of the wavefronts of smaller digits
+    for (uint32_t w = 0; w < wave; ++w) before += scan[w];
+    boff[tid] = pre + before + incl - tot;  // + totals of the smaller digits
   }
-  if (digit_thread) boff[tid] += tid ? scan[tid - 1] : 0u;  // totals of the smaller digits
   // stable ranks: a lane's rank among the chunk's lanes holding its digit
   // (eight ballots), after the wavefront's earlier chunks (running counts)
   const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int m = 0; m < kFrItems; ++m) {
-    const uint32_t i = b * kFrTile + (wave * kFrChunks + m) * 64 + lane;
+  for (int m = 0; m < kItems; ++m) {
+    const uint32_t i = b * kFrTile + (wave * kChunks + m) * 64 + lane;
     const bool ok = i < n;
     const uint32_t d = (uint32_t)(key[m] >> shift) & 0xFFu;
     uint64_t peers = __ballot(ok);
@@ -589,7 +601,7 @@ __global__ __launch_bounds__(kFrThreads) void k_fr_pass(const uint64_t* __restri
   if (digit_thread) {
     uint32_t run = 0;
 #pragma unroll
-    for (int w = 0; w < kFrThreads / 64; ++w) {
+    for (int w = 0; w < THREADS / 64; ++w) {
       const uint32_t c = wcnt[w][tid];
       wcnt[w][tid] = run;
       run += c;
@@ -597,8 +609,8 @@ __global__ __launch_bounds__(kFrThreads) void k_fr_pass(const uint64_t* __restri
   }
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < kFrItems; ++m) {
-    const uint32_t i = b * kFrTile + (wave * kFrChunks + m) * 64 + lane;
+  for (int m = 0; m < kItems; ++m) {
+    const uint32_t i = b * kFrTile + (wave * kChunks + m) * 64 + lane;
     if (i >= n) continue;
     const uint32_t d = (uint32_t)(key[m] >> shift) & 0xFFu;
     const uint32_t pos = boff[d] + wcnt[wave][d] + rank[m];
@@ -611,7 +623,7 @@ __global__ __launch_bounds__(kFrThreads) void k_fr_pass(const uint64_t* __restri
   }
   if (hist_next) {
     __syncthreads();
-    for (uint32_t e = tid; e < 256 * nb; e += kFrThreads) {
+    for (uint32_t e = tid; e < 256 * nb; e += THREADS) {
       const uint32_t c = hn[e];
       if (c) atomicAdd(&hist_next[e], c);
     }
@@ -1003,7 +1015,7 @@ size_t sort_scratch_u32(uint32_t nq) {
 }
 
 void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* order,
-                 uint32_t* scratch, hipStream_t s) {
+                 uint32_t* scratch, hipStream_t s, bool compact) {
   const float* qx = q;
   const float* qy = q + nq;
   const float* qz = t.dim == 3 ? q + 2ull * nq : nullptr;
@@ -1031,7 +1043,11 @@ void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* ord
     for (int pss = 0; pss < 8; ++pss) {
       const uint32_t* h = fh + (size_t)pss * 256 * fb;
       uint32_t* hn = pss < 7 ? fh + (size_t)(pss + 1) * 256 * fb : nullptr;
-      k_fr_pass<<<fb, kFrThreads, 0, s>>>(kin, vin, kout, vout, nq, 8 * pss, fb, h, hn);
+      const size_t lds = 256u * fb * sizeof(uint32_t);
+      if (compact)
+        k_fr_pass<256><<<fb, 256, lds, s>>>(kin, vin, kout, vout, nq, 8 * pss, fb, h, hn);
+      else
+        k_fr_pass<1024><<<fb, 1024, lds, s>>>(kin, vin, kout, vout, nq, 8 * pss, fb, h, hn);
       SBH_HIP(hipGetLastError());
       std::swap(kin, kout);
       std::swap(vin, vout);

@@ -86,10 +86,11 @@ void query_order_double(const DeviceTree& t, const double* q, uint32_t nq, float
 void run_smooth_distance_double(double* w, uint32_t n, double alpha, hipStream_t s);
 
 // Spatial (Hilbert-curve) order of the queries; scratch holds sort_scratch_u32(nq)
-// uint32 values.
+// uint32 values.  compact: smaller workgroups, for a sort that shares the GPU
+// with other running launches (the same order).
 size_t sort_scratch_u32(uint32_t nq);
 void query_order(const DeviceTree& t, const float* q, uint32_t nq, uint32_t* order,
-                 uint32_t* scratch, hipStream_t s);
+                 uint32_t* scratch, hipStream_t s, bool compact = false);
 
 // Exclusive scan of n uint32 values; scratch holds exclusive_scan_scratch(n)
 // values; *total_dev (optional) receives the sum.

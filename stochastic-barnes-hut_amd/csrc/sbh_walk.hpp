@@ -231,8 +231,21 @@ __device__ __forceinline__ LaneResult bh_walk_task(const TreeView& t,
 // partial[k * nq + slot] is task k's part.
 __device__ __forceinline__ float sum_bh_parts(const float* __restrict__ partial, uint32_t nsplit,
                                               uint32_t nq, uint32_t slot) {
-  float s = partial[slot];
-  for (uint32_t k = 1; k < nsplit; ++k) s += partial[(size_t)k * nq + slot];
+  // eight loads in flight at a time; the additions keep task order
+  constexpr uint32_t kBatch = 8;
+  float s = 0.0f;
+  uint32_t k = 0;
+  for (; k + kBatch <= nsplit; k += kBatch) {
+    float v[kBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) v[j] = partial[(size_t)(k + j) * nq + slot];
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) s = (k + j == 0) ? v[j] : s + v[j];
+  }
+  for (; k < nsplit; ++k) {
+    const float v = partial[(size_t)k * nq + slot];
+    s = k == 0 ? v : s + v;
+  }
   return s;
 }
 
@@ -241,8 +254,21 @@ __device__ __forceinline__ float sum_bh_parts(const float* __restrict__ partial,
 __device__ __forceinline__ float sum_stochastic_parts(const float2* __restrict__ partial,
                                                       uint32_t nsplit, uint32_t nq,
                                                       uint32_t slot) {
+  // eight loads in flight at a time; the additions keep subdomain order
+  constexpr uint32_t kBatch = 8;
   float bh = 0.0f, mlcv = 0.0f;
-  for (uint32_t k = 0; k < nsplit; ++k) {
+  uint32_t k = 0;
+  for (; k + kBatch <= nsplit; k += kBatch) {
+    float2 v[kBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) v[j] = partial[(size_t)(k + j) * nq + slot];
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch; ++j) {
+      bh += v[j].x;
+      mlcv += v[j].y;
+    }
+  }
+  for (; k < nsplit; ++k) {
     const float2 p = partial[(size_t)k * nq + slot];
     bh += p.x;
     mlcv += p.y;
@@ -618,6 +644,12 @@ constexpr uint32_t chunk_for() {
 }
 constexpr uint32_t kCacheLevels = 7;    // path levels 1..7 cached
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+// Lanes of the estimator's cache register per field (kCacheLevels + 1 step
+// records): node ids, point and node term counts, step ranges' ends.
+constexpr uint32_t kCacheF = kCacheLevels + 1;
+constexpr uint32_t kLanePt = kCacheF, kLaneNd = 2 * kCacheF;
+constexpr uint32_t kLaneLo = 3 * kCacheF, kLaneHi = 4 * kCacheF;
+static_assert(5 * kCacheF <= 64, "the cache register's fields fit in 64 lanes");
 
 // This wavefront's LDS for the fast estimator.
 struct SortedLds {
@@ -706,9 +738,9 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     return {v, cnt, cnt_nodes};
   }
   // the cached nodes' identities and term counts live in one register:
-  // lane l < kCacheLevels holds level l + 1's node, lane 8 + l its point
-  // terms and lane 16 + l its node terms (read back with readlane)
-  static_assert(kCacheLevels + 1 <= 8, "a register holds eight lanes per field");
+  // lane l < kCacheLevels holds level l + 1's node, lane kCacheF + l its
+  // point terms and lane 2 kCacheF + l its node terms (read back with
+  // readlane); the step records' ranges follow (lanes 3 kCacheF + r, 4 kCacheF + r)
   uint32_t cinfo = lane < kCacheLevels ? kNoNode : 0u;
 
   // Evaluates the children of v (at most 64) for every live lane: returns
@@ -776,20 +808,28 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   // The step taken from each cached node by the latest sample through it
   // (record r = 1..kCacheLevels for path level r, record 0 for the
   // subdomain's root): the chosen child's point range [lo, hi) in lanes
-  // 24 + r and 32 + r of `cinfo`, and in LDS the node stepped to (single-child
+  // kLaneLo + r and kLaneHi + r of `cinfo`, and in LDS the node stepped to (single-child
   // chains skipped) with its walk record, centroid and normal.  A sorted
   // sample whose point lies in the same child takes the step without a load.
   constexpr uint32_t kNoRecord = 0xFFu;
+  // The level-1 node of record 0 (the node stepped to from the subdomain's
+  // root), once a sample has reached it: a leaf, or an internal node whose
+  // traversal probability (a function of the node and the query) is kept per
+  // lane in l1_tp.  A later sample through record 0 then needs only its
+  // level-1 roulette draw to know whether any lane goes on.
+  constexpr uint32_t kL1Unknown = 0, kL1Internal = 1, kL1Leaf = 2;
+  uint32_t l1_state = kL1Unknown;
+  float l1_tp = 0.0f;
   auto range_has = [&](uint32_t r, uint32_t idx) {
-    return idx >= (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(24 + r)) &&
-           idx < (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(32 + r));
+    return idx >= (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneLo + r)) &&
+           idx < (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneHi + r));
   };
   auto take = [&](uint32_t r, uint32_t& cur, uint32_t& cur_cnt, uint4& curL, float4& curC,
                   float4& curN) {
     const uint4 a = lds.nrec[3 * r], c = lds.nrec[3 * r + 1];
     cur = uniform_u32(a.x);
-    cur_cnt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(32 + r)) -
-              (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(24 + r));
+    cur_cnt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneHi + r)) -
+              (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneLo + r));
     curL = make_uint4(a.y, uniform_u32(a.z), 0u, uniform_u32(a.w));
     curC = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z),
                        __uint_as_float(c.w));
@@ -812,7 +852,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     curC = cr.c;
     curN = kWN ? nodeN[cur] : curC;
     if (r != kNoRecord) {
-      cinfo = lane == 24 + r ? lo : lane == 32 + r ? hi : cinfo;
+      if (r == 0) l1_state = kL1Unknown;  // record 0 now names another node
+      cinfo = lane == kLaneLo + r ? lo : lane == kLaneHi + r ? hi : cinfo;
       if (lane == 0) {
         lds.nrec[3 * r] = make_uint4(cur, curL.x, curL.y, curL.w);
         lds.nrec[3 * r + 1] = make_uint4(__float_as_uint(curC.x), __float_as_uint(curC.y),
@@ -834,7 +875,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     node_accumulate<D, K, EXACT>(t, S, x, y, z, bh, cnt, cnt_nodes);
     const uint4 Sl = nodeL[S];
     if (Sl.w & kLeafBit) continue;  // the subdomain's points were summed exactly
-    cinfo = lane == 32 ? 0u : cinfo;  // record 0 belongs to the previous subdomain: empty it
+    cinfo = lane == kLaneHi ? 0u : cinfo;  // record 0 belongs to the previous subdomain: empty it
+    l1_state = kL1Unknown;
     const uint32_t sstart = n_start[S];
     const uint32_t nS = n_end[S] - sstart;
     const float4 SC = nodeC[S];
@@ -875,6 +917,21 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         const uint32_t idx = uniform_u32(key.x);
         const uint32_t w1 = uniform_u32(key.y);
         const uint32_t ss = (uint32_t)s0 + (w1 & 0xFFu);
+        if (l1_state != kL1Unknown && range_has(0, idx)) {
+          // the sample's level-1 node is record 0's: when no lane goes on
+          // from it, the sample adds the level-0 weight alone, as the full
+          // step below would
+          const bool on = l1_state == kL1Internal && live && !(rr_uniform(w1) > l1_tp);
+          if (__ballot(on) == 0) {
+            if constexpr (WORK) wlevels += (live && l1_state == kL1Internal) ? 1 : 0;
+            if constexpr (REF) {
+              cnt += live ? (int32_t)pt0 : 0;
+              cnt_nodes += live ? (int32_t)nd0 : 0;
+            }
+            sub_total += live ? 0.0f + wd0 : 0.0f;
+            continue;
+          }
+        }
         uint32_t cur, cur_cnt;
         uint4 curL;
         float4 curC, curN;
@@ -896,6 +953,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         Philox4 blk{0u, 0u, 0u, w1};  // block 0: only its level-1 word is used here
         uint32_t blk_id = 0;
         uint32_t level = 1;
+        if (curL.w & kLeafBit) l1_state = kL1Leaf;
         while (!(curL.w & kLeafBit)) {
           const uint32_t wi = level + 2;
           if ((wi >> 2) != blk_id) {
@@ -914,6 +972,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           const float ffr = ratio_fast<D>(curC, __uint_as_float(curL.x), x, y, z);
           ++level;
           const float tp = traverse_prob<EXACT>(cur_beta, ffr);
+          if (slot == 0) {
+            l1_state = kL1Internal;
+            l1_tp = tp;
+          }
           // roulette, as selects (no per-lane branches)
           const bool stop = p > tp;
           event_prob = alive ? event_prob * (stop ? 1.0f - tp : tp) : event_prob;
@@ -927,8 +989,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           if (hit) {
             diff = lds.cdiff[cslot];
             if constexpr (REF) {
-              pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(8 + slot));
-              nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(16 + slot));
+              pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLanePt + slot));
+              nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneNd + slot));
             } else {
               pt = nd = 0;
             }
@@ -947,7 +1009,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
               // each lane reads back only its own diff: no wavefront sync
               lds.cdiff[cslot] = diff;
               if constexpr (REF)
-                cinfo = lane == slot ? cur : lane == 8 + slot ? pt : lane == 16 + slot ? nd : cinfo;
+                cinfo = lane == slot            ? cur
+                        : lane == kLanePt + slot ? pt
+                        : lane == kLaneNd + slot ? nd
+                                                 : cinfo;
               else
                 cinfo = lane == slot ? cur : cinfo;
             }

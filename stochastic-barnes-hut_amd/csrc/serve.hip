@@ -586,7 +586,7 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     const size_t qf = (size_t)r->nq * r->dim;
     std::memcpy(L.stage + so, r->queries, qf * sizeof(float));
     SBH_HIP(hipMemcpyAsync(dq, L.stage + so, qf * sizeof(float), hipMemcpyHostToDevice, s));
-    query_order(t, dq, r->nq, dorder, dscratch, s);
+    query_order(t, dq, r->nq, dorder, dscratch, s, /*compact=*/true);
     so += qf;
     L.res_off.push_back(so);
     so += r->nq;
