@@ -121,23 +121,43 @@ __device__ __forceinline__ float ratio_exact(float4 c, float diam, float qx, flo
   return __builtin_sqrtf(sqnorm_exact<D>(dx, dy, dz)) / diam;
 }
 
+// Separation q - p of a query from a source or node centroid and its
+// squared norm with fused squares.  The fast kernels, the opening ratio and
+// the opening test all start from it, so a walk that tests a node and then
+// takes its term computes the distance once.
+struct Sep {
+  float dx, dy, dz, d2;
+};
+template <int D>
+__device__ __forceinline__ Sep separation(float4 p, float qx, float qy, float qz) {
+  Sep s;
+  s.dx = qx - p.x;
+  s.dy = qy - p.y;
+  s.dz = (D == 3) ? (qz - p.z) : 0.0f;
+  s.d2 = (D == 3) ? __builtin_fmaf(s.dx, s.dx, __builtin_fmaf(s.dy, s.dy, s.dz * s.dz))
+                  : __builtin_fmaf(s.dx, s.dx, s.dy * s.dy);
+  return s;
+}
+__device__ __forceinline__ float fast_dist(const Sep& s) { return __builtin_amdgcn_sqrtf(s.d2); }
+
 // ratio >= beta, decided exactly as ratio_exact would decide it, in two
 // parts: a fast estimate (fused squares, hardware sqrt and reciprocal; within
 // 7e-7 relative of ratio_exact) settles every case farther than 2e-6 from
 // beta (1 = far, 0 = near, -1 = undecided); the correctly rounded formula
 // decides the rest.
-// inv_diam = hardware reciprocal of diam, computed by the caller.
+// inv_diam = hardware reciprocal of diam, computed by the caller; s and dist
+// = separation<D>(c, q) and fast_dist(s).
+__device__ __forceinline__ int opens_far_fast(const Sep& s, float dist, float inv_diam,
+                                              float beta) {
+  float r = dist * inv_diam;
+  const bool ok = s.d2 >= 1e-30f;
+  return (ok && r >= beta * 1.000002f) ? 1 : (ok && r <= beta * 0.999998f) ? 0 : -1;
+}
 template <int D>
 __device__ __forceinline__ int opens_far_fast(float4 c, float inv_diam, float beta, float qx,
                                               float qy, float qz) {
-  float dx = qx - c.x;
-  float dy = qy - c.y;
-  float dz = (D == 3) ? (qz - c.z) : 0.0f;
-  float d2 = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
-                      : __builtin_fmaf(dx, dx, dy * dy);
-  float r = __builtin_amdgcn_sqrtf(d2) * inv_diam;
-  const bool ok = d2 >= 1e-30f;
-  return (ok && r >= beta * 1.000002f) ? 1 : (ok && r <= beta * 0.999998f) ? 0 : -1;
+  const Sep s = separation<D>(c, qx, qy, qz);
+  return opens_far_fast(s, fast_dist(s), inv_diam, beta);
 }
 template <int D>
 __device__ __forceinline__ bool opens_far_exact(float4 c, float diam, float beta, float qx,
@@ -150,86 +170,83 @@ __device__ __forceinline__ bool opens_far_exact(float4 c, float diam, float beta
 }
 
 // ---------------------------------------------------------------------------
-// Fast kernels for brute force: hardware sqrt / reciprocal / exp2 (about one
-// ulp each), fused multiply-adds allowed.  Sums stay within 1e-5 relative of
-// the exact formulation.
+// Fast kernels for brute force and the walks: hardware sqrt / reciprocal /
+// exp2 (about one ulp each), fused multiply-adds allowed.  Sums stay within
+// 1e-5 relative of the exact formulation.  They take the separation q - p:
+// where the formula needs p - q (the winding number's normal projection, the
+// gradient's direction) the sign is folded into the mass, which is exact.
 // ---------------------------------------------------------------------------
+// acc + the fast kernel of source src at separation s (dist = fast_dist(s)),
+// with the term's final multiply fused into the accumulation.
 template <int D, int K>
-__device__ __forceinline__ float kernel_fast(float4 src, float4 nrm, float alpha, float qx,
-                                             float qy, float qz) {
-  float dx = src.x - qx;
-  float dy = src.y - qy;
-  float dz = (D == 3) ? (src.z - qz) : 0.0f;
-  float dsq = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
-                       : __builtin_fmaf(dx, dx, dy * dy);
+__device__ __forceinline__ float kernel_fast_acc(float acc, float4 src, float4 nrm, float alpha,
+                                                 const Sep& s, float dist) {
   if constexpr (K == GRAVITY) {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
-    return -src.w * __builtin_amdgcn_rcpf(dist + 1e-1f);
+    return __builtin_fmaf(-src.w, __builtin_amdgcn_rcpf(dist + 1e-1f), acc);
   } else if constexpr (K == WINDING_NUMBER) {
+    // n . (p - q) = -(n . s)
     if constexpr (D == 3) {
-      float dist = __builtin_amdgcn_sqrtf(dsq);
-      float qdn = __builtin_fmaf(nrm.x, dx, __builtin_fmaf(nrm.y, dy, nrm.z * dz));
-      float den = kFourPi * __builtin_fmaf(dsq, dist, 1e-5f);
-      return src.w * qdn * __builtin_amdgcn_rcpf(den);
+      float nds = __builtin_fmaf(nrm.x, s.dx, __builtin_fmaf(nrm.y, s.dy, nrm.z * s.dz));
+      float den = kFourPi * __builtin_fmaf(s.d2, dist, 1e-5f);
+      return __builtin_fmaf(-src.w * nds, __builtin_amdgcn_rcpf(den), acc);
     } else {
-      float qdn = __builtin_fmaf(nrm.x, dx, nrm.y * dy);
-      return src.w * qdn * __builtin_amdgcn_rcpf(kTwoPi * (dsq + 1e-5f));
+      float nds = __builtin_fmaf(nrm.x, s.dx, nrm.y * s.dy);
+      return __builtin_fmaf(-src.w * nds, __builtin_amdgcn_rcpf(kTwoPi * (s.d2 + 1e-5f)), acc);
     }
   } else if constexpr (is_helmholtz(K)) {
     // the hardware sine / cosine take revolutions: cos(2 pi freq d) = cos_rev(freq d)
-    float dist = __builtin_amdgcn_sqrtf(dsq);
-    float rev = __builtin_amdgcn_fractf((K == HELMHOLTZ_X ? 2.0f : 1.0f) * dist);
-    float c = (K == HELMHOLTZ_X) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
-    return src.w * c * __builtin_amdgcn_rcpf(dist + 1e-1f);
-  } else if constexpr (is_gradient(K)) {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
-    float r = __builtin_amdgcn_rcpf(dist + 1e-1f);
-    float comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
-    return -src.w * (r * r * r) * comp;
-  } else {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
-    // exp(-a d) = 2^(-a d log2 e); alpha is pre-scaled by log2(e) by the caller.
-    return __builtin_amdgcn_exp2f(-alpha * dist);
-  }
-}
-
-// acc + kernel_fast(...), with the term's final multiply fused into the
-// accumulation.
-template <int D, int K>
-__device__ __forceinline__ float kernel_fast_acc(float acc, float4 src, float4 nrm, float alpha,
-                                                 float qx, float qy, float qz) {
-  float dx = src.x - qx;
-  float dy = src.y - qy;
-  float dz = (D == 3) ? (src.z - qz) : 0.0f;
-  float dsq = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
-                       : __builtin_fmaf(dx, dx, dy * dy);
-  if constexpr (K == GRAVITY) {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
-    return __builtin_fmaf(-src.w, __builtin_amdgcn_rcpf(dist + 1e-1f), acc);
-  } else if constexpr (K == WINDING_NUMBER) {
-    if constexpr (D == 3) {
-      float dist = __builtin_amdgcn_sqrtf(dsq);
-      float qdn = __builtin_fmaf(nrm.x, dx, __builtin_fmaf(nrm.y, dy, nrm.z * dz));
-      float den = kFourPi * __builtin_fmaf(dsq, dist, 1e-5f);
-      return __builtin_fmaf(src.w * qdn, __builtin_amdgcn_rcpf(den), acc);
-    } else {
-      float qdn = __builtin_fmaf(nrm.x, dx, nrm.y * dy);
-      return __builtin_fmaf(src.w * qdn, __builtin_amdgcn_rcpf(kTwoPi * (dsq + 1e-5f)), acc);
-    }
-  } else if constexpr (is_helmholtz(K)) {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
     float rev = __builtin_amdgcn_fractf((K == HELMHOLTZ_X ? 2.0f : 1.0f) * dist);
     float c = (K == HELMHOLTZ_X) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
     return __builtin_fmaf(src.w * c, __builtin_amdgcn_rcpf(dist + 1e-1f), acc);
   } else if constexpr (is_gradient(K)) {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
+    // component of -m / (d + 0.1)^3 * (q - p)
     float r = __builtin_amdgcn_rcpf(dist + 1e-1f);
-    float comp = (K == GRAVITY_GRAD_X) ? -dx : (K == GRAVITY_GRAD_Y) ? -dy : -dz;
+    float comp = (K == GRAVITY_GRAD_X) ? s.dx : (K == GRAVITY_GRAD_Y) ? s.dy : s.dz;
     return __builtin_fmaf(-src.w * (r * r * r), comp, acc);
   } else {
-    float dist = __builtin_amdgcn_sqrtf(dsq);
+    // exp(-a d) = 2^(-a d log2 e); alpha is pre-scaled by log2(e) by the caller.
     return acc + __builtin_amdgcn_exp2f(-alpha * dist);
   }
+}
+template <int D, int K>
+__device__ __forceinline__ float kernel_fast_acc(float acc, float4 src, float4 nrm, float alpha,
+                                                 float qx, float qy, float qz) {
+  const Sep s = separation<D>(src, qx, qy, qz);
+  return kernel_fast_acc<D, K>(acc, src, nrm, alpha, s, fast_dist(s));
+}
+
+// The fast kernel's term alone.
+template <int D, int K>
+__device__ __forceinline__ float kernel_fast(float4 src, float4 nrm, float alpha, const Sep& s,
+                                             float dist) {
+  if constexpr (K == GRAVITY) {
+    return -src.w * __builtin_amdgcn_rcpf(dist + 1e-1f);
+  } else if constexpr (K == WINDING_NUMBER) {
+    if constexpr (D == 3) {
+      float nds = __builtin_fmaf(nrm.x, s.dx, __builtin_fmaf(nrm.y, s.dy, nrm.z * s.dz));
+      float den = kFourPi * __builtin_fmaf(s.d2, dist, 1e-5f);
+      return -src.w * nds * __builtin_amdgcn_rcpf(den);
+    } else {
+      float nds = __builtin_fmaf(nrm.x, s.dx, nrm.y * s.dy);
+      return -src.w * nds * __builtin_amdgcn_rcpf(kTwoPi * (s.d2 + 1e-5f));
+    }
+  } else if constexpr (is_helmholtz(K)) {
+    float rev = __builtin_amdgcn_fractf((K == HELMHOLTZ_X ? 2.0f : 1.0f) * dist);
+    float c = (K == HELMHOLTZ_X) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+    return src.w * c * __builtin_amdgcn_rcpf(dist + 1e-1f);
+  } else if constexpr (is_gradient(K)) {
+    float r = __builtin_amdgcn_rcpf(dist + 1e-1f);
+    float comp = (K == GRAVITY_GRAD_X) ? s.dx : (K == GRAVITY_GRAD_Y) ? s.dy : s.dz;
+    return -src.w * (r * r * r) * comp;
+  } else {
+    return __builtin_amdgcn_exp2f(-alpha * dist);
+  }
+}
+template <int D, int K>
+__device__ __forceinline__ float kernel_fast(float4 src, float4 nrm, float alpha, float qx,
+                                             float qy, float qz) {
+  const Sep s = separation<D>(src, qx, qy, qz);
+  return kernel_fast<D, K>(src, nrm, alpha, s, fast_dist(s));
 }
 
 // Opening ratio ||q - c|| / diam with hardware sqrt and the caller's
@@ -237,12 +254,7 @@ __device__ __forceinline__ float kernel_fast_acc(float acc, float4 src, float4 n
 template <int D>
 __device__ __forceinline__ float ratio_fast(float4 c, float inv_diam, float qx, float qy,
                                             float qz) {
-  float dx = qx - c.x;
-  float dy = qy - c.y;
-  float dz = (D == 3) ? (qz - c.z) : 0.0f;
-  float d2 = (D == 3) ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))
-                      : __builtin_fmaf(dx, dx, dy * dy);
-  return __builtin_amdgcn_sqrtf(d2) * inv_diam;
+  return fast_dist(separation<D>(c, qx, qy, qz)) * inv_diam;
 }
 
 // float64 formulation of the same kernels (error reporting only).
@@ -405,6 +417,17 @@ __device__ __forceinline__ uint32_t scale_draw(uint32_t lo, uint32_t hi, uint32_
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
+}
+
+// A set of lanes of the wavefront, one bit per lane, held in scalar
+// registers.  Predicates that combine others (a && b, a && !b) are kept as
+// masks and combined with scalar operations: the ballot of a comparison is
+// the comparison's own result, while the ballot of a combined per-lane bool
+// costs two vector instructions (materialise it, compare it).  lane_in reads
+// this lane's bit as a per-lane condition for selects, at no cost.
+using LaneMask = uint64_t;
+__device__ __forceinline__ bool lane_in(LaneMask m) {
+  return __builtin_amdgcn_inverse_ballot_w64(m);
 }
 
 // Element i of an array no kernel writes while it runs, read through the

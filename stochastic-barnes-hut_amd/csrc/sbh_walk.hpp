@@ -112,68 +112,84 @@ template <int D, int K, int GROUP, bool EXACT, bool COUNT = true>
 __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, bool live, float x,
                                               float y, float z, uint32_t v0 = 0,
                                               uint32_t vstop = kEnd) {
-  const uint32_t lane = threadIdx.x & 63;
-  uint64_t my_group;
-  if constexpr (GROUP == 64) {
-    my_group = ~0ull;
-  } else if constexpr (GROUP == 32) {
-    my_group = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
-  } else {
-    my_group = 1ull << lane;
-  }
+  // Lane sets (live, active, far, reject, accept) are lane masks in scalar
+  // registers, combined with scalar operations and read per lane with
+  // lane_in (see LaneMask).
   const WalkRecord* __restrict__ rec = t.dfsR;
   const float4* __restrict__ nodeN = t.dfsN;
+  const float beta_hi = beta * 1.000002f, beta_lo = beta * 0.999998f;
   float acc = 0.0f;
   int32_t cnt = 0, cnt_nodes = 0;
-  bool active = live;
+  const LaneMask live_m = __ballot(live);
+  LaneMask active = live_m;
   uint32_t resume = kEnd;
   uint32_t v = v0;
   float4 c = rec[v0].c;
   uint4 l = rec[v0].l;
   while (v != vstop) {
     v = uniform_u32(v);
-    active = active || (live && resume == v);
+    active |= live_m & __ballot(resume == v);
     const bool leaf = (l.w & kLeafBit) != 0;
     // the record (c, l) of v was loaded at the end of the previous step, as
     // soon as v was known.  The step is branch-free per lane (selects, not
-    // exec-mask branches): the walk's control runs on the scalar unit, which
-    // is the busier one.
-    // every field of the record is read on every path, so it arrives in one
-    // scalar load
+    // exec-mask branches).  Every field of the record is read on every path,
+    // so it arrives in one scalar load.
     const float diam = __uint_as_float(l.x);
-    const float inv_diam = __builtin_amdgcn_rcpf(diam);
-    bool reject = false;
-    if (!leaf) {
-      int far = opens_far_fast<D>(c, inv_diam, beta, x, y, z);
-      if (__ballot(active && far < 0))  // a lane within 2e-6 of beta: decide it exactly
-        far = far < 0 ? (int)opens_far_exact<D>(c, diam, beta, x, y, z) : far;
-      reject = active && far == 0;
-    }
-    const uint64_t rej = __ballot(reject);
-    // per query, a lane's own test decides; a vote group needs every lane
-    const bool accept = active && (GROUP == 1 ? !reject : (rej & my_group) == 0);
     const uint32_t np = leaf ? (l.w & ~kLeafBit) : 0u;
+    // the separation from the record's centroid (or one-point leaf's point),
+    // shared by the opening test and the fast term
+    const Sep s = separation<D>(c, x, y, z);
+    const float dist = fast_dist(s);
+    LaneMask rej = 0;
+    if (!leaf) {
+      // far: ratio >= beta.  The fast ratio (within 7e-7 of the correctly
+      // rounded one) settles every lane farther than 2e-6 from beta; the
+      // correctly rounded formula decides the rest, so every lane opens
+      // exactly the nodes the reference walk opens.
+      const float r = dist * __builtin_amdgcn_rcpf(diam);
+      const LaneMask ok = __ballot(s.d2 >= 1e-30f);
+      LaneMask far = ok & __ballot(r >= beta_hi);
+      const LaneMask near = ok & __ballot(r <= beta_lo);
+      const LaneMask undecided = active & ~(far | near);
+      if (undecided) far |= undecided & __ballot(opens_far_exact<D>(c, diam, beta, x, y, z));
+      rej = active & ~far;
+    }
+    // per query, a lane's own test decides; a vote group needs every lane
+    LaneMask accept;
+    if constexpr (GROUP == 1) {
+      accept = active & ~rej;
+    } else if constexpr (GROUP == 32) {
+      constexpr LaneMask lo = 0x00000000FFFFFFFFull, hi = 0xFFFFFFFF00000000ull;
+      accept = active & (((rej & lo) ? 0ull : lo) | ((rej & hi) ? 0ull : hi));
+    } else {
+      accept = rej ? 0ull : active;
+    }
+    const bool acc_lane = lane_in(accept);
     if (np > 1) {
-      if (accept) {
+      if (acc_lane) {
         for (uint32_t j = l.y; j < l.y + np; ++j)
           acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
       }
-    } else if (__ballot(accept)) {
+    } else if (accept) {
       // an internal node's moment, or a one-point leaf's point (its record
       // holds the point)
-      const float with = accum<D, K, EXACT>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
-      acc = accept ? with : acc;
+      float with;
+      if constexpr (EXACT)
+        with = accum<D, K, true>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
+      else
+        with = kernel_fast_acc<D, K>(acc, c, nodeN[v], t.alpha_k, s, dist);
+      acc = acc_lane ? with : acc;
     }
     if constexpr (COUNT) {
-      cnt += accept ? (int32_t)np : 0;
-      cnt_nodes += (accept && !leaf) ? 1 : 0;
+      cnt += acc_lane ? (int32_t)np : 0;
+      cnt_nodes += (acc_lane && !leaf) ? 1 : 0;
     }
     uint32_t next;
     if (rej == 0) {
       next = l.z;
     } else {
-      resume = accept ? l.z : resume;
-      active = active && !accept;
+      resume = acc_lane ? l.z : resume;
+      active &= ~accept;
       next = l.y;
     }
     next = uniform_u32(next);
@@ -713,6 +729,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   constexpr bool REF = COUNT && !WORK;  // the reference's term counts
   int32_t wterms = 0, wlevels = 0;      // work counts
   const uint32_t lane = threadIdx.x & 63;
+  const LaneMask live_m = __ballot(live);
   const float4* __restrict__ nodeC = t.nodeC;
   const uint4* __restrict__ nodeL = t.nodeL;
   const float4* __restrict__ nodeN = t.nodeN;
@@ -921,8 +938,9 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           // the sample's level-1 node is record 0's: when no lane goes on
           // from it, the sample adds the level-0 weight alone, as the full
           // step below would
-          const bool on = l1_state == kL1Internal && live && !(rr_uniform(w1) > l1_tp);
-          if (__ballot(on) == 0) {
+          const LaneMask on =
+              l1_state == kL1Internal ? live_m & __ballot(!(rr_uniform(w1) > l1_tp)) : 0ull;
+          if (on == 0) {
             if constexpr (WORK) wlevels += (live && l1_state == kL1Internal) ? 1 : 0;
             if constexpr (REF) {
               cnt += live ? (int32_t)pt0 : 0;
@@ -939,13 +957,14 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         if (range_has(0, idx)) {
           take(0, cur, cur_cnt, curL, curC, curN);
         } else {
-          const uint32_t below0 = (uint32_t)__popcll(__ballot(lane < nch0 && c0_start <= idx));
+          const uint32_t below0 =
+              (uint32_t)__popcll(__ballot(lane < nch0) & __ballot(c0_start <= idx));
           step(uniform_u32(f0 + below0 - 1), 0, cur, cur_cnt, curL, curC, curN);
         }
         float event_prob = 1.0f;
         float wd_all = live ? 0.0f + wd0 : 0.0f;
         float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
-        bool alive = live;
+        LaneMask alive = live_m;
         if constexpr (REF) {
           cnt += live ? (int32_t)pt0 : 0;
           cnt_nodes += live ? (int32_t)nd0 : 0;
@@ -977,11 +996,11 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             l1_tp = tp;
           }
           // roulette, as selects (no per-lane branches)
-          const bool stop = p > tp;
-          event_prob = alive ? event_prob * (stop ? 1.0f - tp : tp) : event_prob;
-          if constexpr (WORK) wlevels += alive ? 1 : 0;
-          alive = alive && !stop;
-          if (__ballot(alive) == 0) break;
+          const LaneMask stop = __ballot(p > tp);
+          event_prob = lane_in(alive) ? event_prob * (lane_in(stop) ? 1.0f - tp : tp) : event_prob;
+          if constexpr (WORK) wlevels += lane_in(alive) ? 1 : 0;
+          alive &= ~stop;
+          if (alive == 0) break;
           const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
           float diff;
           uint32_t below = 0, pt, nd;
@@ -1003,7 +1022,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             uint32_t cs;
             const float r = children_sum(f, nch, &cs, &pt, &nd);
             diff = r - term<D, K, EXACT>(curC, curN, alpha, alpha_k, x, y, z);
-            if constexpr (WORK) wterms += alive ? (int32_t)(pt + nd + 1) : 0;
+            if constexpr (WORK) wterms += lane_in(alive) ? (int32_t)(pt + nd + 1) : 0;
             below = (uint32_t)__popcll(__ballot(cs <= idx));
             if (slot < kCacheLevels) {
               // each lane reads back only its own diff: no wavefront sync
@@ -1018,10 +1037,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             }
           }
           const float wd = level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
-          wd_all = alive ? wd_all + wd : wd_all;
+          wd_all = lane_in(alive) ? wd_all + wd : wd_all;
           if constexpr (REF) {
-            cnt += alive ? (int32_t)pt : 0;
-            cnt_nodes += alive ? (int32_t)nd : 0;
+            cnt += lane_in(alive) ? (int32_t)pt : 0;
+            cnt_nodes += lane_in(alive) ? (int32_t)nd : 0;
           }
           cur_beta = fmaxf(cur_beta, ffr);
           if (known)
