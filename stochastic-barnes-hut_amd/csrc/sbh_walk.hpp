@@ -115,7 +115,8 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   // Lane sets (live, active, far, reject, accept) are lane masks in scalar
   // registers, combined with scalar operations and read per lane with
   // lane_in (see LaneMask).
-  const WalkRecord* __restrict__ rec = t.dfsR;
+  // records addressed by 32-bit byte offsets (a tree has < 2^27 nodes)
+  const char* __restrict__ rec = reinterpret_cast<const char*>(t.dfsR);
   const float4* __restrict__ nodeN = t.dfsN;
   const float beta_hi = beta * 1.000002f, beta_lo = beta * 0.999998f;
   float acc = 0.0f;
@@ -124,12 +125,13 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   LaneMask active = live_m;
   uint32_t resume = kEnd;
   uint32_t v = v0;
-  float4 c = rec[v0].c;
-  uint4 l = rec[v0].l;
+  WalkRecord R = *reinterpret_cast<const WalkRecord*>(rec + (v0 << 5));
   while (v != vstop) {
     v = uniform_u32(v);
+    const float4 c = R.c;
+    const uint4 l = R.l;
     active |= live_m & __ballot(resume == v);
-    const bool leaf = (l.w & kLeafBit) != 0;
+    const bool leaf = (int32_t)l.w < 0;
     // the record (c, l) of v was loaded at the end of the previous step, as
     // soon as v was known.  The step is branch-free per lane (selects, not
     // exec-mask branches).  Every field of the record is read on every path,
@@ -146,11 +148,12 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       // rounded one) settles every lane farther than 2e-6 from beta; the
       // correctly rounded formula decides the rest, so every lane opens
       // exactly the nodes the reference walk opens.
-      const float r = dist * __builtin_amdgcn_rcpf(diam);
-      const LaneMask ok = __ballot(s.d2 >= 1e-30f);
-      LaneMask far = ok & __ballot(r >= beta_hi);
-      const LaneMask near = ok & __ballot(r <= beta_lo);
-      const LaneMask undecided = active & ~(far | near);
+      // (a lane whose squared distance is below 1e-30 gets a NaN ratio,
+      // which neither comparison settles)
+      float r = dist * __builtin_amdgcn_rcpf(diam);
+      r = s.d2 >= 1e-30f ? r : __builtin_nanf("");
+      LaneMask far = __ballot(r >= beta_hi);
+      const LaneMask undecided = active & ~(far | __ballot(r <= beta_lo));
       if (undecided) far |= undecided & __ballot(opens_far_exact<D>(c, diam, beta, x, y, z));
       rej = active & ~far;
     }
@@ -165,14 +168,15 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       accept = rej ? 0ull : active;
     }
     const bool acc_lane = lane_in(accept);
-    if (np > 1) {
+    if (l.w > (kLeafBit | 1u)) {  // a leaf of several points
       if (acc_lane) {
         for (uint32_t j = l.y; j < l.y + np; ++j)
           acc = accum<D, K, EXACT>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
       }
-    } else if (accept) {
+    } else {
       // an internal node's moment, or a one-point leaf's point (its record
-      // holds the point)
+      // holds the point); computed for every lane (cheaper than a branch on
+      // the scalar unit), kept where the lane accepts v
       float with;
       if constexpr (EXACT)
         with = accum<D, K, true>(acc, c, nodeN[v], t.alpha, t.alpha_k, x, y, z);
@@ -184,20 +188,15 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
       cnt += acc_lane ? (int32_t)np : 0;
       cnt_nodes += (acc_lane && !leaf) ? 1 : 0;
     }
-    uint32_t next;
-    if (rej == 0) {
-      next = l.z;
-    } else {
-      resume = acc_lane ? l.z : resume;
-      active &= ~accept;
-      next = l.y;
-    }
-    next = uniform_u32(next);
-    if (next != vstop) {
-      const WalkRecord r = rec[next];
-      c = r.c;
-      l = r.l;
-    }
+    // descend when some lane rejects v (the accepting lanes resume at v's
+    // rope), else follow the rope
+    const uint32_t next = uniform_u32(rej ? l.y : l.z);
+    resume = lane_in(rej ? accept : 0ull) ? l.z : resume;
+    if constexpr (GROUP == 1)
+      active = rej ? rej : active;  // the lanes that reject v are the ones left
+    else
+      active = rej ? (active & ~accept) : active;
+    if (next != vstop) R = *reinterpret_cast<const WalkRecord*>(rec + (next << 5));
     v = next;
   }
   return {acc, cnt, cnt_nodes};
