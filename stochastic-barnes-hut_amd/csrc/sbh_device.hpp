@@ -425,6 +425,14 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
 // the comparison's own result, while the ballot of a combined per-lane bool
 // costs two vector instructions (materialise it, compare it).  lane_in reads
 // this lane's bit as a per-lane condition for selects, at no cost.
+// Orders this wavefront's LDS writes before its later LDS reads by other
+// lanes (a wavefront-scope fence and wave barrier; no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 using LaneMask = uint64_t;
 __device__ __forceinline__ bool lane_in(LaneMask m) {
   return __builtin_amdgcn_inverse_ballot_w64(m);

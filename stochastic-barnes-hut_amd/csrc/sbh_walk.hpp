@@ -327,12 +327,6 @@ __device__ __forceinline__ void node_accumulate(const TreeView& t, uint32_t v, f
   cnt_nodes += leaf ? 0 : 1;
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // This wavefront's 64 LDS slots for staging the children of a path node.
 struct SamplerLds {
   float4* src;  // term source: node moment, or the point of a one-point leaf

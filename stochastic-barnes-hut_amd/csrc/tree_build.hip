@@ -28,6 +28,7 @@ namespace {
 
 constexpr int kSmallMax = 4096;
 constexpr int kSmallThreads = 256;
+constexpr int kTinyMax = 512;  // nodes partitioned by one wavefront each
 constexpr int kChunk = 1024;
 constexpr int kChunkThreads = 256;
 constexpr int kMaxCells = 512;  // 3D with split_log 3
@@ -150,13 +151,17 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
 // ---------------------------------------------------------------------------
 // Level bookkeeping
 // ---------------------------------------------------------------------------
+// Internal nodes of the level by size: tiny (<= kTinyMax points, one
+// wavefront each), small (<= kSmallMax, one workgroup each) and large;
+// counters = {small, large, tiny}.
 __global__ void k_classify(uint32_t lb, uint32_t count, const uint32_t* n_start,
                            const uint32_t* n_end, const int32_t* n_depth, int leaf_size,
                            int max_depth, uint32_t* n_nchild, uint32_t* n_first,
-                           uint32_t* small_list, uint32_t* large_list, uint32_t* counters) {
+                           uint32_t* small_list, uint32_t* large_list, uint32_t* tiny_list,
+                           uint32_t* counters) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
-  bool small = false, large = false;
+  bool small = false, large = false, tiny = false;
   uint32_t v = lb + i;
   if (i < count) {
     uint32_t cnt = n_end[v] - n_start[v];
@@ -164,21 +169,25 @@ __global__ void k_classify(uint32_t lb, uint32_t count, const uint32_t* n_start,
     bool leaf = n_depth[v] >= max_depth || cnt <= (uint32_t)leaf_size;
     n_nchild[v] = 0;
     n_first[v] = kEnd;
-    small = !leaf && cnt <= (uint32_t)kSmallMax;
+    tiny = !leaf && cnt <= (uint32_t)kTinyMax;
+    small = !leaf && cnt > (uint32_t)kTinyMax && cnt <= (uint32_t)kSmallMax;
     large = !leaf && cnt > (uint32_t)kSmallMax;
   }
   // one atomic per wavefront and list
-  const uint64_t ms = __ballot(small), ml = __ballot(large);
+  const uint64_t ms = __ballot(small), ml = __ballot(large), mt = __ballot(tiny);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t bs = 0, bl = 0;
+  uint32_t bs = 0, bl = 0, bt = 0;
   if (lane == 0) {
     if (ms) bs = atomicAdd(&counters[0], (uint32_t)__popcll(ms));
     if (ml) bl = atomicAdd(&counters[1], (uint32_t)__popcll(ml));
+    if (mt) bt = atomicAdd(&counters[2], (uint32_t)__popcll(mt));
   }
   bs = __shfl(bs, 0, 64);
   bl = __shfl(bl, 0, 64);
+  bt = __shfl(bt, 0, 64);
   if (small) small_list[bs + __popcll(ms & below)] = v;
   if (large) large_list[bl + __popcll(ml & below)] = v;
+  if (tiny) tiny_list[bt + __popcll(mt & below)] = v;
 }
 
 // Zone ranking inside one LDS-resident node: element at position p of [a, b)
@@ -206,11 +215,12 @@ __device__ uint32_t rank_zone(const uint16_t* dig, uint32_t a, uint32_t b, uint3
 }
 
 // One workgroup per internal node of at most kSmallMax points.
+// counters[0] = the number of small nodes in `list`.
 template <int D, typename V>
 __global__ __launch_bounds__(kSmallThreads) void k_partition_small(
-    const uint32_t* list, const uint32_t* n_start, const uint32_t* n_end, const V* n_lo,
-    const V* n_hi, uint32_t* perm, const V* src_pts, int k, uint32_t* n_nchild,
-    uint4* temp_child) {
+    const uint32_t* list, const uint32_t* counters, const uint32_t* n_start,
+    const uint32_t* n_end, const V* n_lo, const V* n_hi, uint32_t* perm, const V* src_pts, int k,
+    uint32_t* n_nchild, uint4* temp_child) {
   __shared__ uint32_t lperm[kSmallMax];
   __shared__ uint16_t ldig[kSmallMax];
   __shared__ uint16_t L[kSmallMax / 2];
@@ -219,6 +229,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_partition_small(
   __shared__ uint32_t wsum[kSmallThreads / 64];
 
   const uint32_t tid = threadIdx.x;
+  if (blockIdx.x >= counters[0]) return;  // uniform per block
   const uint32_t node = list[blockIdx.x];
   const uint32_t s0 = n_start[node];
   const uint32_t n = n_end[node] - s0;
@@ -263,6 +274,97 @@ __global__ __launch_bounds__(kSmallThreads) void k_partition_small(
   }
   for (uint32_t i = tid; i < n; i += kSmallThreads) perm[s0 + i] = lperm[i];
   if (tid == 0) n_nchild[node] = nch;
+}
+
+// One wavefront per internal node of at most kTinyMax points (counters[2]
+// of them in `list`), four wavefronts a block.  The same passes as
+// k_partition_small: for each non-empty cell c in order, the remainder
+// [s, n) is split at s + t (t = the cell's count) and the i-th point of
+// [s, s + t) outside c swaps with the i-th point of [s + t, n) in c counted
+// from the right.  Ranks come from wavefront ballots over 64-position slabs,
+// so a pass needs no block barrier.
+template <int D, typename V>
+__global__ __launch_bounds__(256) void k_partition_tiny(
+    const uint32_t* list, const uint32_t* counters, const uint32_t* n_start,
+    const uint32_t* n_end, const V* n_lo, const V* n_hi, uint32_t* perm, const V* src_pts, int k,
+    uint32_t* n_nchild, uint4* temp_child) {
+  constexpr int kWaves = 4;
+  __shared__ uint32_t s_perm[kWaves][kTinyMax];
+  __shared__ uint16_t s_dig[kWaves][kTinyMax];
+  __shared__ uint16_t s_L[kWaves][kTinyMax / 2];
+  __shared__ uint16_t s_R[kWaves][kTinyMax / 2];
+  __shared__ uint32_t s_hist[kWaves][kMaxCells];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t idx = blockIdx.x * kWaves + wave;
+  if (idx >= counters[2]) return;  // uniform per wavefront; no block barrier follows
+  uint32_t* lperm = s_perm[wave];
+  uint16_t* ldig = s_dig[wave];
+  uint16_t* L = s_L[wave];
+  uint16_t* R = s_R[wave];
+  uint32_t* hist = s_hist[wave];
+  const uint32_t node = list[idx];
+  const uint32_t s0 = n_start[node];
+  const uint32_t n = n_end[node] - s0;
+  const V lo = n_lo[node], hi = n_hi[node];
+  const uint32_t B = 1u << (D * k);
+
+  for (uint32_t i = lane; i < B; i += 64) hist[i] = 0;
+  wave_lds_sync();
+  for (uint32_t i = lane; i < n; i += 64) {
+    const uint32_t pi = perm[s0 + i];
+    lperm[i] = pi;
+    const uint32_t d = cell_of<D>(src_pts[pi], lo, hi, k);
+    ldig[i] = (uint16_t)d;
+    atomicAdd(&hist[d], 1u);
+  }
+  wave_lds_sync();
+
+  uint32_t s = 0, nch = 0;
+  for (uint32_t cb = 0; cb < B && s < n; cb += 64) {
+    const uint32_t h = cb + lane < B ? hist[cb + lane] : 0u;
+    uint64_t present = __ballot(h != 0);
+    while (present != 0 && s < n) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(present);
+      present &= present - 1;
+      const uint32_t c = cb + j;
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)h, (int)j);
+      if (lane == 0) temp_child[s0 + nch] = make_uint4(c, s0 + s, s0 + s + t, 0);
+      ++nch;
+      if (s + t < n) {
+        // positions of the misplaced points, left zone ascending, right zone ascending
+        uint32_t nl = 0, nr = 0;
+        for (uint32_t p0 = s; p0 < n; p0 += 64) {
+          const uint32_t p = p0 + lane;
+          const uint32_t d = p < n ? (uint32_t)ldig[p] : 0xFFFFFFFFu;
+          const bool fl = p < s + t && d != c;
+          const bool fr = p >= s + t && d == c;
+          const uint64_t ml = __ballot(fl), mr = __ballot(fr);
+          if (fl) L[nl + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(ml >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((uint32_t)ml, 0u))] = (uint16_t)p;
+          if (fr) R[nr + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(mr >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mr, 0u))] = (uint16_t)p;
+          nl += (uint32_t)__popcll(ml);
+          nr += (uint32_t)__popcll(mr);
+        }
+        wave_lds_sync();
+        // nl == nr: the i-th misplaced point on the left pairs with the i-th
+        // misplaced point on the right counted from the right
+        for (uint32_t i = lane; i < nl; i += 64) {
+          const uint32_t a = L[i], b = R[nl - 1 - i];
+          const uint32_t tp = lperm[a];
+          lperm[a] = lperm[b];
+          lperm[b] = tp;
+          const uint16_t td = ldig[a];
+          ldig[a] = ldig[b];
+          ldig[b] = td;
+        }
+        wave_lds_sync();
+      }
+      s += t;
+    }
+  }
+  for (uint32_t i = lane; i < n; i += 64) perm[s0 + i] = lperm[i];
+  if (lane == 0) n_nchild[node] = nch;
 }
 
 // ---------------------------------------------------------------------------
@@ -728,7 +830,28 @@ __global__ void k_dfs_scatter(uint32_t nn, const uint32_t* n_dfs, const uint32_t
   dfsL[d] = l;
 }
 
+template <int D, typename T, typename V>
+__global__ void k_pack_inputs(uint32_t n, const T* pts, const T* m, const T* nrm, V* out_pts,
+                              V* out_nrm, uint32_t* perm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T* p = pts + (size_t)i * D;
+  out_pts[i] = make_vec(p[0], p[1], D == 3 ? p[2] : (T)0, m[i]);
+  if (nrm) {
+    const T* q = nrm + (size_t)i * D;
+    out_nrm[i] = make_vec(q[0], q[1], D == 3 ? q[2] : (T)0, (T)0);
+  } else {
+    out_nrm[i] = make_vec((T)0, (T)0, (T)0, (T)0);
+  }
+  perm[i] = i;
+}
+
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+__global__ void k_zero_u32x4(uint32_t* p) { p[threadIdx.x] = 0; }
+__global__ void k_zero_u32s(uint32_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 
 template <int D, typename V, typename T>
 __global__ void k_set_root(uint32_t n, V lo, V hi, uint32_t* n_start, uint32_t* n_end,
@@ -814,6 +937,18 @@ void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+// Byte offsets of the pieces of one scratch allocation, each 256-byte aligned.
+struct ArenaPlan {
+  size_t end = 0;
+  template <typename T>
+  size_t add(size_t n) {
+    const size_t o = end;
+    end += (n * sizeof(T) + 255) & ~(size_t)255;
+    return o;
+  }
+  size_t bytes() const { return end ? end : 256; }
+};
+
 // The arrays a build of scalar type T writes: the fp32 set, or the float64
 // set of the double build.
 template <typename T>
@@ -851,24 +986,24 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
   DevBuf<V>& n_hi = A::hi(t);
   DevBuf<T>& n_diam = A::diam(t);
 
-  // Inputs as {x, y, z, m} / {nx, ny, nz, 0} rows in input order.
-  std::vector<V> hp(N), hn(N);
-  for (uint32_t i = 0; i < N; ++i) {
-    hp[i] = make_vec(points[(size_t)i * D], points[(size_t)i * D + 1],
-                     D == 3 ? points[(size_t)i * D + 2] : (T)0, masses[i]);
-    if (normals)
-      hn[i] = make_vec(normals[(size_t)i * D], normals[(size_t)i * D + 1],
-                       D == 3 ? normals[(size_t)i * D + 2] : (T)0, (T)0);
-    else
-      hn[i] = make_vec((T)0, (T)0, (T)0, (T)0);
-  }
+  // Inputs as {x, y, z, m} / {nx, ny, nz, 0} rows in input order, packed on
+  // the device from the caller's row-major arrays; perm starts as the identity.
   DevBuf<V> src_pts(N), src_nrm(N);
-  h2d(src_pts.get(), hp.data(), N, s);
-  h2d(src_nrm.get(), hn.data(), N, s);
-  std::vector<uint32_t> iota(N);
-  for (uint32_t i = 0; i < N; ++i) iota[i] = i;
   t.perm.alloc(N);
-  h2d(t.perm.get(), iota.data(), N, s);
+  {
+    DevBuf<T> raw((size_t)N * (2 * D + 1));
+    T* r_pts = raw.get();
+    T* r_m = r_pts + (size_t)N * D;
+    T* r_nrm = r_m + N;
+    h2d(r_pts, points, (size_t)N * D, s);
+    h2d(r_m, masses, N, s);
+    if (normals) h2d(r_nrm, normals, (size_t)N * D, s);
+    k_pack_inputs<D><<<blocks_for(N, 256), 256, 0, s>>>(N, r_pts, r_m, normals ? r_nrm : nullptr,
+                                                        src_pts.get(), src_nrm.get(),
+                                                        t.perm.get());
+    SBH_HIP(hipGetLastError());
+    SBH_HIP(hipStreamSynchronize(s));  // raw is freed next
+  }
 
   // Node arrays with growable capacity.
   size_t cap = std::max<size_t>(16, (size_t)N * 2 + 2);
@@ -899,13 +1034,40 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
     cap = nc;
   };
 
-  // Scratch.
-  DevBuf<uint32_t> small_list(N + 1), large_list(N / kSmallMax + 2), counters(4);
-  DevBuf<uint4> temp_child(N + 1);
-  DevBuf<uint32_t> cbase(N + 1), total_dev(1);
-  DevBuf<uint32_t> scan_scratch(scan_scratch_size(N + 1));
-  DevBuf<uint16_t> digit(N + 1);
-  DevBuf<uint32_t> Lg(N + 1), Rg(N + 1);
+  // Scratch: one allocation for the whole build, carved into the level
+  // lists, the partition work arrays and the large nodes' tables at their
+  // largest sizes (a level's large nodes hold more than kSmallMax points each
+  // and disjoint ranges, so there are at most N / kSmallMax of them, and
+  // their chunks number at most N / kChunk plus one per node).
+  const size_t max_large = (size_t)N / kSmallMax + 2;
+  const size_t max_chunks = (size_t)N / kChunk + max_large;
+  ArenaPlan plan;
+  const size_t o_small = plan.add<uint32_t>(N + 1), o_tiny = plan.add<uint32_t>(N + 1),
+               o_large = plan.add<uint32_t>(max_large),
+               o_counters = plan.add<uint32_t>(4), o_temp = plan.add<uint4>(N + 1),
+               o_cbase = plan.add<uint32_t>(N + 1), o_total = plan.add<uint32_t>(1),
+               o_scan = plan.add<uint32_t>(scan_scratch_size(N + 1)),
+               o_digit = plan.add<uint16_t>(N + 1), o_lg = plan.add<uint32_t>(N + 1),
+               o_rg = plan.add<uint32_t>(N + 1), o_lrange = plan.add<uint32_t>(3 * max_large),
+               o_tables = plan.add<uint32_t>(4 * max_chunks + 4 * max_large),
+               o_hist = plan.add<uint32_t>(max_large * B),
+               o_cellstart = plan.add<uint32_t>(max_large * B),
+               o_m = plan.add<uint32_t>(max_large), o_ccnt = plan.add<uint2>(max_chunks);
+  DevBuf<uint8_t> arena(plan.bytes());
+  uint8_t* ab = arena.get();
+  uint32_t* small_list = (uint32_t*)(ab + o_small);
+  uint32_t* tiny_list = (uint32_t*)(ab + o_tiny);
+  uint32_t* large_list = (uint32_t*)(ab + o_large);
+  uint32_t* counters = (uint32_t*)(ab + o_counters);
+  uint4* temp_child = (uint4*)(ab + o_temp);
+  uint32_t* cbase = (uint32_t*)(ab + o_cbase);
+  uint32_t* total_dev = (uint32_t*)(ab + o_total);
+  uint32_t* scan_scratch = (uint32_t*)(ab + o_scan);
+  uint16_t* digit = (uint16_t*)(ab + o_digit);
+  uint32_t* Lg = (uint32_t*)(ab + o_lg);
+  uint32_t* Rg = (uint32_t*)(ab + o_rg);
+  uint32_t* d_lrange = (uint32_t*)(ab + o_lrange);
+  uint32_t* d_tables = (uint32_t*)(ab + o_tables);
 
   const T* tlo = A::root_lo(t);
   const T* thi = A::root_hi(t);
@@ -918,77 +1080,91 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
 
   t.level_begin.clear();
   uint32_t lb = 0, lcount = 1, num_nodes = 1;
-  std::vector<uint32_t> h_large, h_lstart, h_lend, h_chunk0, h_nchunk, h_hist;
-  std::vector<uint4> h_chunks;
+  std::vector<uint32_t> h_large, h_lstart, h_lend, h_tables, h_hist;
+  // Once a level has no node of more than kSmallMax points, no later level
+  // has one (children hold subsets of their parent's points): from then on
+  // the small-node partition runs over an upper bound of blocks that read the
+  // level's count on the device, and the classification is not read back.
+  bool large_levels = true;
   while (lcount > 0) {
     t.level_begin.push_back(lb);
-    SBH_HIP(hipMemsetAsync(counters.get(), 0, 4 * sizeof(uint32_t), s));
+    k_zero_u32x4<<<1, 4, 0, s>>>(counters);
     k_classify<<<blocks_for(lcount, 256), 256, 0, s>>>(
         lb, lcount, t.n_start.get(), t.n_end.get(), t.n_depth.get(), t.leaf_size, t.max_depth,
-        t.n_nchild.get(), t.n_first.get(), small_list.get(), large_list.get(), counters.get());
+        t.n_nchild.get(), t.n_first.get(), small_list, large_list, tiny_list, counters);
     SBH_HIP(hipGetLastError());
-    uint32_t hc[4];
-    d2h(hc, counters.get(), 4, s);
-    const uint32_t nsmall = hc[0], nlarge = hc[1];
+    uint32_t nsmall = lcount, nlarge = 0, ntiny = lcount;  // upper bounds when not read back
+    if (large_levels) {
+      uint32_t hc[4];
+      d2h(hc, counters, 4, s);
+      nsmall = hc[0];
+      nlarge = hc[1];
+      ntiny = hc[2];
+      large_levels = nlarge > 0;
+    }
 
     if (nlarge > 0) {
       // the large nodes and their point ranges, gathered into one copy
-      DevBuf<uint32_t> d_lrange((size_t)nlarge * 3);
-      k_gather_large<<<blocks_for(nlarge, 64), 64, 0, s>>>(large_list.get(), nlarge,
+      k_gather_large<<<blocks_for(nlarge, 64), 64, 0, s>>>(large_list, nlarge,
                                                           t.n_start.get(), t.n_end.get(),
-                                                          d_lrange.get());
+                                                          d_lrange);
       SBH_HIP(hipGetLastError());
       std::vector<uint32_t> h_lrange((size_t)nlarge * 3);
-      d2h(h_lrange.data(), d_lrange.get(), (size_t)nlarge * 3, s);
+      d2h(h_lrange.data(), d_lrange, (size_t)nlarge * 3, s);
       h_large.assign(h_lrange.begin(), h_lrange.begin() + nlarge);
       h_lstart.assign(h_lrange.begin() + nlarge, h_lrange.begin() + 2 * nlarge);
       h_lend.assign(h_lrange.begin() + 2 * nlarge, h_lrange.end());
-      h_chunks.clear();
-      h_chunk0.resize(nlarge);
-      h_nchunk.resize(nlarge);
+      // the chunk table {large index, begin, end, node} and the per-node
+      // starts, ends, first chunks and chunk counts, in one copy
+      uint32_t nchunks = 0;
+      for (uint32_t li = 0; li < nlarge; ++li)
+        nchunks += (h_lend[li] - h_lstart[li] + kChunk - 1) / kChunk;
+      SBH_REQUIRE(nlarge <= max_large && nchunks <= max_chunks, "large-node tables overflow");
+      h_tables.assign(4 * (size_t)nchunks + 4 * (size_t)nlarge, 0u);
+      uint32_t* hc4 = h_tables.data();
+      uint32_t* hls = hc4 + 4 * (size_t)nchunks;
+      uint32_t *hle = hls + nlarge, *hc0 = hle + nlarge, *hnc = hc0 + nlarge;
+      uint32_t ci = 0;
       for (uint32_t li = 0; li < nlarge; ++li) {
-        h_chunk0[li] = (uint32_t)h_chunks.size();
-        for (uint32_t p = h_lstart[li]; p < h_lend[li]; p += kChunk)
-          h_chunks.push_back(make_uint4(li, p, std::min(p + kChunk, h_lend[li]), h_large[li]));
-        h_nchunk[li] = (uint32_t)h_chunks.size() - h_chunk0[li];
+        hls[li] = h_lstart[li];
+        hle[li] = h_lend[li];
+        hc0[li] = ci;
+        for (uint32_t p = h_lstart[li]; p < h_lend[li]; p += kChunk, ++ci) {
+          hc4[4 * ci] = li;
+          hc4[4 * ci + 1] = p;
+          hc4[4 * ci + 2] = std::min(p + (uint32_t)kChunk, h_lend[li]);
+          hc4[4 * ci + 3] = h_large[li];
+        }
+        hnc[li] = ci - hc0[li];
       }
-      const uint32_t nchunks = (uint32_t)h_chunks.size();
-      DevBuf<uint4> d_chunks(nchunks);
-      DevBuf<uint32_t> d_lstart(nlarge), d_lend(nlarge), d_chunk0(nlarge), d_nchunk(nlarge);
-      DevBuf<uint32_t> d_hist((size_t)nlarge * B), d_cellstart((size_t)nlarge * B),
-          d_m(nlarge);
-      DevBuf<uint2> d_ccnt(nchunks);
-      h2d(d_chunks.get(), h_chunks.data(), nchunks, s);
-      h2d(d_lstart.get(), h_lstart.data(), nlarge, s);
-      h2d(d_lend.get(), h_lend.data(), nlarge, s);
-      h2d(d_chunk0.get(), h_chunk0.data(), nlarge, s);
-      h2d(d_nchunk.get(), h_nchunk.data(), nlarge, s);
-      SBH_HIP(hipMemsetAsync(d_hist.get(), 0, d_hist.bytes(), s));
+      h2d(d_tables, h_tables.data(), h_tables.size(), s);
+      const uint32_t* dt = d_tables;
       LargeCtx cx;
-      cx.chunks = d_chunks.get();
-      cx.l_start = d_lstart.get();
-      cx.l_end = d_lend.get();
-      cx.hist = d_hist.get();
-      cx.cellstart = d_cellstart.get();
-      cx.mcount = d_m.get();
-      cx.ccnt = d_ccnt.get();
-      cx.l_chunk0 = d_chunk0.get();
-      cx.l_nchunk = d_nchunk.get();
-      cx.digit = digit.get();
-      cx.Lg = Lg.get();
-      cx.Rg = Rg.get();
+      cx.chunks = reinterpret_cast<const uint4*>(dt);
+      cx.l_start = dt + 4 * (size_t)nchunks;
+      cx.l_end = cx.l_start + nlarge;
+      cx.l_chunk0 = cx.l_end + nlarge;
+      cx.l_nchunk = cx.l_chunk0 + nlarge;
+      cx.hist = (uint32_t*)(ab + o_hist);
+      cx.cellstart = (uint32_t*)(ab + o_cellstart);
+      cx.mcount = (uint32_t*)(ab + o_m);
+      cx.ccnt = (uint2*)(ab + o_ccnt);
+      cx.digit = digit;
+      cx.Lg = Lg;
+      cx.Rg = Rg;
+      k_zero_u32s<<<blocks_for((size_t)nlarge * B, 256), 256, 0, s>>>(cx.hist, nlarge * B);
       k_large_digits<D><<<nchunks, kChunkThreads, 0, s>>>(cx, n_lo.get(), n_hi.get(),
                                                           t.perm.get(), src_pts.get(), k);
       SBH_HIP(hipGetLastError());
-      k_large_cells<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, large_list.get(), nlarge, B,
-                                                         t.n_nchild.get(), temp_child.get());
+      k_large_cells<<<blocks_for(nlarge, 64), 64, 0, s>>>(cx, large_list, nlarge, B,
+                                                         t.n_nchild.get(), temp_child);
       SBH_HIP(hipGetLastError());
       // The pass of cell c moves a node's points of cell c to the front of
       // its unsorted remainder.  It changes nothing in a node whose cell c is
       // empty or is its last non-empty cell (the remainder then holds only
       // cell c), so only passes some large node needs are launched.
       h_hist.resize((size_t)nlarge * B);
-      d2h(h_hist.data(), d_hist.get(), (size_t)nlarge * B, s);
+      d2h(h_hist.data(), cx.hist, (size_t)nlarge * B, s);
       std::vector<uint8_t> need(B, 0);
       for (uint32_t li = 0; li < nlarge; ++li) {
         const uint32_t* h = h_hist.data() + (size_t)li * B;
@@ -1003,23 +1179,29 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
         k_large_swap<<<nchunks, kChunkThreads, 0, s>>>(cx, B, c, t.perm.get());
       }
       SBH_HIP(hipGetLastError());
-      SBH_HIP(hipStreamSynchronize(s));  // scratch buffers of this scope are freed next
     }
     if (nsmall > 0) {
+      // blocks past the level's small-node count (on the device) exit at once
       k_partition_small<D><<<nsmall, kSmallThreads, 0, s>>>(
-          small_list.get(), t.n_start.get(), t.n_end.get(), n_lo.get(), n_hi.get(),
-          t.perm.get(), src_pts.get(), k, t.n_nchild.get(), temp_child.get());
+          small_list, counters, t.n_start.get(), t.n_end.get(), n_lo.get(), n_hi.get(),
+          t.perm.get(), src_pts.get(), k, t.n_nchild.get(), temp_child);
+      SBH_HIP(hipGetLastError());
+    }
+    if (ntiny > 0) {
+      k_partition_tiny<D><<<blocks_for(ntiny, 4), 256, 0, s>>>(
+          tiny_list, counters, t.n_start.get(), t.n_end.get(), n_lo.get(), n_hi.get(),
+          t.perm.get(), src_pts.get(), k, t.n_nchild.get(), temp_child);
       SBH_HIP(hipGetLastError());
     }
     // Breadth-first numbering of the next level.
-    scan_rec(t.n_nchild.get() + lb, cbase.get(), lcount, total_dev.get(), scan_scratch.get(), s);
+    scan_rec(t.n_nchild.get() + lb, cbase, lcount, total_dev, scan_scratch, s);
     uint32_t nnext = 0;
-    d2h(&nnext, total_dev.get(), 1, s);
+    d2h(&nnext, total_dev, 1, s);
     grow_all((size_t)num_nodes + nnext, num_nodes);
     k_emit<D><<<blocks_for(lcount, 128), 128, 0, s>>>(
-        lb, lcount, num_nodes, cbase.get(), k, t.n_start.get(), t.n_end.get(), t.n_parent.get(),
+        lb, lcount, num_nodes, cbase, k, t.n_start.get(), t.n_end.get(), t.n_parent.get(),
         t.n_first.get(), t.n_nchild.get(), t.n_next.get(), t.n_depth.get(), n_lo.get(),
-        n_hi.get(), n_diam.get(), temp_child.get());
+        n_hi.get(), n_diam.get(), temp_child);
     SBH_HIP(hipGetLastError());
     lb = num_nodes;
     lcount = nnext;
