@@ -896,6 +896,8 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
     const uint32_t f0 = uniform_u32(Sl.y), nch0 = uniform_u32(Sl.w);
     uint32_t c0_start, pt0, nd0;
     const float r0sum = children_sum(f0, nch0, &c0_start, &pt0, &nd0);
+    const LaneMask kids0 = __ballot(lane < nch0);
+    const float beta0 = fmaxf(fmaxf(1.0f, cS), cS);  // the roulette's starting beta
     const float wd0 = level_weight<EXACT>((float)nS, 1.0f, (float)nS,
                                           r0sum - term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z));
     if constexpr (WORK) wterms += live ? (int32_t)(pt0 + nd0 + 1) : 0;
@@ -951,12 +953,12 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           take(0, cur, cur_cnt, curL, curC, curN);
         } else {
           const uint32_t below0 =
-              (uint32_t)__popcll(__ballot(lane < nch0) & __ballot(c0_start <= idx));
+              (uint32_t)__popcll(kids0 & __ballot(c0_start <= idx));
           step(uniform_u32(f0 + below0 - 1), 0, cur, cur_cnt, curL, curC, curN);
         }
         float event_prob = 1.0f;
         float wd_all = live ? 0.0f + wd0 : 0.0f;
-        float cur_beta = fmaxf(fmaxf(1.0f, cS), cS);
+        float cur_beta = beta0;
         LaneMask alive = live_m;
         if constexpr (REF) {
           cnt += live ? (int32_t)pt0 : 0;
@@ -1035,7 +1037,7 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
             cnt += lane_in(alive) ? (int32_t)pt : 0;
             cnt_nodes += lane_in(alive) ? (int32_t)nd : 0;
           }
-          cur_beta = fmaxf(cur_beta, ffr);
+          cur_beta = ffr > cur_beta ? ffr : cur_beta;  // max, as a compare and select
           if (known)
             take(slot + 1, cur, cur_cnt, curL, curC, curN);
           else
