@@ -86,6 +86,28 @@ def test_tree_bit_exact(sbh_gpu, name, dim, k, leaf, n):
     assert g.info()["depth"] == o.depth
 
 
+@pytest.mark.parametrize("dim,k,leaf", [(3, 1, 1), (3, 2, 1), (3, 1, 8), (2, 2, 1)])
+def test_tree_partition_size_classes(sbh_gpu, dim, k, leaf):
+    """Nodes of exactly 63 ... 1000 points: the sizes around the wavefront
+    partition's 64-position slabs and its 512-point limit (above it a
+    workgroup partitions the node).  Each cluster fills one cell of the
+    4-per-axis grid alone, so some node holds exactly its points."""
+    rng = np.random.default_rng(17 + dim + k + leaf)
+    sizes = [63, 64, 65, 127, 128, 129, 511, 512, 513, 514, 1000]
+    cells = rng.permutation(4 ** dim)[: len(sizes)]
+    pts = []
+    for s, c in zip(sizes, cells):
+        idx = np.array([(c >> (2 * a)) & 3 for a in range(dim)])
+        pts.append(0.125 + 0.25 * idx + 0.01 * (rng.random((s, dim)) - 0.5))
+    pts = np.concatenate(pts)
+    m = rng.uniform(0.5, 1.5, len(pts))
+    g, o = make_pair(sbh_gpu, pts, np.zeros_like(pts), m, np.zeros(dim), np.ones(dim), k,
+                     leaf_size=leaf)
+    counts = o.nodes()["end"] - o.nodes()["start"]
+    assert set(sizes) <= set(counts.tolist())
+    assert_tree_equal(g, o)
+
+
 def test_tree_duplicates_and_max_depth(sbh_gpu):
     """Coincident points never separate: leaves hit max_depth=100 and hold
     several points, whose internal order comes from the partition passes."""
