@@ -281,7 +281,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
   __shared__ uint2 sRec[kThreads];
   __shared__ uint2 sKeys[kWaves * chunk_for<K>()];
-  __shared__ float sDiff[kWaves * kCacheLevels * 64];
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t wbase = threadIdx.x & ~63u;
@@ -292,7 +291,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
   const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
-                      sKeys + wave * chunk_for<K>(), sDiff + wave * kCacheLevels * 64,
+                      sKeys + wave * chunk_for<K>(),
                       sNext + wave * (kCacheLevels + 1) * 3};
   const LaneResult r = stochastic_estimate_sorted<D, K, COUNT, false, WORK>(t, spl, key0, key1,
                                                                           g, live, x, y, z, lds);
@@ -331,7 +330,6 @@ k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
   __shared__ uint2 sRec[kThreads];
   __shared__ uint2 sKeys[kWaves * chunk_for<K>()];
-  __shared__ float sDiff[kWaves * kCacheLevels * 64];
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t wbase = threadIdx.x & ~63u;
@@ -342,7 +340,7 @@ k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
   const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
-                      sKeys + wave * chunk_for<K>(), sDiff + wave * kCacheLevels * 64,
+                      sKeys + wave * chunk_for<K>(),
                       sNext + wave * (kCacheLevels + 1) * 3};
   float bh = 0.0f;
   const LaneResult r = stochastic_estimate_sorted<D, K, COUNT, true>(

@@ -667,7 +667,6 @@ struct SortedLds {
   uint2* rec;         // 64 staging slots: children's {first point, leaf flag | count}
   uint2* keys;        // kChunk samples: {sampled point, level-1 roulette word with the
                       // sample's index in the chunk in its low 8 bits}
-  float* cdiff;       // kCacheLevels x 64: diff of the cached node, per lane
   uint4* nrec;        // (kCacheLevels + 1) x 3: step records (node and walk record,
                       // centroid, normal)
 };
@@ -752,6 +751,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
   // point terms and lane 2 kCacheF + l its node terms (read back with
   // readlane); the step records' ranges follow (lanes 3 kCacheF + r, 4 kCacheF + r)
   uint32_t cinfo = lane < kCacheLevels ? kNoNode : 0u;
+  // per cached path level, per lane: the cached node's opening probability,
+  // its weight (the level's term of a sample through it) and the running
+  // beta after it; registers, indexed only by compile-time levels
+  float c_tp[kCacheLevels], c_wd[kCacheLevels], c_beta[kCacheLevels];
 
   // Evaluates the children of v (at most 64) for every live lane: returns
   // their sum; *start = this lane's child's start (~0 past the last child);
@@ -956,7 +959,9 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
               (uint32_t)__popcll(kids0 & __ballot(c0_start <= idx));
           step(uniform_u32(f0 + below0 - 1), 0, cur, cur_cnt, curL, curC, curN);
         }
-        float event_prob = 1.0f;
+        // epc: the product of the opening probabilities along the path (the
+        // event probability of a lane still on it)
+        float epc = 1.0f;
         float wd_all = live ? 0.0f + wd0 : 0.0f;
         float cur_beta = beta0;
         LaneMask alive = live_m;
@@ -968,7 +973,13 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         uint32_t blk_id = 0;
         uint32_t level = 1;
         if (curL.w & kLeafBit) l1_state = kL1Leaf;
-        while (!(curL.w & kLeafBit)) {
+        // One path level.  SLOT >= 0: path level SLOT + 1, whose node may be
+        // the cached one (its opening probability, weight and running beta
+        // are then read back instead of computed: they depend only on the
+        // node and the query); SLOT < 0: an uncached deeper level.  Returns
+        // false when the sample's path ends.
+        auto path_level = [&]<int SLOT>() -> bool {
+          if (curL.w & kLeafBit) return false;
           const uint32_t wi = level + 2;
           if ((wi >> 2) != blk_id) {
             blk_id = wi >> 2;
@@ -977,72 +988,85 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           const uint32_t wsel = wi & 3;
           const uint32_t word = wsel == 0 ? blk.x : wsel == 1 ? blk.y : wsel == 2 ? blk.z : blk.w;
           const float p = rr_uniform(word);
-          const uint32_t slot = level - 1;
-          const uint32_t cslot = min(slot, kCacheLevels - 1) * 64 + lane;  // in-bounds address
-          const bool hit = slot < kCacheLevels &&
-                           (uint32_t)__builtin_amdgcn_readlane(
-                               (int)cinfo, (int)min(slot, kCacheLevels - 1)) == cur;
-          // the breadth-first records hold the diameter's hardware reciprocal
-          const float ffr = ratio_fast<D>(curC, __uint_as_float(curL.x), x, y, z);
+          bool hit = false;
+          if constexpr (SLOT >= 0)
+            hit = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, SLOT) == cur;
           ++level;
-          const float tp = traverse_prob<EXACT>(cur_beta, ffr);
-          if (slot == 0) {
-            l1_state = kL1Internal;
-            l1_tp = tp;
-          }
-          // roulette, as selects (no per-lane branches)
-          const LaneMask stop = __ballot(p > tp);
-          event_prob = lane_in(alive) ? event_prob * (lane_in(stop) ? 1.0f - tp : tp) : event_prob;
-          if constexpr (WORK) wlevels += lane_in(alive) ? 1 : 0;
-          alive &= ~stop;
-          if (alive == 0) break;
           const uint32_t f = uniform_u32(curL.y), nch = uniform_u32(curL.w);
-          float diff;
-          uint32_t below = 0, pt, nd;
+          uint32_t below = 0, pt = 0, nd = 0;
           bool known = false;  // the next node is the one recorded for this cached node
           if (hit) {
-            diff = lds.cdiff[cslot];
-            if constexpr (REF) {
-              pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLanePt + slot));
-              nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneNd + slot));
-            } else {
-              pt = nd = 0;
-            }
-            known = range_has(slot + 1, idx);
-            if (!known) {
-              const uint32_t cs = lane < nch ? n_start[f + lane] : 0xFFFFFFFFu;
-              below = (uint32_t)__popcll(__ballot(cs <= idx));
+            if constexpr (SLOT >= 0) {
+              const float tp = c_tp[SLOT];
+              const LaneMask stop = __ballot(p > tp);
+              if constexpr (WORK) wlevels += lane_in(alive) ? 1 : 0;
+              alive &= ~stop;
+              if (alive == 0) return false;
+              epc = epc * tp;
+              cur_beta = c_beta[SLOT];
+              wd_all = lane_in(alive) ? wd_all + c_wd[SLOT] : wd_all;
+              if constexpr (REF) {
+                pt = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLanePt + SLOT));
+                nd = (uint32_t)__builtin_amdgcn_readlane((int)cinfo, (int)(kLaneNd + SLOT));
+              }
+              known = range_has(SLOT + 1, idx);
+              if (!known) {
+                const uint32_t cs = lane < nch ? n_start[f + lane] : 0xFFFFFFFFu;
+                below = (uint32_t)__popcll(__ballot(cs <= idx));
+              }
             }
           } else {
+            // the breadth-first records hold the diameter's hardware reciprocal
+            const float ffr = ratio_fast<D>(curC, __uint_as_float(curL.x), x, y, z);
+            const float tp = traverse_prob<EXACT>(cur_beta, ffr);
+            if (SLOT == 0) {
+              l1_state = kL1Internal;
+              l1_tp = tp;
+            }
+            const LaneMask stop = __ballot(p > tp);
+            if constexpr (WORK) wlevels += lane_in(alive) ? 1 : 0;
+            alive &= ~stop;
+            if (alive == 0) return false;
+            epc = epc * tp;
             uint32_t cs;
             const float r = children_sum(f, nch, &cs, &pt, &nd);
-            diff = r - term<D, K, EXACT>(curC, curN, alpha, alpha_k, x, y, z);
+            const float diff = r - term<D, K, EXACT>(curC, curN, alpha, alpha_k, x, y, z);
             if constexpr (WORK) wterms += lane_in(alive) ? (int32_t)(pt + nd + 1) : 0;
             below = (uint32_t)__popcll(__ballot(cs <= idx));
-            if (slot < kCacheLevels) {
-              // each lane reads back only its own diff: no wavefront sync
-              lds.cdiff[cslot] = diff;
+            const float wd = level_weight<EXACT>((float)nS, epc, (float)cur_cnt, diff);
+            wd_all = lane_in(alive) ? wd_all + wd : wd_all;
+            cur_beta = ffr > cur_beta ? ffr : cur_beta;  // max, as a compare and select
+            if constexpr (SLOT >= 0) {
+              c_tp[SLOT] = tp;
+              c_wd[SLOT] = wd;
+              c_beta[SLOT] = cur_beta;
               if constexpr (REF)
-                cinfo = lane == slot            ? cur
-                        : lane == kLanePt + slot ? pt
-                        : lane == kLaneNd + slot ? nd
+                cinfo = lane == SLOT            ? cur
+                        : lane == kLanePt + SLOT ? pt
+                        : lane == kLaneNd + SLOT ? nd
                                                  : cinfo;
               else
-                cinfo = lane == slot ? cur : cinfo;
+                cinfo = lane == SLOT ? cur : cinfo;
             }
           }
-          const float wd = level_weight<EXACT>((float)nS, event_prob, (float)cur_cnt, diff);
-          wd_all = lane_in(alive) ? wd_all + wd : wd_all;
           if constexpr (REF) {
             cnt += lane_in(alive) ? (int32_t)pt : 0;
             cnt_nodes += lane_in(alive) ? (int32_t)nd : 0;
           }
-          cur_beta = ffr > cur_beta ? ffr : cur_beta;  // max, as a compare and select
           if (known)
-            take(slot + 1, cur, cur_cnt, curL, curC, curN);
+            take(SLOT + 1, cur, cur_cnt, curL, curC, curN);
           else
-            step(uniform_u32(f + below - 1), slot < kCacheLevels ? slot + 1 : kNoRecord, cur,
-                 cur_cnt, curL, curC, curN);
+            step(uniform_u32(f + below - 1), SLOT >= 0 ? SLOT + 1 : kNoRecord, cur, cur_cnt,
+                 curL, curC, curN);
+          return true;
+        };
+        static_assert(kCacheLevels == 7, "the cached path levels are unrolled below");
+        if (path_level.template operator()<0>() && path_level.template operator()<1>() &&
+            path_level.template operator()<2>() && path_level.template operator()<3>() &&
+            path_level.template operator()<4>() && path_level.template operator()<5>() &&
+            path_level.template operator()<6>()) {
+          while (path_level.template operator()<-1>()) {
+          }
         }
         sub_total += wd_all;
       }
