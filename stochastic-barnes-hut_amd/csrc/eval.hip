@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void k_stochastic(TreeView t, int spl, ui
                                                           int32_t* __restrict__ counts,
                                                           int split_counts) {
   __shared__ float4 sSrc[kThreads];
-  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
+  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 1];  // normals: winding number only
   __shared__ uint4 sRec[kThreads];
   __shared__ uint2 sAux[kThreads];
   const uint32_t wbase = threadIdx.x & ~63u;
@@ -278,8 +278,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
     int split_counts) {
   constexpr uint32_t kWaves = kThreads / 64;
   __shared__ float4 sSrc[kThreads];
-  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
-  __shared__ uint2 sRec[kThreads];
+  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 1];  // normals: winding number only
   __shared__ uint2 sKeys[kWaves * chunk_for<K>()];
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t wave = threadIdx.x >> 6;
@@ -290,7 +289,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
+  const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0),
                       sKeys + wave * chunk_for<K>(),
                       sNext + wave * (kCacheLevels + 1) * 3};
   const LaneResult r = stochastic_estimate_sorted<D, K, COUNT, false, WORK>(t, spl, key0, key1,
@@ -327,8 +326,7 @@ k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
                           int2* __restrict__ pcnt) {
   constexpr uint32_t kWaves = kThreads / 64;
   __shared__ float4 sSrc[kThreads];
-  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 64];
-  __shared__ uint2 sRec[kThreads];
+  __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 1];  // normals: winding number only
   __shared__ uint2 sKeys[kWaves * chunk_for<K>()];
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t wave = threadIdx.x >> 6;
@@ -339,7 +337,7 @@ k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0), sRec + wbase,
+  const SortedLds lds{sSrc + wbase, sNrm + (K == WINDING_NUMBER ? wbase : 0),
                       sKeys + wave * chunk_for<K>(),
                       sNext + wave * (kCacheLevels + 1) * 3};
   float bh = 0.0f;

@@ -644,9 +644,13 @@ __device__ __forceinline__ LaneResult stochastic_estimate(const TreeView& t, int
 // without reading the tree.  Only the order in which the samples'
 // contributions are summed differs from the per-sample loop.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kChunk = 256;        // samples drawn and sorted together (at most)
-// The winding-number kernels stage normals too; half-size chunks keep their
-// LDS within seven workgroups per CU.
+constexpr uint32_t kChunk = 512;        // samples drawn and sorted together (at most)
+// A 512-sample chunk's keys pack {point << 9 | sample, roulette word}, which
+// needs point indices below 2^23; in larger trees chunks hold 256 samples and
+// the sample index rides in the roulette word's low 8 bits (which the
+// roulette does not use).  The winding-number kernels stage normals too;
+// half-size chunks keep their LDS within seven workgroups per CU.
+constexpr uint32_t kPackedPointLimit = 1u << 23;
 template <int K>
 constexpr uint32_t chunk_for() {
   return K == WINDING_NUMBER ? kChunk / 2 : kChunk;
@@ -664,9 +668,9 @@ static_assert(5 * kCacheF <= 64, "the cache register's fields fit in 64 lanes");
 struct SortedLds {
   float4* src;        // 64 staging slots: term sources
   float4* nrm;        // 64 staging slots: normals (winding number only)
-  uint2* rec;         // 64 staging slots: children's {first point, leaf flag | count}
-  uint2* keys;        // kChunk samples: {sampled point, level-1 roulette word with the
-                      // sample's index in the chunk in its low 8 bits}
+  uint2* keys;        // kChunk samples: {sampled point << 9 | sample, level-1 roulette
+                      // word}, or {sampled point, roulette word with the sample in its
+                      // low 8 bits}
   uint4* nrec;        // (kCacheLevels + 1) x 3: step records (node and walk record,
                       // centroid, normal)
 };
@@ -771,7 +775,6 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       lds.src[lane] = kidC[c];
       if (kWN) lds.nrm[lane] = kidN[c];
       if (l.w & kLeafBit) np_lane = l.w & ~kLeafBit;
-      lds.rec[lane] = make_uint2(l.y, l.w);
     }
     const uint64_t multi = __ballot(np_lane > 1);
     wave_lds_sync();
@@ -787,11 +790,13 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
           r = accum<D, K, EXACT>(r, src, nv, alpha, alpha_k, x, y, z);
         }
       } else {
+        // some child is a leaf of several points (rare: coincident points
+        // or leaf_size > 1): its record is read again, as a scalar load
         for (uint32_t j = 0; j < nch; ++j) {
-          const uint2 l = lds.rec[j];
-          const uint32_t np = l.y & ~kLeafBit;
-          if ((l.y & kLeafBit) && np > 1) {
-            for (uint32_t pp = l.x; pp < l.x + np; ++pp)
+          const uint4 l = load_invariant(nodeL, f + j);
+          const uint32_t np = l.w & ~kLeafBit;
+          if ((l.w & kLeafBit) && np > 1) {
+            for (uint32_t pp = l.y; pp < l.y + np; ++pp)
               r = accum<D, K, EXACT>(r, pts[pp], nrm[pp], alpha, alpha_k, x, y, z);
           } else {
             const float4 src = lds.src[j];
@@ -905,12 +910,13 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
                                           r0sum - term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z));
     if constexpr (WORK) wterms += live ? (int32_t)(pt0 + nd0 + 1) : 0;
     float sub_total = 0.0f;
-    for (int s0 = 0; s0 < spl; s0 += (int)kCh) {
-      const uint32_t ncur = (uint32_t)min((int)kCh, spl - s0);
-      // draw the chunk's sampled points (Philox block 0 of each sample); the
-      // level-1 roulette uses only the word's top 24 bits (rr_uniform), so
-      // its low 8 bits carry the sample's index within the chunk
-      static_assert(kChunk <= 256, "the sample index must fit in 8 bits");
+    // the chunk size and key format of this subdomain (its point indices
+    // must fit 23 bits for packed keys)
+    const bool packed = kCh > 256 && sstart + nS <= kPackedPointLimit;
+    const uint32_t ch_size = packed ? kCh : (kCh < 256 ? kCh : 256u);
+    for (int s0 = 0; s0 < spl; s0 += (int)ch_size) {
+      const uint32_t ncur = (uint32_t)min((int)ch_size, spl - s0);
+      // draw the chunk's sampled points (Philox block 0 of each sample)
 #pragma unroll
       for (uint32_t m = 0; m < kCh / 64; ++m) {
         const uint32_t i = lane + 64 * m;
@@ -918,9 +924,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
         if (i < ncur) {
           const uint32_t ss = (uint32_t)s0 + i;
           const Philox4 r = philox4x32_10(ss, si, g, 0u, key0, key1);
-          key = make_uint2(sstart + scale_draw(r.x, r.y, nS), (r.w & 0xFFFFFF00u) | i);
+          const uint32_t pt = sstart + scale_draw(r.x, r.y, nS);
+          key = packed ? make_uint2((pt << 9) | i, r.w) : make_uint2(pt, (r.w & 0xFFFFFF00u) | i);
         }
-        lds.keys[i] = key;
+        if (i < ch_size) lds.keys[i] = key;
       }
       wave_lds_sync();
       // sort the drawn samples (and sentinels up to the next power of two)
@@ -929,9 +936,10 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
       wave_sort_keys<kCh>(lds.keys, nsort);
       for (uint32_t i = 0; i < ncur; ++i) {
         const uint2 key = lds.keys[i];
-        const uint32_t idx = uniform_u32(key.x);
+        const uint32_t kx = uniform_u32(key.x);
         const uint32_t w1 = uniform_u32(key.y);
-        const uint32_t ss = (uint32_t)s0 + (w1 & 0xFFu);
+        const uint32_t idx = packed ? kx >> 9 : kx;
+        const uint32_t ss = (uint32_t)s0 + (packed ? kx & 511u : w1 & 0xFFu);
         if (l1_state != kL1Unknown && range_has(0, idx)) {
           // the sample's level-1 node is record 0's: when no lane goes on
           // from it, the sample adds the level-0 weight alone, as the full

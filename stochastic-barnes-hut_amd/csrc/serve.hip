@@ -132,7 +132,6 @@ __global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(4
   __shared__ uint2 sAux[kServeThreads];
   constexpr uint32_t kWaves = kServeThreads / 64;
   __shared__ uint2 sKeys[kWaves * kChunk];
-  __shared__ uint2 sRec2[kServeThreads];
   __shared__ uint4 sNext[kWaves * (kCacheLevels + 1) * 3];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
@@ -155,7 +154,7 @@ __global__ __launch_bounds__(kServeThreads) __attribute__((amdgpu_waves_per_eu(4
     const bool stoch = variant & 1;
     const SamplerLds c{sSrc + wbase, sNrm + wbase, sRec + wbase, sAux + wbase};
     const uint32_t wave = threadIdx.x >> 6;
-    const SortedLds sc{sSrc + wbase, sNrm + wbase, sRec2 + wbase, sKeys + wave * kChunk,
+    const SortedLds sc{sSrc + wbase, sNrm + wbase, sKeys + wave * kChunk,
                        sNext + wave * (kCacheLevels + 1) * 3};
     const uint32_t sub = uniform_u32(tk.sub);
     switch (variant >> 1) {

@@ -202,6 +202,20 @@ def test_plummer_headline_shape_small(sbh_gpu):
                      no_split=True, sort_queries=True)
 
 
+def test_sorted_estimator_past_packed_keys(sbh_gpu):
+    """2^23 + 3001 points: the sorted estimator draws 512-sample chunks with
+    the sample index packed under the point index while a subdomain's points
+    lie below 2^23, and 256-sample chunks with the index in the roulette
+    word's low bits past it (the last subdomains here); both against the
+    oracle, in the whole-walk and the split launch."""
+    pts, nrm, m, q, lo, hi = _inputs((1 << 23) + 3001, 512, 3, 303, "gravity")
+    g, o = _trees(sbh_gpu, pts, nrm, m, lo, hi, 2, 1.0)
+    for spl in (300, 600):
+        check_stochastic(sbh_gpu, g, o, q, "gravity", spl, 23, f"2^23+3001 spl{spl}",
+                         no_split=True)
+    check_stochastic(sbh_gpu, g, o, q, "gravity", 300, 29, "2^23+3001 spl300 split")
+
+
 def test_work_counts(sbh_gpu):
     """SBH_EVAL_WORK_COUNTS (the instrumentation behind the benchmark's
     roofline): values unchanged; with one sample per subdomain nothing is
