@@ -501,7 +501,9 @@ FLOPS_PER_PATH_LEVEL = 22
 
 def estimator_roofline(head):
     """Roofline of the dominant kernel (the sorted-sample estimator launch,
-    timed alone with hipEvents on its stream): algorithmic flops = the kernel
+    split into tasks per 64-query block and range of root subdomains, and the
+    pass adding the ranges' parts, timed together with hipEvents on their
+    stream): algorithmic flops = the kernel
     terms it evaluates x flops per term + the path levels it steps x flops per
     level, counted by its work-count mode; peak = MI355X FP32 (157.3 TFLOP/s).
     The counters put it on VALU issue; HBM traffic is a few percent of peak."""
@@ -524,7 +526,8 @@ def estimator_roofline(head):
     return {
         "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-        "kernel": "k_stochastic_sorted<3, GRAVITY>", "kernel_ms": kms,
+        "kernel": "k_stochastic_sorted_split<3, GRAVITY> + k_reduce_stoch_split",
+        "kernel_ms": kms,
         "units": {"terms": w["terms"], "flops_per_term": F, "path_levels": w["path_levels"],
                   "flops_per_level": FLOPS_PER_PATH_LEVEL},
         "frac_non_packed_fp32": achieved / (FP32_PEAK_TFLOPS / 2),

@@ -201,12 +201,13 @@ int sbh_eval(sbh_tree* tree, int32_t method, int32_t kernel, const sbh_eval_para
  * query) and of split launches (partial sums) comes from one static array of
  * SBH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  An
  * evaluation that needs more fails with SBH_ERR_INVALID and a message giving
- * the size (e.g. sorted launches above about 2.5M queries; SBH_EVAL_NO_SPLIT
- * removes the partial sums).  Every call on one GPU shares that GPU's array:
+ * the size (e.g. sorted launches above about 10M queries, or a sorted
+ * estimator launch split over 12 subdomain ranges above about 2M;
+ * SBH_EVAL_NO_SPLIT removes the partial sums).  Every call on one GPU shares that GPU's array:
  * calls that overlap on one GPU (two trees on two threads or streams) race
  * and must be serialised by the caller.
  */
-#define SBH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define SBH_NOALLOC_SCRATCH_BYTES (256ull << 20)
 int sbh_eval_noalloc(sbh_tree* tree, int32_t method, int32_t kernel,
                      const sbh_eval_params* params, const float* queries, int64_t q,
                      float* results, int32_t* counts, float* kernel_ms);

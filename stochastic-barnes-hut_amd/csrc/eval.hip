@@ -310,20 +310,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))
   }
 }
 
-// The sorted-sample estimator for launches too small to fill the GPU: one
-// task per (64 queries, root subdomain) (blockIdx.y = subdomain), each
-// writing its Barnes-Hut and control-variate parts; k_reduce_stoch_split adds
-// the parts in subdomain order.  Same sample paths (keyed by the wavefront's
-// slot block and the subdomain) and terms; the fast terms' fused
-// accumulation rounds differently across the split, so values agree with
-// the whole estimate to fp32 rounding.
+// The sorted-sample estimator split into tasks: one per (64 queries, range
+// of root subdomains) (blockIdx.y = the range's task in sub_tasks, heaviest
+// first), each writing its Barnes-Hut and control-variate parts to the
+// range's slot; k_reduce_stoch_split adds the parts in subdomain order.  Same
+// sample paths (keyed by the wavefront's slot block and the subdomain) and
+// terms; the fast terms' fused accumulation rounds differently across the
+// split, so values agree with the whole estimate to fp32 rounding.
 template <int D, int K, bool COUNT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 7))) void
 k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
                           const float* __restrict__ qx, const float* __restrict__ qy,
                           const float* __restrict__ qz, uint32_t nq,
-                          const uint32_t* __restrict__ order, float2* __restrict__ partial,
-                          int2* __restrict__ pcnt) {
+                          const uint32_t* __restrict__ order, const uint4* __restrict__ tasks,
+                          float2* __restrict__ partial, int2* __restrict__ pcnt) {
   constexpr uint32_t kWaves = kThreads / 64;
   __shared__ float4 sSrc[kThreads];
   __shared__ float4 sNrm[K == WINDING_NUMBER ? kThreads : 1];  // normals: winding number only
@@ -341,10 +341,11 @@ k_stochastic_sorted_split(TreeView t, int spl, uint32_t key0, uint32_t key1,
                       sKeys + wave * chunk_for<K>(),
                       sNext + wave * (kCacheLevels + 1) * 3};
   float bh = 0.0f;
+  const uint4 tk = load_invariant(tasks, blockIdx.y);
   const LaneResult r = stochastic_estimate_sorted<D, K, COUNT, true>(
-      t, spl, key0, key1, g, live, x, y, z, lds, blockIdx.y, blockIdx.y + 1, &bh);
+      t, spl, key0, key1, g, live, x, y, z, lds, tk.x, tk.y, &bh);
   if (live) {
-    const size_t e = (size_t)blockIdx.y * nq + slot;
+    const size_t e = (size_t)tk.z * nq + slot;
     partial[e] = make_float2(bh, r.value);
     if constexpr (COUNT) pcnt[e] = make_int2(r.nodes, r.points);
   }
@@ -799,14 +800,16 @@ void launch_stoch(const DeviceTree& t, int spl, uint64_t seed, bool per_query, c
     float2* part = reinterpret_cast<float2*>(partial);
     int2* pcnt = reinterpret_cast<int2*>(part + (size_t)nsplit * nq);
     const dim3 grid(nblk(nq, kThreads), nsplit);
+    SBH_REQUIRE(nsplit == t.n_sub_tasks, "split estimator: task count changed");
+    const uint4* tasks = t.sub_tasks.get();
     if (counts)
       k_stochastic_sorted_split<D, K, true><<<grid, kThreads, 0, s>>>(
-          view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, part,
-          pcnt);
+          view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, tasks,
+          part, pcnt);
     else
       k_stochastic_sorted_split<D, K, false><<<grid, kThreads, 0, s>>>(
-          view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, part,
-          pcnt);
+          view_of(t), spl, (uint32_t)seed, (uint32_t)(seed >> 32), qx, qy, qz, nq, order, tasks,
+          part, pcnt);
     SBH_HIP(hipGetLastError());
     k_reduce_stoch_split<<<nblk(nq, 256), 256, 0, s>>>(part, pcnt, nsplit, nq, order, out,
                                                         counts, split);
@@ -949,13 +952,15 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
 }
 
 uint32_t stochastic_split_count(const DeviceTree& t, uint32_t nq, bool per_query, bool exact) {
-  // the sorted-sample estimator in a launch of fewer than four wavefronts per
-  // SIMD: one task per root subdomain, if there are between 2 and 64
-  if (exact || per_query || t.is_double || t.dim * t.split_log > 6 || t.level_begin.size() < 3)
+  // the sorted-sample estimator: one task per 64-query block and range of
+  // root subdomains holding one internal subdomain (the tree's sub_tasks).
+  // Smaller tasks, heaviest dispatched first, leave a shorter tail of
+  // wavefronts than whole walks, whose lengths vary with the queries'
+  // positions.
+  (void)nq;
+  if (exact || per_query || t.is_double || t.dim * t.split_log > 6 || t.n_sub_tasks < 2)
     return 1;
-  const uint32_t nsub = t.level_begin[2] - t.level_begin[1];
-  if (nsub < 2 || nsub > 64 || nq >= 4u * 1024u * 64u) return 1;
-  return nsub;
+  return t.n_sub_tasks;
 }
 
 void run_stochastic(const DeviceTree& t, int kernel, int spl, uint64_t seed, bool per_query,

@@ -49,8 +49,8 @@ void run_bh(const DeviceTree& t, int kernel, float beta, int group, const float*
 // per_query: one sample path per query keyed by its index (otherwise the 64
 // queries of a wavefront share their paths, keyed by the wavefront's index).
 // partial (optional): stochastic_split_count(...) * nq * 2 floats (and with
-// counts as many int32 again); launches too small to fill the GPU then run
-// one task per root subdomain.
+// counts as many int32 again); the sorted-sample estimator then runs one task
+// per 64-query block and range of root subdomains (DeviceTree::sub_tasks).
 uint32_t stochastic_split_count(const DeviceTree& t, uint32_t nq, bool per_query, bool exact);
 // work: counts (2 nq values) receive the work counts of the sorted-sample
 // estimator (kernel terms evaluated, then path levels stepped); never split.

@@ -156,6 +156,15 @@ struct DeviceTree {
   // walk_tasks[walk_frontiers[f].first, + .second).
   DevBuf<WalkTask> walk_tasks;
   std::vector<std::pair<uint32_t, uint32_t>> walk_frontiers;
+  // Tasks of the split sorted estimator (derived): the root's subdomains in
+  // ranges {first, end, part, 0} of at most one internal subdomain each (leaf
+  // and empty subdomains join the range before them, or the first range),
+  // listed heaviest first (points of the internal subdomain), the order a
+  // split launch dispatches them in; `part` is the range's position in
+  // subdomain order, the slot of its partial sums.  n_sub_tasks is 0 unless
+  // the root has 2 to 64 subdomains, two or more of them internal.
+  DevBuf<uint4> sub_tasks;
+  uint32_t n_sub_tasks = 0;
 
   // Double build (the reference's USE_DOUBLE configuration): points, cell
   // bounds, diameters and moments in float64 instead of the float arrays
@@ -182,6 +191,7 @@ struct DeviceTree {
            nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
            n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
            kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes() + walk_tasks.bytes() +
+           sub_tasks.bytes() +
            pts64.bytes() + nrm64.bytes() + n_lo64.bytes() + n_hi64.bytes() +
            n_diam64.bytes() + nodeC64.bytes() + nodeN64.bytes() + dfsR64.bytes() +
            dfsN64.bytes();
@@ -207,5 +217,8 @@ void derive_child_sources(DeviceTree& t);
 // The split frontiers of the Barnes-Hut walk (walk_tasks, walk_frontiers),
 // from host copies of the top levels; blocks on t.stream.
 void derive_walk_tasks(DeviceTree& t);
+// The split sorted estimator's tasks (sub_tasks), from host copies of the
+// root's subdomains; blocks on t.stream.
+void derive_subdomain_tasks(DeviceTree& t);
 
 }  // namespace sbh

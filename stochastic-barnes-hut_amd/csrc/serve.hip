@@ -17,12 +17,13 @@
 // are per query (Barnes-Hut without votes) or keyed on (seed, block of the
 // request's sorted order) (stochastic), so a result is identical alone and in
 // any mix, and equal to a one-shot sorted evaluation of the request.  Like a
-// one-shot launch, a request too small to fill the GPU on its own has its
-// walks split (one task per 64-query block and subtree of the walk's split
-// frontier, or root subdomain of the estimator, the parts added in task order
-// by a reduce pass after the work-queue kernel): a small request on a large
-// tree then spreads over the whole GPU instead of a few wavefronts walking
-// serially.
+// one-shot launch, a request has its walks split where that launch splits
+// them (one task per 64-query block and subtree of the Barnes-Hut walk's
+// split frontier when the request is too small to fill the GPU; per block and
+// range of root subdomains for the sorted estimator), the parts added in
+// subtree or subdomain order by a reduce pass after the work-queue kernel: a
+// small request on a large tree then spreads over the whole GPU instead of a
+// few wavefronts walking serially.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,6 +74,7 @@ struct ReqDesc {
   uint32_t nsplit;  // > 1: the walks are split into nsplit tasks
   void* partial;    // split requests: nsplit x nq parts (float, or float2 for the estimator)
   const WalkTask* wtasks;  // split Barnes-Hut requests: the frontier's task table
+  const uint4* stasks;     // split estimator requests: the tree's subdomain ranges
 };
 
 constexpr uint32_t kWhole = 0xFFFFFFFFu;
@@ -81,7 +83,7 @@ struct Task {
   uint32_t req;    // index into the launch's descriptor array
   uint32_t block;  // 64-query block within the request
   uint32_t sub;    // task of a split request (a walk task, or the estimator's
-                   // root subdomain), or kWhole
+                   // range of root subdomains), or kWhole
 };
 
 // One task for one lane: the whole walk (result to out[qi]) or one task's
@@ -94,16 +96,17 @@ __device__ __forceinline__ void serve_one(const ReqDesc& r, bool stochastic, uin
                                           float x, float y, float z, SamplerLds lds,
                                           SortedLds sorted_lds) {
   if (sub != kWhole) {
-    const size_t e = (size_t)sub * r.nq + slot;
     if (stochastic) {
       float bh = 0.0f;
+      const uint4 tk = load_invariant(r.stasks, sub);
       const LaneResult p = stochastic_estimate_sorted<D, K, false, true>(
-          r.tree, r.spl, r.key0, r.key1, block, live, x, y, z, sorted_lds, sub, sub + 1, &bh);
-      if (live) static_cast<float2*>(r.partial)[e] = make_float2(bh, p.value);
+          r.tree, r.spl, r.key0, r.key1, block, live, x, y, z, sorted_lds, tk.x, tk.y, &bh);
+      if (live)
+        static_cast<float2*>(r.partial)[(size_t)tk.z * r.nq + slot] = make_float2(bh, p.value);
     } else {
       const LaneResult p =
           bh_walk_task<D, K, false>(r.tree, r.wtasks, sub, r.beta, live, x, y, z);
-      if (live) static_cast<float*>(r.partial)[e] = p.value;
+      if (live) static_cast<float*>(r.partial)[(size_t)sub * r.nq + slot] = p.value;
     }
     return;
   }
@@ -605,10 +608,14 @@ void Server::launch(Launch& L, std::vector<Request*>& batch) {
     d.nsplit = r->nsplit;
     d.partial = r->nsplit > 1 ? dparts : nullptr;
     d.wtasks = nullptr;
+    d.stasks = nullptr;
     if (r->nsplit > 1 && r->method != SBH_STOCHASTIC) {
       const WalkSplit ws = bh_split(t, r->nq, 1, false);
       SBH_REQUIRE(ws.n == r->nsplit, "serving: split plan changed");
       d.wtasks = ws.tasks;
+    } else if (r->nsplit > 1) {
+      SBH_REQUIRE(t.n_sub_tasks == r->nsplit, "serving: split plan changed");
+      d.stasks = t.sub_tasks.get();
     }
     L.h_descs.push_back(d);
     const uint32_t nb = (r->nq + 63) / 64;

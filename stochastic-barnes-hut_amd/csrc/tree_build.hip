@@ -1313,6 +1313,52 @@ void derive_child_sources(DeviceTree& t) {
       t.dfsR.get(), t.bfsR.get(), t.stepR.get());
   SBH_HIP(hipGetLastError());
   derive_walk_tasks(t);
+  derive_subdomain_tasks(t);
+}
+
+void derive_subdomain_tasks(DeviceTree& t) {
+  t.n_sub_tasks = 0;
+  if (t.level_begin.size() < 3) {
+    t.sub_tasks.alloc(1);
+    return;
+  }
+  const uint32_t s0 = t.level_begin[1], nsub = t.level_begin[2] - s0;
+  if (nsub < 2 || nsub > 64) {
+    t.sub_tasks.alloc(1);
+    return;
+  }
+  std::vector<uint32_t> nch(nsub), st(nsub), en(nsub);
+  hipStream_t s = t.stream;
+  SBH_HIP(hipMemcpyAsync(nch.data(), t.n_nchild.get() + s0, nsub * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipMemcpyAsync(st.data(), t.n_start.get() + s0, nsub * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipMemcpyAsync(en.data(), t.n_end.get() + s0, nsub * 4, hipMemcpyDeviceToHost, s));
+  SBH_HIP(hipStreamSynchronize(s));
+  // a range starts at every internal subdomain but the first
+  std::vector<uint4> r;
+  std::vector<uint32_t> w;
+  for (uint32_t i = 0; i < nsub; ++i) {
+    if (nch[i] == 0) continue;
+    if (r.empty()) {
+      r.push_back(make_uint4(0u, 0u, 0u, 0u));
+    } else {
+      r.back().y = i;
+      r.push_back(make_uint4(i, 0u, (uint32_t)r.size(), 0u));
+    }
+    w.push_back(en[i] - st[i]);
+  }
+  t.sub_tasks.alloc(1);
+  if (r.size() < 2) return;
+  r.back().y = nsub;
+  std::vector<uint32_t> rank(r.size());
+  for (uint32_t i = 0; i < rank.size(); ++i) rank[i] = i;
+  std::stable_sort(rank.begin(), rank.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
+  std::vector<uint4> tasks;
+  for (uint32_t i : rank) tasks.push_back(r[i]);
+  t.sub_tasks.alloc(tasks.size());
+  SBH_HIP(hipMemcpyAsync(t.sub_tasks.get(), tasks.data(), tasks.size() * sizeof(uint4),
+                         hipMemcpyHostToDevice, s));
+  SBH_HIP(hipStreamSynchronize(s));
+  t.n_sub_tasks = (uint32_t)tasks.size();
 }
 
 void derive_walk_tasks(DeviceTree& t) {

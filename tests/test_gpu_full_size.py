@@ -10,7 +10,9 @@ N = Q = 2^20, octree k = 2): properties that do not depend on the size.
 * Barnes-Hut per query is independent of the query order (Hilbert-sorted and
   unsorted launches agree bit for bit);
 * the estimator is a function of (seed, query group): the same call twice
-  gives the same bits, another seed gives a different sample.
+  gives the same bits, another seed gives a different sample;
+* the estimator's default launch, split over ranges of root subdomains,
+  agrees with its whole-walk launch to fp32 rounding.
 """
 import numpy as np
 import pytest
@@ -78,3 +80,17 @@ def test_full_size_estimator_is_a_function_of_the_seed(sbh_gpu, headline):
     c, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, q, 16, seed=8)
     np.testing.assert_array_equal(a, b)
     assert np.mean(a != c) > 0.5
+
+
+def test_full_size_estimator_split_matches_whole_walk(sbh_gpu, headline):
+    """The default sorted launch (tasks per 64-query block and range of root
+    subdomains, heaviest ranges first) against the whole-walk launch: the
+    same sample paths and terms, the parts added in subdomain order, so
+    equal to fp32 rounding."""
+    *_, q, g, _ = headline
+    a, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, q, 32, seed=5, sort_queries=True)
+    b, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, q, 32, seed=5, sort_queries=True,
+                                                              no_split=True)
+    err = np.abs(a.astype(np.float64) - b) / np.abs(b.astype(np.float64))
+    print(f"[case] split vs whole: mean {err.mean():.2e} max {err.max():.2e}", flush=True)
+    assert err.mean() <= 1e-6 and err.max() <= 1e-4

@@ -56,8 +56,9 @@ def make_requests(trees, count, seed, max_q=4000):
 
 
 def one_shot(sbh_gpu, trees, r, no_split=False):
-    """The request as one sorted launch (its walks split into tasks below
-    2^18 queries unless no_split), with term counts: (values, counts)."""
+    """The request as one sorted launch (its walks split into tasks as the
+    one-shot launch splits them, unless no_split), with term counts:
+    (values, counts)."""
     kern, tree, *_ = trees[r["ti"]]
     if r["method"] == "barnes_hut":
         fn = getattr(sbh_gpu, "eval_barnes_hut_" + kern)
@@ -134,14 +135,19 @@ def test_admission_by_budget_and_cancel(sbh_gpu, trees):
     whole = probe.stats()["peak_bytes"]
     assert q * (dim + 1) * 4 < whole < q * (dim + 12) * 4
     del probe
-    # split over the root's children, a request also holds its parts: a
-    # {Barnes-Hut, control-variate} float pair per query and child
+    # split over ranges of the root's children (one internal child each), a
+    # request also holds its parts: a {Barnes-Hut, control-variate} float
+    # pair per query and range
     probe = sbh_gpu.Server(64 << 20, 1)
     probe.add_tree(tree)
     probe.submit(0, "stochastic", kern, pts, samples_per_subdomain=1).wait(60.0)
     per_request = probe.stats()["peak_bytes"]
-    nsub = int(tree.nodes()["nchild"][0])
-    assert 0 <= per_request - whole - nsub * q * 8 < 512
+    nodes = tree.nodes()
+    nch = np.asarray(nodes["nchild"])
+    subs = np.arange(nodes["first_child"][0], nodes["first_child"][0] + nch[0])
+    nranges = int(np.count_nonzero(nch[subs]))
+    assert nranges >= 2
+    assert 0 <= per_request - whole - nranges * q * 8 < 512
     del probe
     srv = sbh_gpu.Server(2 * per_request, 1)
     tid = srv.add_tree(tree)

@@ -2,10 +2,11 @@
 walk and the sorted-sample estimator, with and without term counts) against
 the oracle on the same seeded inputs.
 
-Launches of fewer than 2^18 queries normally split each walk into tasks;
-`no_split=True` (SBH_EVAL_NO_SPLIT) makes small launches run the
-whole-walk kernels, and one case of 2^18 + 77 queries reaches them by size
-alone.  Bars:
+Barnes-Hut launches of fewer than 2^18 queries, and estimator launches of any
+size on trees with two or more internal root subdomains, normally split each
+walk into tasks; `no_split=True` (SBH_EVAL_NO_SPLIT) makes them run the
+whole-walk kernels, and one case of 2^18 + 77 queries reaches the Barnes-Hut
+one by size alone.  Bars:
 
   * Barnes-Hut per query, fast terms: within 1e-5 of
     oracle.barnes_hut(vote_group=1) (include/barnes_hut.h:23-78), counts
@@ -169,8 +170,10 @@ def test_whole_walk_kernels_sorted_launch(sbh_gpu, kernel, dim, k):
 
 def test_whole_walk_kernels_by_size(sbh_gpu):
     """2^18 + 77 queries on a 4096-point octree (k = 2): large enough that
-    the default launch takes the whole-walk kernels without any flag, in
-    input order and in Hilbert order."""
+    the default Barnes-Hut launch takes the whole-walk kernel without any
+    flag, in input order and in Hilbert order; the estimator's default
+    launch (split over ranges of root subdomains) agrees with its whole-walk
+    launch to fp32 rounding."""
     pts, nrm, m, q, lo, hi = _inputs(4096, (1 << 18) + 77, 3, 171, "gravity")
     g, o = _trees(sbh_gpu, pts, nrm, m, lo, hi, 2, 1.0)
     a = check_bh(sbh_gpu, g, o, q, "gravity", 2.0, "2^18+77")
@@ -183,9 +186,9 @@ def test_whole_walk_kernels_by_size(sbh_gpu):
     order = launch_order(pts, m, lo, hi, q)
     s = check_stochastic(sbh_gpu, g, o, q, "gravity", 2, 19, "2^18+77 spl2 sorted", order=order,
                          sort_queries=True)
-    t, _ = sbh_gpu.multi_level_prefix_control_variate_gravity(g, q, 2, seed=19,
-                                                              sort_queries=True, no_split=True)
-    np.testing.assert_array_equal(s, t)
+    t = check_stochastic(sbh_gpu, g, o, q, "gravity", 2, 19, "2^18+77 spl2 sorted whole",
+                         order=order, sort_queries=True, no_split=True)
+    assert rel_err(s, t, "gravity").max() <= TOL
 
 
 def test_plummer_headline_shape_small(sbh_gpu):

@@ -89,16 +89,17 @@ def test_noalloc_refusals(setup):
     tree, _ = trees[0]
     dev = torch.device("cuda:0")
     # a sorted launch whose order and sort work space exceed the fixed scratch
-    nq = 2_700_000
+    nq = 10_500_000
     qd = torch.rand((3, nq), dtype=torch.float32, device=dev)
     out = torch.empty(nq, dtype=torch.float32, device=dev)
     with pytest.raises(C.SbhError, match="scratch") as e:
         tree.eval_device_noalloc(C.BARNES_HUT, "gravity", qd.data_ptr(), nq, out.data_ptr(),
                                  beta=2.0, flags=C.EVAL_SORT_QUERIES)
     assert e.value.code == C.ERR_INVALID
-    # a small launch split over the 64 root subdomains with counts: its
-    # partial sums (16 bytes per query and subdomain) exceed the array
-    nq2 = 70001
+    # a launch split over the 64 root subdomains (all internal) with counts:
+    # its partial sums (16 bytes per query and subdomain range) exceed the
+    # array
+    nq2 = 270001
     q2 = torch.rand((3, nq2), dtype=torch.float32, device=dev)
     o2 = torch.empty(nq2, dtype=torch.float32, device=dev)
     c2 = torch.empty(2 * nq2, dtype=torch.int32, device=dev)
