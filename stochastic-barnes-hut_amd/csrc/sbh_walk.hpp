@@ -910,6 +910,24 @@ __device__ __forceinline__ LaneResult stochastic_estimate_sorted(const TreeView&
                                           r0sum - term<D, K, EXACT>(SC, SN, alpha, alpha_k, x, y, z));
     if constexpr (WORK) wterms += live ? (int32_t)(pt0 + nd0 + 1) : 0;
     float sub_total = 0.0f;
+    // When every child of S steps (past its single-child chain) to a leaf,
+    // every sample's path ends at level 1 and adds the level-0 weight alone,
+    // whichever point it draws: the samples need not be drawn.  (Small
+    // outlying subdomains; the sum is the per-sample loop's, term for term.)
+    {
+      uint32_t inner = 0;
+      if (lane < nch0) inner = (bfsR[stepR[f0 + lane].z].l.w & kLeafBit) ? 0u : 1u;
+      if (__ballot(inner) == 0) {
+        const float w = live ? 0.0f + wd0 : 0.0f;
+        for (int s = 0; s < spl; ++s) sub_total += w;
+        if constexpr (REF) {
+          cnt += live ? spl * (int32_t)pt0 : 0;
+          cnt_nodes += live ? spl * (int32_t)nd0 : 0;
+        }
+        mlcv += sub_total / (float)spl;
+        continue;
+      }
+    }
     // the chunk size and key format of this subdomain (its point indices
     // must fit 23 bits for packed keys)
     const bool packed = kCh > 256 && sstart + nS <= kPackedPointLimit;
