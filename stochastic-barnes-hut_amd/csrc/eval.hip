@@ -158,7 +158,8 @@ __global__ __launch_bounds__(kThreads) void k_bh(TreeView t, float beta,
   const uint32_t qi = live ? (order ? order[slot] : slot) : 0;
   float x, y, z;
   load_query(qx, qy, qz, qi, D, x, y, z);
-  const LaneResult r = bh_walk<D, K, GROUP, EXACT, COUNT>(t, beta, live, x, y, z);
+  const LaneResult r =
+      bh_walk<D, K, GROUP, EXACT, COUNT, GROUP == 1 && !EXACT>(t, beta, live, x, y, z);
   if (live) {
     out[qi] = r.value;
     if (COUNT && counts) {
@@ -173,9 +174,9 @@ __global__ __launch_bounds__(kThreads) void k_bh(TreeView t, float beta,
 }
 
 // Barnes-Hut per query for launches too small to fill the GPU: the walk is
-// split over a frontier of disjoint subtrees (blockIdx.y = task, tasks in
-// depth-first order), each task walking one subtree for 64 queries, and the
-// partial sums are added in task order afterwards.  A query whose opening
+// split over a frontier of disjoint subtrees (blockIdx.y = task, heaviest
+// first), each task walking one subtree for 64 queries, and the partial sums
+// are added in depth-first order afterwards.  A query whose opening
 // test accepts an ancestor of a subtree takes that ancestor's moment (in the
 // first task below it) and nothing else, as the whole walk would.  Fast terms
 // only: the association of the sum changes.  With COUNT the tasks also count
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void k_bh_split(TreeView t,
   load_query(qx, qy, qz, qi, D, x, y, z);
   const LaneResult r = bh_walk_task<D, K, COUNT>(t, tasks, blockIdx.y, beta, live, x, y, z);
   if (live) {
-    const size_t e = (size_t)blockIdx.y * nq + slot;
+    const size_t e = (size_t)load_invariant(&tasks[blockIdx.y].part, 0) * nq + slot;
     partial[e] = r.value;
     if constexpr (COUNT) pcnt[e] = make_int2(r.nodes, r.points);
   }
@@ -911,17 +912,22 @@ void run_brute_f64(const DeviceTree& t, int kernel, const float* q, uint32_t nq,
   });
 }
 
+// Split Barnes-Hut launches: wavefronts wanted (task count x 64-query blocks)
+// and the most partial sums (task count x queries, 64 MiB of floats).
+constexpr uint32_t kSplitWaves = 16u * 1024u;
+constexpr size_t kSplitParts = size_t(1) << 24;
+
 WalkSplit bh_split(const DeviceTree& t, uint32_t nq, int group, bool exact) {
   // fewer than four wavefronts per SIMD (1024 SIMDs): split the walk over the
-  // shallowest frontier that makes kSplitWaves wavefronts, else the deepest
-  // one (at most kMaxWalkTasks subtrees).  Counted launches split the same way
-  // (the values must not depend on it).
-  constexpr uint32_t kSplitWaves = 16u * 1024u;
+  // smallest frontier that makes kSplitWaves wavefronts, else the largest one
+  // whose partial sums stay within kSplitParts values.  Counted launches split
+  // the same way (the values must not depend on it).
   WalkSplit ws;
   if (exact || group != 1 || t.is_double || t.walk_frontiers.empty() || nq >= 4u * 1024u * 64u)
     return ws;
   const uint32_t waves = nblk(nq, kWave);
   for (const auto& f : t.walk_frontiers) {
+    if (ws.n > 1 && (size_t)f.second * nq > kSplitParts) break;
     ws.n = f.second;
     ws.tasks = t.walk_tasks.get() + f.first;
     if ((size_t)f.second * waves >= kSplitWaves) break;

@@ -25,12 +25,15 @@ struct alignas(32) WalkRecord {
 // that covers the tree: the subtree at depth-first positions [v0, vstop) and
 // the depth-first positions of its nanc ancestors, root first.  Bit d of
 // `first` is set when this is the first task (in depth-first order) below
-// anc[d]: a query that accepts anc[d] takes that moment there.
-constexpr int kMaxWalkAnc = 4;
-constexpr uint32_t kMaxWalkTasks = 64;
-struct alignas(32) WalkTask {
-  uint32_t v0, vstop, nanc, first;
+// anc[d]: a query that accepts anc[d] takes that moment there.  `part` is
+// the task's depth-first rank: its slot among the walk's partial sums, which
+// are added in that order (tasks are stored, and launched, heaviest first).
+constexpr int kMaxWalkAnc = 8;
+constexpr uint32_t kMaxWalkTasks = 256;
+struct alignas(64) WalkTask {
+  uint32_t v0, vstop, nanc, first, part;
   uint32_t anc[kMaxWalkAnc];
+  uint32_t pad[3];
 };
 
 // include/kernel.h:24-28 (bound kernels) and :61-83 (unbound: Helmholtz,

@@ -105,13 +105,100 @@ __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
 // correctly rounded operations, so every lane opens exactly the nodes the
 // reference walk opens.
 // ---------------------------------------------------------------------------
+// A wavefront whose live queries are spread over a region that holds, at the
+// tree's mean density, more than kLaneSpread of its points (points x
+// (spread / root diameter)^D; spread = the diagonal of the queries' bounding
+// box) walks with every lane on its own path (bh_walk_lanes): the shared walk
+// would visit the union of the lanes' walks, which such lanes share little
+// of.
+constexpr float kLaneSpread = 512.0f;
+template <int D>
+__device__ __forceinline__ bool lanes_spread_out(const TreeView& t, bool live, float x, float y,
+                                                 float z) {
+  constexpr float kInf = __builtin_huge_valf();
+  float lo[3] = {live ? x : kInf, live ? y : kInf, live ? z : kInf};
+  float hi[3] = {live ? x : -kInf, live ? y : -kInf, live ? z : -kInf};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  }
+  if (!(lo[0] <= hi[0])) return false;  // no live lane
+  float d2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) d2 += (hi[a] - lo[a]) * (hi[a] - lo[a]);
+  const uint4 rl = load_invariant(t.nodeL, 0);
+  if (rl.w & kLeafBit) return false;
+  const float n = (float)(load_invariant(t.n_end, 0) - load_invariant(t.n_start, 0));
+  const float r = __builtin_sqrtf(d2) / __uint_as_float(rl.x);
+  const float frac = (D == 3) ? r * r * r : r * r;
+  return uniform_u32(n * frac > kLaneSpread ? 1u : 0u) != 0;
+}
+
+// The per-query fast walk with every lane on its own path: the same rope
+// walk, the same opening decisions and the same terms in the same
+// (depth-first) order as bh_walk<D, K, 1, false>, so the same values; each
+// lane loads its own records (vector loads).  A wavefront then takes as many
+// steps as its longest walk, where the shared walk takes as many as the union
+// of its lanes' walks: far fewer when the lanes' queries are spread out
+// (sparse queries over a deep tree).
+// v0 is per lane; acc0, cnt0, nodes0: the sums a walk begun in the shared
+// mode carries over (it switches to this one part-way, see bh_walk).
+template <int D, int K, bool COUNT = true>
+__device__ __forceinline__ LaneResult bh_walk_lanes(const TreeView& t, float beta, bool live,
+                                                    float x, float y, float z, uint32_t v0 = 0,
+                                                    uint32_t vstop = kEnd, float acc0 = 0.0f,
+                                                    int32_t cnt0 = 0, int32_t nodes0 = 0) {
+  const WalkRecord* __restrict__ rec = t.dfsR;
+  const float4* __restrict__ nodeN = t.dfsN;
+  const float beta_hi = beta * 1.000002f, beta_lo = beta * 0.999998f;
+  float acc = acc0;
+  int32_t cnt = cnt0, cnt_nodes = nodes0;
+  uint32_t v = live ? v0 : vstop;
+  while (v != vstop) {
+    const WalkRecord R = rec[v];
+    const float4 c = R.c;
+    const uint4 l = R.l;
+    const bool leaf = (int32_t)l.w < 0;
+    const uint32_t np = leaf ? (l.w & ~kLeafBit) : 0u;
+    const Sep s = separation<D>(c, x, y, z);
+    const float dist = fast_dist(s);
+    bool far = true;
+    if (!leaf) {
+      // the shared walk's opening test (fast ratio, exact where undecided)
+      float r = dist * __builtin_amdgcn_rcpf(__uint_as_float(l.x));
+      r = s.d2 >= 1e-30f ? r : __builtin_nanf("");
+      far = r >= beta_hi;
+      if (!far && !(r <= beta_lo)) far = opens_far_exact<D>(c, __uint_as_float(l.x), beta, x, y, z);
+    }
+    if (l.w > (kLeafBit | 1u)) {  // a leaf of several points
+      for (uint32_t j = l.y; j < l.y + np; ++j)
+        acc = accum<D, K, false>(acc, t.pts[j], t.nrm[j], t.alpha, t.alpha_k, x, y, z);
+    } else if (far) {
+      acc = kernel_fast_acc<D, K>(acc, c, nodeN[v], t.alpha_k, s, dist);
+    }
+    if constexpr (COUNT) {
+      cnt += far ? (int32_t)np : 0;
+      cnt_nodes += (far && !leaf) ? 1 : 0;
+    }
+    v = far ? l.z : l.y;
+  }
+  return {acc, cnt, cnt_nodes};
+}
+
 // COUNT: also count the terms (the counting mode).  [v0, vstop): the walk
 // starts at depth-first position v0 and ends where the ropes reach vstop (the
 // whole tree by default; a subtree's position and its rope walk the subtree).
-template <int D, int K, int GROUP, bool EXACT, bool COUNT = true>
+// HYBRID (per query, fast terms): the walk takes bh_walk_lanes instead when
+// lanes_spread_out; same values either way.
+template <int D, int K, int GROUP, bool EXACT, bool COUNT = true, bool HYBRID = false>
 __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, bool live, float x,
                                               float y, float z, uint32_t v0 = 0,
                                               uint32_t vstop = kEnd) {
+  static_assert(!HYBRID || (GROUP == 1 && !EXACT), "the lane walk is per query, fast terms");
   // Lane sets (live, active, far, reject, accept) are lane masks in scalar
   // registers, combined with scalar operations and read per lane with
   // lane_in (see LaneMask).
@@ -125,6 +212,10 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   LaneMask active = live_m;
   uint32_t resume = kEnd;
   uint32_t v = v0;
+  if constexpr (HYBRID) {
+    if (lanes_spread_out<D>(t, live, x, y, z))
+      return bh_walk_lanes<D, K, COUNT>(t, beta, live, x, y, z, v0, vstop);
+  }
   WalkRecord R = *reinterpret_cast<const WalkRecord*>(rec + (v0 << 5));
   while (v != vstop) {
     v = uniform_u32(v);
@@ -207,8 +298,9 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
 // subtree for this lane's query.  A query whose opening test accepts an
 // ancestor of the subtree takes the moment of the topmost ancestor it accepts
 // in the first task below that ancestor, and nothing else, as the whole walk
-// would.  .value is the task's part of the sum; the parts are added in task
-// (depth-first) order (sum_bh_parts).  Fast terms only.
+// would.  .value is the task's part of the sum, for slot tasks[k].part; the
+// parts are added in slot (depth-first) order (sum_bh_parts).  Fast terms
+// only.
 template <int D, int K, bool COUNT>
 __device__ __forceinline__ LaneResult bh_walk_task(const TreeView& t,
                                                    const WalkTask* __restrict__ tasks,
@@ -237,13 +329,13 @@ __device__ __forceinline__ LaneResult bh_walk_task(const TreeView& t,
     }
   }
   const LaneResult r =
-      bh_walk<D, K, 1, false, COUNT>(t, beta, live && !taken, x, y, z, w.v0, w.vstop);
+      bh_walk<D, K, 1, false, COUNT, true>(t, beta, live && !taken, x, y, z, w.v0, w.vstop);
   if (taken) return {acc, 0, nodes};
   return r;
 }
 
-// The parts of a split walk for launch slot `slot`, added in task order:
-// partial[k * nq + slot] is task k's part.
+// The parts of a split walk for launch slot `slot`, added in depth-first
+// order: partial[k * nq + slot] is the part of the task whose .part is k.
 __device__ __forceinline__ float sum_bh_parts(const float* __restrict__ partial, uint32_t nsplit,
                                               uint32_t nq, uint32_t slot) {
   // eight loads in flight at a time; the additions keep task order

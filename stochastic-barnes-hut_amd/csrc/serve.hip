@@ -106,7 +106,9 @@ __device__ __forceinline__ void serve_one(const ReqDesc& r, bool stochastic, uin
     } else {
       const LaneResult p =
           bh_walk_task<D, K, false>(r.tree, r.wtasks, sub, r.beta, live, x, y, z);
-      if (live) static_cast<float*>(r.partial)[(size_t)sub * r.nq + slot] = p.value;
+      if (live)
+        static_cast<float*>(r.partial)[(size_t)load_invariant(&r.wtasks[sub].part, 0) * r.nq +
+                                       slot] = p.value;
     }
     return;
   }
@@ -120,7 +122,7 @@ __device__ __forceinline__ void serve_one(const ReqDesc& r, bool stochastic, uin
                                          lds, false)
             .value;
   else
-    v = bh_walk<D, K, 1, false, false>(r.tree, r.beta, live, x, y, z).value;
+    v = bh_walk<D, K, 1, false, false, true>(r.tree, r.beta, live, x, y, z).value;
   if (live) r.out[qi] = v;
 }
 

@@ -1362,21 +1362,25 @@ void derive_subdomain_tasks(DeviceTree& t) {
 }
 
 void derive_walk_tasks(DeviceTree& t) {
-  // Frontier at depth L: the nodes at depth L and the leaves above it, in
-  // depth-first order; a frontier's size never falls as L grows (every
-  // internal node has a child).  Frontiers of 2 to kMaxWalkTasks subtrees at
-  // depths 1 .. kMaxWalkAnc are kept, shallowest first.
+  // Balanced frontiers: starting from the root's children, the heaviest
+  // subtree (most nodes) that has children, lies above depth kMaxWalkAnc and
+  // leaves the frontier within kMaxWalkTasks tasks is replaced by its
+  // children, until none is left.  The frontier is kept each time its size
+  // first reaches a power of two (4, 8, ...), and at the end, smallest
+  // first.  A frontier's tasks are stored heaviest first, each with its
+  // depth-first rank as the slot of its partial sum.
   t.walk_frontiers.clear();
   const int levels = (int)t.level_begin.size() - 1;
-  int top = 0;  // deepest level a kept frontier can reach
-  while (top + 1 < levels && top + 1 <= kMaxWalkAnc &&
-         t.level_begin[top + 2] - t.level_begin[top + 1] <= kMaxWalkTasks)
+  // nodes of depths 0 .. top on the host (at most kWalkHostNodes of them)
+  constexpr uint32_t kWalkHostNodes = 1u << 20;
+  int top = 0;
+  while (top + 1 < levels && top + 1 <= kMaxWalkAnc && t.level_begin[top + 2] <= kWalkHostNodes)
     ++top;
   if (top == 0) {
     t.walk_tasks.alloc(1);
     return;
   }
-  const uint32_t m = t.level_begin[top + 1];  // nodes of levels 0 .. top
+  const uint32_t m = t.level_begin[top + 1];
   std::vector<uint32_t> nch(m), first(m), dfs(m), size(m);
   hipStream_t s = t.stream;
   SBH_HIP(hipMemcpyAsync(nch.data(), t.n_nchild.get(), m * 4, hipMemcpyDeviceToHost, s));
@@ -1384,39 +1388,78 @@ void derive_walk_tasks(DeviceTree& t) {
   SBH_HIP(hipMemcpyAsync(dfs.data(), t.n_dfs.get(), m * 4, hipMemcpyDeviceToHost, s));
   SBH_HIP(hipMemcpyAsync(size.data(), t.n_size.get(), m * 4, hipMemcpyDeviceToHost, s));
   SBH_HIP(hipStreamSynchronize(s));
+  struct Sub {
+    uint32_t v;                   // breadth-first node id
+    uint32_t depth;               // = number of ancestors
+    uint32_t path[kMaxWalkAnc];   // ancestors' breadth-first ids, root first
+  };
+  std::vector<Sub> front;
+  for (uint32_t c = first[0]; c < first[0] + nch[0]; ++c) {
+    Sub u{};
+    u.v = c;
+    u.depth = 1;
+    u.path[0] = 0;
+    front.push_back(u);
+  }
   std::vector<WalkTask> all;
-  for (int L = 1; L <= top; ++L) {
-    std::vector<WalkTask> f;
-    uint32_t path[kMaxWalkAnc];
-    // depth-first over the levels above L (children in order, so the tasks
-    // come out in depth-first order)
-    auto visit = [&](auto&& self, uint32_t v, int depth) -> void {
-      if (depth == L || nch[v] == 0) {
-        WalkTask w{};
-        w.v0 = dfs[v];
-        const uint32_t rope = dfs[v] + size[v];
-        w.vstop = rope < t.num_nodes ? rope : kEnd;
-        w.nanc = (uint32_t)depth;
-        for (int d = 0; d < depth; ++d) w.anc[d] = dfs[path[d]];
-        f.push_back(w);
-        return;
-      }
-      path[depth] = v;
-      for (uint32_t c = first[v]; c < first[v] + nch[v]; ++c) self(self, c, depth + 1);
-    };
-    visit(visit, 0, 0);
-    if (f.size() > kMaxWalkTasks) break;
-    if (f.size() < 2) continue;
+  size_t kept_size = 0;
+  auto keep = [&]() {
+    if (front.size() < 2 || front.size() == kept_size) return;
+    kept_size = front.size();
+    std::vector<Sub> f = front;
+    std::sort(f.begin(), f.end(), [&](const Sub& a, const Sub& b) { return dfs[a.v] < dfs[b.v]; });
+    std::vector<WalkTask> w(f.size());
     for (size_t k = 0; k < f.size(); ++k) {
-      for (uint32_t d = 0; d < f[k].nanc; ++d) {
-        const bool below_prev = k > 0 && f[k - 1].nanc > d && f[k - 1].anc[d] == f[k].anc[d];
-        if (!below_prev) f[k].first |= 1u << d;
+      WalkTask& x = w[k];
+      x = WalkTask{};
+      x.v0 = dfs[f[k].v];
+      const uint32_t rope = dfs[f[k].v] + size[f[k].v];
+      x.vstop = rope < t.num_nodes ? rope : kEnd;
+      x.nanc = f[k].depth;
+      x.part = (uint32_t)k;
+      for (uint32_t d = 0; d < f[k].depth; ++d) {
+        x.anc[d] = dfs[f[k].path[d]];
+        // tasks below one ancestor are consecutive in depth-first order
+        const bool below_prev = k > 0 && f[k - 1].depth > d && f[k - 1].path[d] == f[k].path[d];
+        if (!below_prev) x.first |= 1u << d;
       }
     }
-    t.walk_frontiers.emplace_back((uint32_t)all.size(), (uint32_t)f.size());
-    all.insert(all.end(), f.begin(), f.end());
+    // heaviest first (launch order); ties in depth-first order
+    std::stable_sort(w.begin(), w.end(), [&](const WalkTask& a, const WalkTask& b) {
+      const uint32_t sa = (a.vstop == kEnd ? t.num_nodes : a.vstop) - a.v0;
+      const uint32_t sb = (b.vstop == kEnd ? t.num_nodes : b.vstop) - b.v0;
+      return sa > sb;
+    });
+    t.walk_frontiers.emplace_back((uint32_t)all.size(), (uint32_t)w.size());
+    all.insert(all.end(), w.begin(), w.end());
+  };
+  for (;;) {
+    if (front.size() >= 4 && (front.size() & (front.size() - 1)) == 0) keep();
+    size_t best = front.size();
+    for (size_t k = 0; k < front.size(); ++k) {
+      const Sub& u = front[k];
+      if (u.depth >= (uint32_t)top || nch[u.v] == 0 ||
+          front.size() - 1 + nch[u.v] > kMaxWalkTasks)
+        continue;
+      if (best == front.size() || size[u.v] > size[front[best].v]) best = k;
+    }
+    if (best == front.size()) break;
+    // a power of two passed over is kept before the frontier grows past it
+    const size_t grown = front.size() - 1 + nch[front[best].v];
+    for (size_t p = 4; p <= kMaxWalkTasks; p <<= 1)
+      if (front.size() < p && grown > p) keep();
+    const Sub u = front[best];
+    front.erase(front.begin() + best);
+    for (uint32_t c = first[u.v]; c < first[u.v] + nch[u.v]; ++c) {
+      Sub w = u;
+      w.v = c;
+      w.path[u.depth] = u.v;
+      w.depth = u.depth + 1;
+      front.push_back(w);
+    }
   }
-  t.walk_tasks.alloc(all.size());
+  keep();
+  t.walk_tasks.alloc(all.empty() ? 1 : all.size());
   if (!all.empty()) {
     SBH_HIP(hipMemcpyAsync(t.walk_tasks.get(), all.data(), all.size() * sizeof(WalkTask),
                            hipMemcpyHostToDevice, s));
