@@ -101,16 +101,17 @@ typedef struct sbh_tree sbh_tree;
                                       results do not depend on the query order */
 #define SBH_EVAL_NO_SPLIT 256      /* Barnes-Hut (vote_group 1) / stochastic (shared
                                       paths), fast terms: run the whole-walk kernels
-                                      at any launch size.  By default a launch of
-                                      fewer than 2^18 queries splits each walk into
-                                      tasks (Barnes-Hut: up to 64 disjoint subtrees
-                                      that cover the tree, a frontier chosen by the
-                                      launch size; the estimator: the root's
-                                      subdomains), partial sums added in depth-first
-                                      task order, which visits the same terms but
-                                      rounds the sum differently; with this flag the
-                                      values are bit-identical to those of a large
-                                      launch and of the serving engine */
+                                      at any launch size.  By default Barnes-Hut
+                                      launches of fewer than 2^18 queries split each
+                                      walk into tasks over up to 256 disjoint subtrees
+                                      that cover the tree (a balanced frontier chosen
+                                      by the launch size), and estimator launches on
+                                      trees with two or more internal root subdomains
+                                      split into ranges of subdomains at any size;
+                                      partial sums are added in depth-first task
+                                      order, which visits the same terms but rounds
+                                      the sum differently.  With this flag the values
+                                      are those of the whole walks */
 #define SBH_EVAL_WORK_COUNTS 512   /* stochastic, fast shared-path estimator on trees of
                                       at most 64 children per node, with counts: counts
                                       holds 2q values describing the work the launch
@@ -302,9 +303,8 @@ int sbh_tree_load(const char* path, sbh_tree** out);
  * keys paths on (seed, 64-query block of the request's sorted order): a
  * request's values are identical alone and in any mix, and bit-identical to
  * those of sbh_eval with SBH_EVAL_SORT_QUERIES plus the request's flags: a
- * request of fewer than 2^18 queries has its walks split into tasks as a
- * one-shot launch of that size does, unless params.flags holds
- * SBH_EVAL_NO_SPLIT.  Terms use the fast kernels; params.flags may hold only
+ * request has its walks split into tasks as a one-shot launch of its size
+ * does, unless params.flags holds SBH_EVAL_NO_SPLIT.  Terms use the fast kernels; params.flags may hold only
  * SBH_EVAL_SORT_QUERIES and SBH_EVAL_NO_SPLIT, anything else is refused with
  * SBH_ERR_INVALID.
  *
