@@ -16,7 +16,12 @@ inline TreeView view_of(const DeviceTree& t) {
                   t.dfsN.get(), t.n_start.get(), t.n_end.get(), t.n_skip.get(), t.pts.get(),
                   t.nrm.get(), t.dfsR.get(), t.bfsR.get(), t.stepR.get(), t.kidC.get(),
                   t.kidN.get(), t.alpha,
-                  t.alpha * 1.4426950408889634f};
+                  t.alpha * 1.4426950408889634f,
+                  t.dens.get(),
+                  {t.lo[0], t.lo[1], t.lo[2]},
+                  {t.dens_inv[0], t.dens_inv[1], t.dens_inv[2]},
+                  t.dens_diag,
+                  t.dens_g};
 }
 
 uint32_t brute_split_count(uint32_t n, uint32_t nq);

@@ -165,6 +165,12 @@ struct DeviceTree {
   // the root has 2 to 64 subdomains, two or more of them internal.
   DevBuf<uint4> sub_tasks;
   uint32_t n_sub_tasks = 0;
+  // Point counts of a grid of dens_g^dim cells over the root box (derived):
+  // the walks' local density estimate (lanes_spread_out).
+  DevBuf<uint32_t> dens;
+  uint32_t dens_g = 0;
+  float dens_inv[3] = {0, 0, 0};
+  float dens_diag = 0.0f;
 
   // Double build (the reference's USE_DOUBLE configuration): points, cell
   // bounds, diameters and moments in float64 instead of the float arrays
@@ -191,7 +197,7 @@ struct DeviceTree {
            nodeN.bytes() + nodeL.bytes() + n_m64.bytes() + n_skip.bytes() + n_size.bytes() +
            n_dfs.bytes() + dfsC.bytes() + dfsN.bytes() + dfsL.bytes() + kidC.bytes() +
            kidN.bytes() + dfsR.bytes() + bfsR.bytes() + stepR.bytes() + walk_tasks.bytes() +
-           sub_tasks.bytes() +
+           sub_tasks.bytes() + dens.bytes() +
            pts64.bytes() + nrm64.bytes() + n_lo64.bytes() + n_hi64.bytes() +
            n_diam64.bytes() + nodeC64.bytes() + nodeN64.bytes() + dfsR64.bytes() +
            dfsN64.bytes();
@@ -220,5 +226,7 @@ void derive_walk_tasks(DeviceTree& t);
 // The split sorted estimator's tasks (sub_tasks), from host copies of the
 // root's subdomains; blocks on t.stream.
 void derive_subdomain_tasks(DeviceTree& t);
+// The point-count grid over the root box (dens and its geometry).
+void derive_density_grid(DeviceTree& t);
 
 }  // namespace sbh

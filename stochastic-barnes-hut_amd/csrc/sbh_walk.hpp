@@ -30,6 +30,12 @@ struct TreeView {
   const float4* kidN;
   float alpha;    // smooth-distance exponent
   float alpha_k;  // alpha * log2(e), for exp2
+  // point counts of a grid of dg^dim cells over the root box: the cell of
+  // (x, y, z) is ((x - dlo) * dinv) per axis, clamped; ddiag = a cell's diagonal
+  const uint32_t* dens;
+  float dlo[3], dinv[3];
+  float ddiag;
+  uint32_t dg;
 };
 
 struct LaneResult {
@@ -106,36 +112,46 @@ __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
 // reference walk opens.
 // ---------------------------------------------------------------------------
 // A wavefront whose live queries are spread over a region that holds, at the
-// tree's mean density, more than kLaneSpread of its points (points x
-// (spread / root diameter)^D; spread = the diagonal of the queries' bounding
-// box) walks with every lane on its own path (bh_walk_lanes): the shared walk
-// would visit the union of the lanes' walks, which such lanes share little
-// of.
+// density of the grid cells its queries lie in (their mean point count), more
+// than kLaneSpread of the tree's points (mean count x (spread / cell
+// diagonal)^D; spread = the diagonal of the queries' bounding box) walks with
+// every lane on its own path (bh_walk_lanes): the shared walk would visit the
+// union of the lanes' walks, which such lanes share little of.
 constexpr float kLaneSpread = 512.0f;
 template <int D>
 __device__ __forceinline__ bool lanes_spread_out(const TreeView& t, bool live, float x, float y,
                                                  float z) {
   constexpr float kInf = __builtin_huge_valf();
-  float lo[3] = {live ? x : kInf, live ? y : kInf, live ? z : kInf};
-  float hi[3] = {live ? x : -kInf, live ? y : -kInf, live ? z : -kInf};
+  const float q[3] = {x, y, z};
+  float lo[3], hi[3];
+  uint32_t cell = 0;
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
+  for (int a = D - 1; a >= 0; --a) {
+    lo[a] = live ? q[a] : kInf;
+    hi[a] = live ? q[a] : -kInf;
+    float c = (q[a] - t.dlo[a]) * t.dinv[a];
+    c = fminf(fmaxf(c, 0.0f), (float)(t.dg - 1));  // NaN-free: clamps to [0, dg - 1]
+    cell = cell * t.dg + (uint32_t)c;
+  }
+  float count = live ? (float)t.dens[cell] : 0.0f;
+  float lanes = live ? 1.0f : 0.0f;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
       lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
       hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
     }
+    count += __shfl_xor(count, off, 64);
+    lanes += __shfl_xor(lanes, off, 64);
   }
-  if (!(lo[0] <= hi[0])) return false;  // no live lane
+  if (!(lanes > 0.0f)) return false;  // no live lane
   float d2 = 0.0f;
 #pragma unroll
   for (int a = 0; a < D; ++a) d2 += (hi[a] - lo[a]) * (hi[a] - lo[a]);
-  const uint4 rl = load_invariant(t.nodeL, 0);
-  if (rl.w & kLeafBit) return false;
-  const float n = (float)(load_invariant(t.n_end, 0) - load_invariant(t.n_start, 0));
-  const float r = __builtin_sqrtf(d2) / __uint_as_float(rl.x);
+  const float r = __builtin_sqrtf(d2) / t.ddiag;
   const float frac = (D == 3) ? r * r * r : r * r;
-  return uniform_u32(n * frac > kLaneSpread ? 1u : 0u) != 0;
+  return uniform_u32((count / lanes) * frac > kLaneSpread ? 1u : 0u) != 0;
 }
 
 // The per-query fast walk with every lane on its own path: the same rope

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -1300,6 +1301,7 @@ void build_impl(DeviceTree& t, const T* points, const T* normals, const T* masse
 }  // namespace
 
 void derive_child_sources(DeviceTree& t) {
+  derive_density_grid(t);
   const uint32_t nn = t.num_nodes;
   t.kidC.alloc(nn);
   t.kidN.alloc(nn);
@@ -1359,6 +1361,52 @@ void derive_subdomain_tasks(DeviceTree& t) {
                          hipMemcpyHostToDevice, s));
   SBH_HIP(hipStreamSynchronize(s));
   t.n_sub_tasks = (uint32_t)tasks.size();
+}
+
+// One point per thread: its cell of the density grid (cell = ((x - lo) *
+// inv) per axis, clamped), counted.
+__global__ void k_density(const float4* __restrict__ pts, uint32_t n, int dim, float lx, float ly,
+                          float lz, float ix, float iy, float iz, uint32_t g,
+                          uint32_t* __restrict__ dens) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pts[i];
+  const float lo[3] = {lx, ly, lz}, inv[3] = {ix, iy, iz}, q[3] = {p.x, p.y, p.z};
+  uint32_t cell = 0;
+  for (int a = dim - 1; a >= 0; --a) {
+    float c = (q[a] - lo[a]) * inv[a];
+    c = fminf(fmaxf(c, 0.0f), (float)(g - 1));
+    cell = cell * g + (uint32_t)c;
+  }
+  atomicAdd(&dens[cell], 1u);
+}
+
+__global__ void k_zero_dens(uint32_t* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+void derive_density_grid(DeviceTree& t) {
+  // 32^3 cells in 3D, 128^2 in 2D (about 2^15 counts)
+  const uint32_t g = t.dim == 3 ? 32u : 128u;
+  const uint32_t cells = t.dim == 3 ? g * g * g : g * g;
+  t.dens.alloc(cells);
+  t.dens_g = g;
+  float d2 = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    const float w = a < t.dim ? t.hi[a] - t.lo[a] : 0.0f;
+    t.dens_inv[a] = w > 0.0f ? (float)g / w : 0.0f;
+    d2 += (w / (float)g) * (w / (float)g);
+  }
+  t.dens_diag = d2 > 0.0f ? std::sqrt(d2) : 1.0f;
+  k_zero_dens<<<blocks_for(cells, 256), 256, 0, t.stream>>>(t.dens.get(), cells);
+  SBH_HIP(hipGetLastError());
+  if (t.n > 0 && !t.is_double) {
+    k_density<<<blocks_for(t.n, 256), 256, 0, t.stream>>>(
+        t.pts.get(), t.n, t.dim, t.lo[0], t.lo[1], t.lo[2], t.dens_inv[0], t.dens_inv[1],
+        t.dens_inv[2], g, t.dens.get());
+    SBH_HIP(hipGetLastError());
+  }
 }
 
 void derive_walk_tasks(DeviceTree& t) {

@@ -486,13 +486,37 @@ def _walk_split_tasks(sbh_gpu, tree, kern, q):
     return max(ns, 1)
 
 
+def _spread_estimates(pts, lo, hi, q, order):
+    """Per 64-query wavefront of a launch in `order`: the points its queries'
+    bounding box would hold at the density of the 32^3-cell grid cells the
+    queries lie in (mean point count x (box diagonal / cell diagonal)^3),
+    the measure the walk compares with 512 to walk lane by lane."""
+    lo32, hi32 = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
+    g = 32
+    width = (hi32 - lo32).astype(np.float64)
+
+    def cells(x):
+        c = np.clip((x.astype(np.float32) - lo32) * (np.float32(g) / (hi32 - lo32)), 0, g - 1)
+        c = c.astype(np.int64)
+        return (c[:, 2] * g + c[:, 1]) * g + c[:, 0]
+
+    counts = np.bincount(cells(pts), minlength=g ** 3)
+    diag = np.linalg.norm(width / g)
+    est = []
+    for w in range(0, len(q), 64):
+        qq = q[order[w:w + 64]].astype(np.float32).astype(np.float64)
+        dens = counts[cells(qq)].mean()
+        est.append(dens * (np.linalg.norm(qq.max(0) - qq.min(0)) / diag) ** 3)
+    return np.asarray(est)
+
+
 def test_barnes_hut_spread_wavefronts(sbh_gpu):
     """Wavefronts whose queries are spread out walk with every lane on its own
     path, the others share one walk (the union of their lanes' walks): either
     way each query takes exactly the oracle's per-query walk
     (include/barnes_hut.h:23-78).  Sparse uniform queries over a concentrated
-    Plummer tree (a wavefront's queries span a region holding, at the tree's
-    mean density, far more than 512 of its points) beside a tight cluster of
+    Plummer tree (a wavefront's queries span a region holding, at the local
+    density, far more than 512 of its points) beside a tight cluster of
     queries (far fewer), in input order and Hilbert-sorted, split into tasks
     and as whole walks."""
     pts, nrm, m, _ = W.plummer(1 << 14, 1, seed_sources=31, seed_queries=32, rmax=10.0)
@@ -502,14 +526,9 @@ def test_barnes_hut_spread_wavefronts(sbh_gpu):
     tight = 0.01 * rng.standard_normal((700, 3))
     q = np.concatenate([sparse, tight])
     g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1)
-    diam0 = float(g.nodes()["diam"][0])
 
     def spread(order):
-        est = []
-        for w in range(0, len(q), 64):
-            qq = q[order[w:w + 64]].astype(np.float32).astype(np.float64)
-            est.append(len(pts) * (np.linalg.norm(qq.max(0) - qq.min(0)) / diam0) ** 3)
-        return np.asarray(est)
+        return _spread_estimates(pts, lo, hi, q, order)
 
     for order in (np.arange(len(q)), np.argsort(hilbert_keys(q, lo, hi), kind="stable")):
         est = spread(order)
