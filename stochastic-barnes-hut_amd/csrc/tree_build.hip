@@ -1363,22 +1363,36 @@ void derive_subdomain_tasks(DeviceTree& t) {
   t.n_sub_tasks = (uint32_t)tasks.size();
 }
 
-// One point per thread: its cell of the density grid (cell = ((x - lo) *
-// inv) per axis, clamped), counted.
+// The cells of the density grid (cell = ((x - lo) * inv) per axis, clamped)
+// of kDensRun consecutive points per thread, counted: points are in the
+// tree's leaf order, so a run of them mostly shares a cell and adds it once
+// (one atomic per change of cell, not per point: a dense cell takes few).
+constexpr uint32_t kDensRun = 64;
 __global__ void k_density(const float4* __restrict__ pts, uint32_t n, int dim, float lx, float ly,
                           float lz, float ix, float iy, float iz, uint32_t g,
                           uint32_t* __restrict__ dens) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 p = pts[i];
-  const float lo[3] = {lx, ly, lz}, inv[3] = {ix, iy, iz}, q[3] = {p.x, p.y, p.z};
-  uint32_t cell = 0;
-  for (int a = dim - 1; a >= 0; --a) {
-    float c = (q[a] - lo[a]) * inv[a];
-    c = fminf(fmaxf(c, 0.0f), (float)(g - 1));
-    cell = cell * g + (uint32_t)c;
+  const uint64_t i0 = (uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * kDensRun;
+  if (i0 >= n) return;
+  const uint32_t i1 = (uint32_t)(i0 + kDensRun < n ? i0 + kDensRun : n);
+  const float lo[3] = {lx, ly, lz}, inv[3] = {ix, iy, iz};
+  uint32_t run_cell = 0xFFFFFFFFu, run = 0;
+  for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
+    const float4 p = pts[i];
+    const float q[3] = {p.x, p.y, p.z};
+    uint32_t cell = 0;
+    for (int a = dim - 1; a >= 0; --a) {
+      float c = (q[a] - lo[a]) * inv[a];
+      c = fminf(fmaxf(c, 0.0f), (float)(g - 1));
+      cell = cell * g + (uint32_t)c;
+    }
+    if (cell != run_cell) {
+      if (run) atomicAdd(&dens[run_cell], run);
+      run_cell = cell;
+      run = 0;
+    }
+    ++run;
   }
-  atomicAdd(&dens[cell], 1u);
+  if (run) atomicAdd(&dens[run_cell], run);
 }
 
 __global__ void k_zero_dens(uint32_t* __restrict__ p, uint32_t n) {
@@ -1402,7 +1416,7 @@ void derive_density_grid(DeviceTree& t) {
   k_zero_dens<<<blocks_for(cells, 256), 256, 0, t.stream>>>(t.dens.get(), cells);
   SBH_HIP(hipGetLastError());
   if (t.n > 0 && !t.is_double) {
-    k_density<<<blocks_for(t.n, 256), 256, 0, t.stream>>>(
+    k_density<<<blocks_for(blocks_for(t.n, kDensRun), 256), 256, 0, t.stream>>>(
         t.pts.get(), t.n, t.dim, t.lo[0], t.lo[1], t.lo[2], t.dens_inv[0], t.dens_inv[1],
         t.dens_inv[2], g, t.dens.get());
     SBH_HIP(hipGetLastError());
@@ -1420,7 +1434,7 @@ void derive_walk_tasks(DeviceTree& t) {
   t.walk_frontiers.clear();
   const int levels = (int)t.level_begin.size() - 1;
   // nodes of depths 0 .. top on the host (at most kWalkHostNodes of them)
-  constexpr uint32_t kWalkHostNodes = 1u << 20;
+  constexpr uint32_t kWalkHostNodes = 1u << 18;
   int top = 0;
   while (top + 1 < levels && top + 1 <= kMaxWalkAnc && t.level_begin[top + 2] <= kWalkHostNodes)
     ++top;
