@@ -118,6 +118,7 @@ __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
 // every lane on its own path (bh_walk_lanes): the shared walk would visit the
 // union of the lanes' walks, which such lanes share little of.
 constexpr float kLaneSpread = 512.0f;
+constexpr uint32_t kLaneMinNodes = 1024;
 template <int D>
 __device__ __forceinline__ bool lanes_spread_out(const TreeView& t, bool live, float x, float y,
                                                  float z) {
@@ -210,10 +211,12 @@ __device__ __forceinline__ LaneResult bh_walk_lanes(const TreeView& t, float bet
 // whole tree by default; a subtree's position and its rope walk the subtree).
 // HYBRID (per query, fast terms): the walk takes bh_walk_lanes instead when
 // lanes_spread_out; same values either way.
+// (spread_check false: walk shared without asking, for walks known to be
+// short.)
 template <int D, int K, int GROUP, bool EXACT, bool COUNT = true, bool HYBRID = false>
 __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, bool live, float x,
                                               float y, float z, uint32_t v0 = 0,
-                                              uint32_t vstop = kEnd) {
+                                              uint32_t vstop = kEnd, bool spread_check = true) {
   static_assert(!HYBRID || (GROUP == 1 && !EXACT), "the lane walk is per query, fast terms");
   // Lane sets (live, active, far, reject, accept) are lane masks in scalar
   // registers, combined with scalar operations and read per lane with
@@ -229,7 +232,7 @@ __device__ __forceinline__ LaneResult bh_walk(const TreeView& t, float beta, boo
   uint32_t resume = kEnd;
   uint32_t v = v0;
   if constexpr (HYBRID) {
-    if (lanes_spread_out<D>(t, live, x, y, z))
+    if (spread_check && lanes_spread_out<D>(t, live, x, y, z))
       return bh_walk_lanes<D, K, COUNT>(t, beta, live, x, y, z, v0, vstop);
   }
   WalkRecord R = *reinterpret_cast<const WalkRecord*>(rec + (v0 << 5));
@@ -344,8 +347,10 @@ __device__ __forceinline__ LaneResult bh_walk_task(const TreeView& t,
       taken = taken || far != 0;
     }
   }
-  const LaneResult r =
-      bh_walk<D, K, 1, false, COUNT, true>(t, beta, live && !taken, x, y, z, w.v0, w.vstop);
+  // a subtree of fewer than kLaneMinNodes nodes is walked shared without
+  // measuring the queries' spread (its walks are short either way)
+  const LaneResult r = bh_walk<D, K, 1, false, COUNT, true>(
+      t, beta, live && !taken, x, y, z, w.v0, w.vstop, w.vstop - w.v0 >= kLaneMinNodes);
   if (taken) return {acc, 0, nodes};
   return r;
 }

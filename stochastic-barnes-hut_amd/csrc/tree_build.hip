@@ -1364,16 +1364,15 @@ void derive_subdomain_tasks(DeviceTree& t) {
 }
 
 // The cells of the density grid (cell = ((x - lo) * inv) per axis, clamped)
-// of kDensRun consecutive points per thread, counted: points are in the
-// tree's leaf order, so a run of them mostly shares a cell and adds it once
-// (one atomic per change of cell, not per point: a dense cell takes few).
-constexpr uint32_t kDensRun = 64;
-__global__ void k_density(const float4* __restrict__ pts, uint32_t n, int dim, float lx, float ly,
-                          float lz, float ix, float iy, float iz, uint32_t g,
+// of `per` consecutive points per thread, counted: points are in the tree's
+// leaf order, so a run of them mostly shares a cell and adds it once (one
+// atomic per change of cell, not per point: a dense cell takes few).
+__global__ void k_density(const float4* __restrict__ pts, uint32_t n, uint32_t per, int dim,
+                          float lx, float ly, float lz, float ix, float iy, float iz, uint32_t g,
                           uint32_t* __restrict__ dens) {
-  const uint64_t i0 = (uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * kDensRun;
+  const uint64_t i0 = (uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * per;
   if (i0 >= n) return;
-  const uint32_t i1 = (uint32_t)(i0 + kDensRun < n ? i0 + kDensRun : n);
+  const uint32_t i1 = (uint32_t)(i0 + per < n ? i0 + per : n);
   const float lo[3] = {lx, ly, lz}, inv[3] = {ix, iy, iz};
   uint32_t run_cell = 0xFFFFFFFFu, run = 0;
   for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
@@ -1416,8 +1415,10 @@ void derive_density_grid(DeviceTree& t) {
   k_zero_dens<<<blocks_for(cells, 256), 256, 0, t.stream>>>(t.dens.get(), cells);
   SBH_HIP(hipGetLastError());
   if (t.n > 0 && !t.is_double) {
-    k_density<<<blocks_for(blocks_for(t.n, kDensRun), 256), 256, 0, t.stream>>>(
-        t.pts.get(), t.n, t.dim, t.lo[0], t.lo[1], t.lo[2], t.dens_inv[0], t.dens_inv[1],
+    // runs of up to 64 points, fewer while that leaves under 2^14 threads
+    const uint32_t per = std::min<uint32_t>(64u, std::max<uint32_t>(1u, t.n >> 14));
+    k_density<<<blocks_for(blocks_for(t.n, per), 256), 256, 0, t.stream>>>(
+        t.pts.get(), t.n, per, t.dim, t.lo[0], t.lo[1], t.lo[2], t.dens_inv[0], t.dens_inv[1],
         t.dens_inv[2], g, t.dens.get());
     SBH_HIP(hipGetLastError());
   }

@@ -519,13 +519,17 @@ def test_barnes_hut_spread_wavefronts(sbh_gpu):
     density, far more than 512 of its points) beside a tight cluster of
     queries (far fewer), in input order and Hilbert-sorted, split into tasks
     and as whole walks."""
-    pts, nrm, m, _ = W.plummer(1 << 14, 1, seed_sources=31, seed_queries=32, rmax=10.0)
+    # 2^18 points: the split launches' subtrees hold over 1024 nodes on average,
+    # so their wavefronts measure the spread too
+    pts, nrm, m, _ = W.plummer(1 << 18, 1, seed_sources=31, seed_queries=32, rmax=10.0)
     lo, hi = W.bounds(pts)
     rng = np.random.default_rng(33)
     sparse = lo + (hi - lo) * rng.random((1500, 3))
     tight = 0.01 * rng.standard_normal((700, 3))
     q = np.concatenate([sparse, tight])
     g, o = make_pair(sbh_gpu, pts, nrm, m, lo, hi, 1)
+    ns = _walk_split_tasks(sbh_gpu, g, "gravity", q)
+    assert ns > 1 and len(g.nodes()["diam"]) / ns > 1024, ns
 
     def spread(order):
         return _spread_estimates(pts, lo, hi, q, order)
