@@ -24,6 +24,7 @@ import ctypes
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -366,8 +367,8 @@ def main():
                     help="profiling mode precision: the kernel of the 2^18 uniform-cube case")
     ap.add_argument("--detail", default=None,
                     help="where the full record (sweeps, variants, serving, CPU legs) is written "
-                         "(default gpurun_out/bench_detail.json; profiling modes write none "
-                         "unless given)")
+                         "(default sbh_bench_detail.json in the system's temporary directory; "
+                         "profiling modes write none unless given)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="with --profile-kernel: also run a plain read+write kernel over this "
                          "many MiB (a counter calibration kernel of known traffic)")
@@ -388,7 +389,7 @@ def main():
         profile(wl, args)
         return
     if args.detail is None:
-        args.detail = os.path.join(ROOT, "gpurun_out", "bench_detail.json")
+        args.detail = os.path.join(tempfile.gettempdir(), "sbh_bench_detail.json")
 
     head = measure(wl, args, (0.3, 0.4, 0.5, 0.6, 0.7, 1.0), args.spl,
                    headline_steps=args.steps, warmup=args.warmup, want_counts=True)
@@ -464,7 +465,7 @@ def main():
                      "poisson_queries_per_s": serving["poisson"]["queries_per_s"],
                      "p50_ms": serving["poisson"]["p50_ms"], "p99_ms": serving["poisson"]["p99_ms"],
                      "requests": serving["requests"]} if serving else None),
-        "detail": os.path.relpath(args.detail, ROOT),
+        "detail": args.detail,
     }
     detail = dict(line, headline=head, variants=variants, precision_variant=precision,
                   double_build=double_build, serving_detail=serving, cpu_baseline_full=cpu,
