@@ -117,11 +117,15 @@ __device__ __forceinline__ float traverse_prob(float cur_beta, float ratio) {
 // diagonal)^D; spread = the diagonal of the queries' bounding box) walks with
 // every lane on its own path (bh_walk_lanes): the shared walk would visit the
 // union of the lanes' walks, which such lanes share little of.
+// Split walks' tasks on subtrees of fewer than kLaneMinNodes nodes skip the
+// measurement (bh_walk_task): their walks are short either way.  A tree
+// without the grid (dens null) always walks shared.
 constexpr float kLaneSpread = 512.0f;
 constexpr uint32_t kLaneMinNodes = 1024;
 template <int D>
 __device__ __forceinline__ bool lanes_spread_out(const TreeView& t, bool live, float x, float y,
                                                  float z) {
+  if (t.dens == nullptr || t.dg == 0) return false;
   constexpr float kInf = __builtin_huge_valf();
   const float q[3] = {x, y, z};
   float lo[3], hi[3];
